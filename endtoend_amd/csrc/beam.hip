@@ -1,0 +1,1089 @@
+// beam.hip -- resident context and the plain full-beam CCCS path on gfx950.
+//
+// Per beam (replacing radconvolveTexture, CCCS/dosecalc-beam/nvbbRayConvolve_host.cu:376-725):
+//   1. terma_kernel      Siddon radiological path + poly-energetic TERMA per calc_bbox voxel
+//                        (DoseCalcAlgo/cudaSiddon.cu:117-308)
+//   2. crop_partial/final  BEV-rotated extent of non-zero TERMA, reduced on the GPU instead of the
+//                        reference's D2H copy + host triple loop (:417-499); 28 bytes come back
+//   3. host geometry     crop box + one REV grid per cone (findREV :75-155)
+//   4. cone_kernel       one workgroup per REV line, all cones in one launch: resample density and
+//                        TERMA along the line into LDS (cudaRaytrace, nvbbRayConvolve_device.cu:11-80)
+//                        then the collapsed-cone walk in both directions from LDS (RowConvolve
+//                        :87-218).  The REV grid is sized per cone; nothing is cleared per cone.
+//   5. pvcs_kernel       one pass over the PVCS voxels the crop box can reach, gathering every
+//                        cone's REV dose trilinearly and adding the sum once (REV2XYZdose
+//                        :222-261 per cone in the reference).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+namespace cccs {
+
+// ------------------------------------------------------------------------------------------------
+// buffers
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes) return CCCS_OK;
+    if (p) {
+        (void)hipDeviceSynchronize();  // queued kernels may still read the old buffer
+        (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    size_t want = std::max(need, bytes + bytes / 4);
+    if (hipMalloc(&p, want) != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return set_error(CCCS_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    bytes = want;
+    return CCCS_OK;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+int HostBuf::ensure(size_t need) {
+    if (need <= bytes) return CCCS_OK;
+    if (p) { (void)hipHostFree(p); p = nullptr; bytes = 0; }
+    size_t want = std::max(need, bytes + bytes / 4);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return set_error(CCCS_ENOMEM, "hipHostMalloc failed");
+    }
+    bytes = want;
+    return CCCS_OK;
+}
+void HostBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_terma", "bl_raytrace",
+                                            "bl_cone", "bl_rev2bev", "bl_unpack", "bl_compact"};
+
+// Sums one uint32 per thread over the workgroup (LDS tree) and stores it at part[block].
+__device__ __forceinline__ void block_count(uint32_t v, uint32_t* __restrict__ part, size_t block) {
+    __shared__ uint32_t s_cnt[256];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    s_cnt[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && t + w < nt) s_cnt[t] += s_cnt[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[block] = s_cnt[0];
+}
+
+__global__ void __launch_bounds__(256) count_fold_kernel(const uint32_t* __restrict__ part, size_t n,
+                                                         uint64_t* __restrict__ acc) {
+    __shared__ uint64_t s[256];
+    uint64_t v = 0;
+    for (size_t i = threadIdx.x; i < n; i += 256) v += part[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += s[0];
+}
+
+int prof_begin(cccs_ctx* c, int kind) {
+    Profiler& P = c->prof;
+    if (!P.on) return -1;
+    if (P.used + 2 > P.pool.size()) {
+        for (int i = 0; i < 64; i++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            P.pool.push_back(e);
+        }
+    }
+    ProfRec r;
+    r.kind = kind;
+    r.a = P.pool[P.used++];
+    r.b = P.pool[P.used++];
+    r.bytes = r.flops = 0;
+    r.items = 0;
+    (void)hipEventRecord(r.a, c->stream);
+    P.recs.push_back(r);
+    return (int)P.recs.size() - 1;
+}
+void prof_end(cccs_ctx* c, int token, double bytes, double flops, uint64_t items) {
+    if (token < 0) return;
+    ProfRec& r = c->prof.recs[token];
+    (void)hipEventRecord(r.b, c->stream);
+    r.bytes = bytes;
+    r.flops = flops;
+    r.items = items;
+}
+uint32_t* prof_counter(cccs_ctx* c, size_t blocks) {
+    Profiler& P = c->prof;
+    if (!P.on) return nullptr;
+    if (blocks > P.part_cap) {
+        // the previous array may still be read by queued kernels: wait before replacing it
+        (void)hipStreamSynchronize(c->stream);
+        if (P.d_part) (void)hipFree(P.d_part);
+        P.d_part = nullptr;
+        size_t cap = std::max(blocks, (size_t)1 << 16);
+        if (hipMalloc((void**)&P.d_part, cap * sizeof(uint32_t)) != hipSuccess) { P.part_cap = 0; return nullptr; }
+        P.part_cap = cap;
+    }
+    return P.d_part;
+}
+void prof_fold(cccs_ctx* c, int kind, size_t blocks) {
+    Profiler& P = c->prof;
+    if (!P.on || !P.d_part || !P.d_counts) return;
+    hipLaunchKernelGGL(count_fold_kernel, dim3(1), dim3(256), 0, c->stream, P.d_part, blocks, P.d_counts + kind);
+}
+
+BeamDev make_beam_dev(const cccs_beam& b) {
+    BeamDev d;
+    d.src = ld3(b.source);
+    d.dir_n = norm3(ld3(b.direction));
+    d.iso = ld3(b.isocenter);
+    d.sad = b.sad;
+    d.iso_dist = len3(d.iso - d.src);
+    d.fmx = (int)b.fmap_size[0];
+    d.fmy = (int)b.fmap_size[1];
+    d.bsx = b.beamlet_size[0];
+    d.bsy = b.beamlet_size[1];
+    d.beam_fwd = to_f(dm_beam_fwd(b.azimuth, b.zenith, b.coll));
+    return d;
+}
+
+int check_beam(const cccs_ctx* ctx, const cccs_beam& b) {
+    (void)ctx;
+    if (b.fmap_size[0] == 0 || b.fmap_size[1] == 0 || b.fmap_size[0] > 4096 || b.fmap_size[1] > 4096)
+        return set_error(CCCS_EINVAL, "fluence map dims must be in [1, 4096]");
+    if (!(b.beamlet_size[0] > 0.f) || !(b.beamlet_size[1] > 0.f)) return set_error(CCCS_EINVAL, "beamlet size must be > 0");
+    if (!(b.sad > 0.f)) return set_error(CCCS_EINVAL, "sad must be > 0");
+    return CCCS_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// kernels
+
+struct TermaArgs {
+    float* terma;
+    const float* fmap;
+    const float* dens;
+    VolDesc vd;
+    BeamDev beam;
+    SpecDev spec;
+    int bx, by, bz;          // calc_bbox size
+    int ox, oy, oz;          // calc_bbox start
+    int ss;                  // supersampling factor
+    float bhc;
+    int beamletnum;
+    uint32_t* counts;         // per-block Siddon step counts (profiling) or nullptr
+};
+
+// One lane per calc_bbox voxel (x fastest => coalesced store).  Each lane marches its own ray;
+// neighbouring lanes follow near-parallel rays, so the density taps stay in L1/L2.
+__global__ void __launch_bounds__(256) terma_kernel(TermaArgs a) {
+    const int X = blockIdx.x * 64 + threadIdx.x;
+    const int Y = blockIdx.y * 4 + threadIdx.y;
+    const int Z = blockIdx.z;
+    const bool in = (X < a.bx && Y < a.by && Z < a.bz);
+    uint32_t steps = 0;
+    float acc = 0.f;
+    if (!in) {
+    } else if (a.ss > 1) {
+        // supersampled: sub-voxel centres, summed in sub-voxel order then averaged
+        const int nsub = a.ss * a.ss * a.ss;
+        v3 subsz = mk3(a.vd.voxel.x / (float)a.ss, a.vd.voxel.y / (float)a.ss, a.vd.voxel.z / (float)a.ss);
+        acc = 0.f;
+        for (int s = 0; s < nsub; s++) {
+            int sx = s % a.ss, sy = (s / a.ss) % a.ss, sz = s / (a.ss * a.ss);
+            v3 p = mk3(a.vd.start.x + ((float)a.ox + X - 0.5f) * a.vd.voxel.x + (sx + 0.5f) * subsz.x,
+                       a.vd.start.y + ((float)a.oy + Y - 0.5f) * a.vd.voxel.y + (sy + 0.5f) * subsz.y,
+                       a.vd.start.z + ((float)a.oz + Z - 0.5f) * a.vd.voxel.z + (sz + 0.5f) * subsz.z);
+            int it = 0;
+            acc += point_terma(p, a.fmap, a.beam, a.spec, a.vd, a.dens, a.bhc, a.beamletnum, &it);
+            steps += (uint32_t)it;
+        }
+        acc /= (float)nsub;
+    } else {
+        v3 p = mk3(a.vd.start.x + ((float)a.ox + X) * a.vd.voxel.x, a.vd.start.y + ((float)a.oy + Y) * a.vd.voxel.y,
+                   a.vd.start.z + ((float)a.oz + Z) * a.vd.voxel.z);
+        int it = 0;
+        acc = point_terma(p, a.fmap, a.beam, a.spec, a.vd, a.dens, a.bhc, a.beamletnum, &it);
+        steps = (uint32_t)it;
+    }
+    if (in) a.terma[(size_t)X + (size_t)a.bx * ((size_t)Y + (size_t)a.by * Z)] = acc;
+    if (a.counts) block_count(steps, a.counts, blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z));
+}
+
+struct PathArgs {
+    const float* dens;
+    VolDesc vd;
+    v3 src;
+    const float* pts;
+    int n, maxseg;
+    int32_t* ijk;
+    float* mid;
+    float* len;
+    int32_t* counts;
+    float* rpl;
+};
+
+__global__ void __launch_bounds__(256) siddon_path_kernel(PathArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    int c = 0;
+    v3 p = mk3(a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2]);
+    float r = siddon_march(p, a.vd.start, a.vd.end, a.vd.voxel, a.src, a.dens, a.vd.nx, a.vd.ny, a.vd.nz,
+                           [&](float fx, float fy, float fz, float l) {
+                               if (c < a.maxseg) {
+                                   size_t o = (size_t)i * a.maxseg + c;
+                                   a.ijk[3 * o] = f2i_rd(fx + 0.5f);
+                                   a.ijk[3 * o + 1] = f2i_rd(fy + 0.5f);
+                                   a.ijk[3 * o + 2] = f2i_rd(fz + 0.5f);
+                                   a.mid[3 * o] = fx;
+                                   a.mid[3 * o + 1] = fy;
+                                   a.mid[3 * o + 2] = fz;
+                                   a.len[o] = l;
+                               }
+                               c++;
+                           },
+                           nullptr);
+    a.counts[i] = c;
+    a.rpl[i] = r;
+}
+
+// Crop box: min/max of the BEV-rotated bbox index of every voxel with non-zero TERMA.
+// Pass 1 writes one partial (min xyz, max xyz, count) per workgroup.
+constexpr int kCropBlocks = 1024;
+__global__ void __launch_bounds__(256) crop_partial_kernel(const float* __restrict__ terma, int bx, int by, int bz,
+                                                           m33 rot, float* __restrict__ part) {
+    __shared__ float s[7][256];
+    float mn0 = FLT_MAX, mn1 = FLT_MAX, mn2 = FLT_MAX, mx0 = -FLT_MAX, mx1 = -FLT_MAX, mx2 = -FLT_MAX, cnt = 0.f;
+    const size_t n = (size_t)bx * by * bz;
+    for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (size_t)gridDim.x * 256) {
+        if (terma[v] != 0.f) {
+            int x = (int)(v % bx);
+            int y = (int)((v / bx) % by);
+            int z = (int)(v / ((size_t)bx * by));
+            v3 b = rot(mk3((float)x, (float)y, (float)z));
+            mn0 = fminf(mn0, b.x); mn1 = fminf(mn1, b.y); mn2 = fminf(mn2, b.z);
+            mx0 = fmaxf(mx0, b.x); mx1 = fmaxf(mx1, b.y); mx2 = fmaxf(mx2, b.z);
+            cnt += 1.f;
+        }
+    }
+    const int t = threadIdx.x;
+    s[0][t] = mn0; s[1][t] = mn1; s[2][t] = mn2; s[3][t] = mx0; s[4][t] = mx1; s[5][t] = mx2; s[6][t] = cnt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int q = 0; q < 3; q++) s[q][t] = fminf(s[q][t], s[q][t + w]);
+            for (int q = 3; q < 6; q++) s[q][t] = fmaxf(s[q][t], s[q][t + w]);
+            s[6][t] += s[6][t + w];
+        }
+        __syncthreads();
+    }
+    if (t < 7) part[(size_t)blockIdx.x * 8 + t] = s[t][0];
+}
+
+__global__ void __launch_bounds__(256) crop_final_kernel(const float* __restrict__ part, int nblocks, float* __restrict__ out) {
+    __shared__ float s[7][256];
+    const int t = threadIdx.x;
+    float v[7] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f};
+    for (int b = t; b < nblocks; b += 256) {
+        for (int q = 0; q < 3; q++) v[q] = fminf(v[q], part[(size_t)b * 8 + q]);
+        for (int q = 3; q < 6; q++) v[q] = fmaxf(v[q], part[(size_t)b * 8 + q]);
+        v[6] += part[(size_t)b * 8 + 6];
+    }
+    for (int q = 0; q < 7; q++) s[q][t] = v[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int q = 0; q < 3; q++) s[q][t] = fminf(s[q][t], s[q][t + w]);
+            for (int q = 3; q < 6; q++) s[q][t] = fmaxf(s[q][t], s[q][t + w]);
+            s[6][t] += s[6][t + w];
+        }
+        __syncthreads();
+    }
+    if (t < 7) out[t] = s[t][0];
+}
+
+struct ConeArgs {
+    const float* dens;
+    const float* terma;
+    const float* tables;      // kernel | radii | inverse widths
+    const ConeDev* cones;
+    float* rev;               // REV dose output, all cones back to back
+    int ncones;
+    int ntheta, nradii, nphi;
+    float delr;
+    uint32_t* counts;         // per-line walk iteration counts (profiling) or nullptr
+    VolDesc vd;
+    m33 binv;                 // PVCS <- BEV rotation (index units)
+    v3 off;                   // (bevStart - start) / voxel
+    v3 bev_size;              // crop-box size (index units)
+    v3 bb0, bb1;              // calc_bbox bounds (PVCS index)
+    int bbx, bby, bbz;        // calc_bbox size
+};
+
+// Walks one direction along the line from destination i, accumulating the cumulative-kernel
+// differences weighted by TERMA (the body of RowConvolve's do/while loops).
+__device__ __forceinline__ float walk(const float* __restrict__ sD, const float* __restrict__ sT, int i, int step,
+                                      int n, const float* __restrict__ R, const float* __restrict__ IW,
+                                      const float* __restrict__ row, int nr, float delr, uint32_t& iters) {
+    float sum = 0.f, r = 0.f, last = 0.f;
+    int rs = 0;
+    float lo = 0.f, hi = R[0], klo = 0.f, khi = row[0], iw = IW[0];
+    const float rmax = R[nr - 1];
+    for (int k = i + step; k >= 0 && k < n; k += step) {
+        const float d = sD[k];
+        if (d < 0.f) break;  // left the crop-box / calc_bbox segment (TERMA is 0 beyond it)
+        iters++;
+        const float de = delr * d;
+        r += 0.5f * de;
+        if (r > rmax) break;
+        while (hi < r && rs < nr - 1) {
+            rs++;
+            lo = hi;
+            hi = R[rs];
+            klo = khi;
+            khi = row[rs];
+            iw = IW[rs];
+        }
+        const float cum = klo + ((r - lo) * iw) * (khi - klo);
+        sum += (cum - last) * sT[k];
+        last = cum;
+        r += 0.5f * de;
+        if (rs >= nr - 1) break;
+    }
+    return sum;
+}
+
+// One workgroup per REV line of any cone.  Phase 1 resamples density and TERMA at the line's
+// samples into LDS; phase 2 runs both walks for every valid sample from LDS and stores the
+// line's cone dose contiguously.
+__global__ void __launch_bounds__(256) cone_kernel(ConeArgs a) {
+    __shared__ float sD[kMaxLine];
+    __shared__ float sT[kMaxLine];
+    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    __shared__ int s_cone;
+    const uint64_t line = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = a.ncones - 1;
+        while (lo < hi) {  // last cone whose first line <= this line
+            int mid = (lo + hi + 1) >> 1;
+            if (a.cones[mid].line0 <= line) lo = mid; else hi = mid - 1;
+        }
+        s_cone = lo;
+    }
+    __syncthreads();
+    const ConeDev& c = a.cones[s_cone];
+    const int nr = a.nradii;
+    if (threadIdx.x < nr) {
+        const float* K = a.tables;
+        const float* R = a.tables + (size_t)a.ntheta * nr;
+        sR[threadIdx.x] = R[threadIdx.x];
+        sIW[threadIdx.x] = R[nr + threadIdx.x];
+        sF[threadIdx.x] = K[(size_t)c.rowf * nr + threadIdx.x];
+        sB[threadIdx.x] = K[(size_t)c.rowb * nr + threadIdx.x];
+    }
+    const uint64_t l = line - c.line0;
+    const int tY = (int)(l % (uint64_t)c.ny);
+    const int tZ = (int)(l / (uint64_t)c.ny);
+    const int n = c.nx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        float d = -1.f, t = 0.f;
+        v3 bev = c.kinv(mk3(c.rs.x + (float)tZ, c.rs.y + (float)i, c.rs.z + (float)tY));
+        if (bev.x >= 0.f && bev.y >= 0.f && bev.z >= 0.f && bev.x < a.bev_size.x && bev.y < a.bev_size.y &&
+            bev.z < a.bev_size.z) {
+            v3 g = a.binv(bev) + a.off;
+            if (g.x >= a.bb0.x && g.x < a.bb1.x && g.y >= a.bb0.y && g.y < a.bb1.y && g.z >= a.bb0.z && g.z < a.bb1.z) {
+                d = tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, g.x + 0.5f, g.y + 0.5f, g.z + 0.5f);
+                v3 tq = g - a.bb0;
+                t = tex3_border(a.terma, a.bbx, a.bby, a.bbz, tq.x + 0.5f, tq.y + 0.5f, tq.z + 0.5f);
+            }
+        }
+        sD[i] = d;
+        sT[i] = t;
+    }
+    __syncthreads();
+    float* out = a.rev + c.off + (size_t)n * ((size_t)tY + (size_t)c.ny * tZ);
+    const float nphi_f = (float)a.nphi;
+    uint32_t iters = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        float v = 0.f;
+        if (sD[i] >= 0.f) {
+            float s = walk(sD, sT, i, -1, n, sR, sIW, sF, nr, a.delr, iters);
+            s += walk(sD, sT, i, +1, n, sR, sIW, sB, nr, a.delr, iters);
+            v = s / nphi_f;
+        }
+        out[i] = v;
+    }
+    if (a.counts) block_count(iters, a.counts, blockIdx.x);
+}
+
+struct PvcsArgs {
+    const ConeDev* cones;
+    const float* rev;
+    float* dose;
+    int ncones;
+    int nx, ny;                 // full volume dims (x, y)
+    int x0, y0, z0, sx, sy, sz; // PVCS index box
+};
+
+// One lane per PVCS voxel of the reachable box; sums every cone's trilinear REV dose in a
+// register and adds it to the dose volume once.
+__global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
+    __shared__ ConeDev sc[kMaxRays];
+    const int tid = threadIdx.x + threadIdx.y * 64;
+    for (int q = tid; q < a.ncones; q += 256) sc[q] = a.cones[q];
+    __syncthreads();
+    const int X = blockIdx.x * 64 + threadIdx.x;
+    const int Y = blockIdx.y * 4 + threadIdx.y;
+    const int Z = blockIdx.z;
+    if (X >= a.sx || Y >= a.sy || Z >= a.sz) return;
+    const int gx = a.x0 + X, gy = a.y0 + Y, gz = a.z0 + Z;
+    const v3 g = mk3((float)gx, (float)gy, (float)gz);
+    float sum = 0.f;
+    for (int q = 0; q < a.ncones; q++) {
+        const ConeDev& c = sc[q];
+        v3 r = c.a(g) + c.t;
+        if (r.y >= (float)c.nx || r.z >= (float)c.ny || r.x >= (float)c.nz || r.x < 0.f || r.y < 0.f || r.z < 0.f) continue;
+        sum += tex3_border(a.rev + c.off, c.nx, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
+    }
+    if (sum != 0.f) a.dose[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] += sum;
+}
+
+__global__ void add_kernel(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] += src[i];
+}
+
+// ------------------------------------------------------------------------------------------------
+// host pipeline
+
+namespace {
+
+inline size_t bbox_voxels(const cccs_ctx* ctx) {
+    return (size_t)ctx->prob.bbox_size[0] * ctx->prob.bbox_size[1] * ctx->prob.bbox_size[2];
+}
+inline size_t vol_voxels(const cccs_ctx* ctx) {
+    return (size_t)ctx->prob.size[0] * ctx->prob.size[1] * ctx->prob.size[2];
+}
+
+TermaArgs terma_args(cccs_ctx* ctx, const cccs_beam& b, const float* d_fmap, float* d_terma, int beamletnum) {
+    TermaArgs a;
+    a.terma = d_terma;
+    a.fmap = d_fmap;
+    a.dens = ctx->d_dens;
+    a.vd = ctx->vol;
+    a.beam = make_beam_dev(b);
+    a.spec = ctx->spec;
+    a.bx = (int)ctx->prob.bbox_size[0]; a.by = (int)ctx->prob.bbox_size[1]; a.bz = (int)ctx->prob.bbox_size[2];
+    a.ox = (int)ctx->prob.bbox_start[0]; a.oy = (int)ctx->prob.bbox_start[1]; a.oz = (int)ctx->prob.bbox_start[2];
+    a.ss = std::max(1u, ctx->prob.ss_factor);
+    a.bhc = ctx->prob.beamhard_correct;
+    a.beamletnum = beamletnum;
+    a.counts = nullptr;
+    return a;
+}
+
+int launch_terma(cccs_ctx* ctx, const TermaArgs& a0) {
+    dim3 blk(64, 4, 1), grd((a0.bx + 63) / 64, (a0.by + 3) / 4, a0.bz);
+    const size_t blocks = (size_t)grd.x * grd.y * grd.z;
+    TermaArgs a = a0;
+    a.counts = prof_counter(ctx, blocks);
+    const double nbb = (double)a.bx * a.by * a.bz;
+    const double nct = (double)a.vd.nx * a.vd.ny * a.vd.nz;
+    int tok = prof_begin(ctx, K_TERMA);
+    hipLaunchKernelGGL(terma_kernel, grd, blk, 0, ctx->stream, a);
+    prof_end(ctx, tok, 4.0 * nct + 4.0 * nbb, 0.0, (uint64_t)nbb);
+    CCCS_HIP(hipGetLastError());
+    if (a.counts) prof_fold(ctx, K_TERMA, blocks);
+    return CCCS_OK;
+}
+
+// Crop box + REV grids (nvbbRayConvolve_host.cu:429-587), from the reduced extent.
+int beam_geometry(cccs_ctx* ctx, const cccs_beam& b, const float* red, BeamGeom& g) {
+    const cccs_problem& p = ctx->prob;
+    // the reference starts max at FLT_MIN and min at FLT_MAX (:436-437)
+    v3 bmin = mk3(std::min(red[0], FLT_MAX), std::min(red[1], FLT_MAX), std::min(red[2], FLT_MAX));
+    v3 bmax = mk3(std::max(red[3], FLT_MIN), std::max(red[4], FLT_MIN), std::max(red[5], FLT_MIN));
+    if (red[6] < 0.5f) return set_error(CCCS_EGEOM, "beam has no non-zero TERMA inside the calc_bbox");
+    bmin.x -= p.penumbra / p.rev_latspacing;
+    bmin.y -= p.penumbra / p.rev_longspacing;
+    bmin.z -= p.penumbra / p.rev_latspacing;
+    bmax.x += p.penumbra / p.rev_latspacing;
+    bmax.y += p.penumbra / p.rev_longspacing;
+    bmax.z += p.penumbra / p.rev_latspacing;
+    float sx = std::ceil(std::fabs(bmax.x - bmin.x)), sy = std::ceil(std::fabs(bmax.y - bmin.y)),
+          sz = std::ceil(std::fabs(bmax.z - bmin.z));
+    if (!(sx >= 1.f && sx < 1000.f && sy >= 1.f && sy < 1000.f && sz >= 1.f && sz < 1000.f))
+        return set_error(CCCS_EGEOM, "BEV crop box size out of range (0, 1000)");
+    g.bev_size[0] = (uint32_t)sx; g.bev_size[1] = (uint32_t)sy; g.bev_size[2] = (uint32_t)sz;
+    g.bev_min = bmin;
+    g.bev_max = bmax;
+    dm33 binv = dm_beam_inv(b.azimuth, b.zenith, b.coll);
+    v3 tmin = dm_apply_f(binv, bmin);
+    g.lim_min = mk3(p.start[0] + (tmin.x + (float)p.bbox_start[0]) * p.voxel[0],
+                    p.start[1] + (tmin.y + (float)p.bbox_start[1]) * p.voxel[1],
+                    p.start[2] + (tmin.z + (float)p.bbox_start[2]) * p.voxel[2]);
+    // REV grid per cone: bounding box of the rotated crop box (cm), sized in spacing units
+    v3 lim = mk3(sx * p.rev_latspacing, sy * p.rev_longspacing, sz * p.rev_latspacing);
+    g.nrays = ctx->nrays;
+    g.total_lines = 0;
+    g.total_rev = 0;
+    for (int r = 0; r < ctx->nrays; r++) {
+        float th = ctx->theta[r] * 3.14159265358979323846f / 180.f;
+        float ph = ctx->phi[r] * 3.14159265358979323846f / 180.f;
+        dm33 kf = dm_kernel_fwd(th, ph);
+        v3 mn = mk3(FLT_MAX, FLT_MAX, FLT_MAX), mx = mk3(FLT_MIN, FLT_MIN, FLT_MIN);
+        for (int c = 0; c < 8; c++) {
+            v3 in = mk3(lim.x * (c & 1), lim.y * ((c >> 1) & 1), lim.z * ((c >> 2) & 1));
+            v3 o = dm_apply_f(kf, in);
+            mn = mk3(std::min(mn.x, o.x), std::min(mn.y, o.y), std::min(mn.z, o.z));
+            if (o.x > mx.x) mx.x = o.x;
+            if (o.y > mx.y) mx.y = o.y;
+            if (o.z > mx.z) mx.z = o.z;
+        }
+        uint32_t n0 = (uint32_t)std::ceil(std::fabs(mx.y - mn.y) / p.rev_longspacing);
+        uint32_t n1 = (uint32_t)std::ceil(std::fabs(mx.z - mn.z) / p.rev_latspacing);
+        uint32_t n2 = (uint32_t)std::ceil(std::fabs(mx.x - mn.x) / p.rev_latspacing);
+        if (n0 > ctx->max_rev_beam[0] || n1 > ctx->max_rev_beam[1] || n2 > ctx->max_rev_beam[2])
+            return set_error(CCCS_EGEOM, "REV grid exceeds max_rev_size (ArraySizeError)");
+        if (n0 > (uint32_t)kMaxLine) return set_error(CCCS_EGEOM, "REV line longer than 1024 samples");
+        g.rev_min[r] = mn;
+        g.rev_size[r][0] = n0; g.rev_size[r][1] = n1; g.rev_size[r][2] = n2;
+        g.total_lines += (uint64_t)n1 * n2;
+        g.total_rev += (uint64_t)n0 * n1 * n2;
+    }
+    // PVCS index box reachable from the crop box dilated by 2 index units (trilinear support of
+    // a REV sample is within sqrt(3) of it), clamped to the calc_bbox
+    v3 off = mk3((g.lim_min.x - p.start[0]) / p.voxel[0], (g.lim_min.y - p.start[1]) / p.voxel[1],
+                 (g.lim_min.z - p.start[2]) / p.voxel[2]);
+    double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+    for (int c = 0; c < 8; c++) {
+        v3 cb = mk3((c & 1) ? sx + 2.f : -2.f, ((c >> 1) & 1) ? sy + 2.f : -2.f, ((c >> 2) & 1) ? sz + 2.f : -2.f);
+        v3 q = dm_apply_f(binv, cb) + off;
+        lo[0] = std::min(lo[0], (double)q.x); lo[1] = std::min(lo[1], (double)q.y); lo[2] = std::min(lo[2], (double)q.z);
+        hi[0] = std::max(hi[0], (double)q.x); hi[1] = std::max(hi[1], (double)q.y); hi[2] = std::max(hi[2], (double)q.z);
+    }
+    for (int k = 0; k < 3; k++) {
+        long l0 = (long)std::floor(lo[k]) - 1, h0 = (long)std::ceil(hi[k]) + 2;
+        long b0 = (long)p.bbox_start[k], b1 = b0 + (long)p.bbox_size[k];
+        g.pvcs_lo[k] = (int)std::max(b0, std::min(b1, l0));
+        g.pvcs_hi[k] = (int)std::max(b0, std::min(b1, h0));
+    }
+    return CCCS_OK;
+}
+
+int run_crop(cccs_ctx* ctx, const float* d_terma, const cccs_beam& b, float* red_host) {
+    CCCS_TRY(ctx->partial.ensure(sizeof(float) * (kCropBlocks * 8 + 8)));
+    CCCS_TRY(ctx->h_small.ensure(64));
+    float* part = ctx->partial.as<float>();
+    m33 rot = to_f(dm_beam_fwd(b.azimuth, b.zenith, b.coll));
+    const cccs_problem& p = ctx->prob;
+    size_t n = bbox_voxels(ctx);
+    int blocks = (int)std::min<size_t>(kCropBlocks, (n + 255) / 256);
+    int tok = prof_begin(ctx, K_CROP);
+    hipLaunchKernelGGL(crop_partial_kernel, dim3(blocks), dim3(256), 0, ctx->stream, d_terma, (int)p.bbox_size[0],
+                       (int)p.bbox_size[1], (int)p.bbox_size[2], rot, part);
+    hipLaunchKernelGGL(crop_final_kernel, dim3(1), dim3(256), 0, ctx->stream, part, blocks, part + kCropBlocks * 8);
+    prof_end(ctx, tok, 4.0 * (double)n, 0.0, n);
+    CCCS_HIP(hipGetLastError());
+    CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, part + kCropBlocks * 8, 7 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(red_host, ctx->h_small.p, 7 * sizeof(float));
+    return CCCS_OK;
+}
+
+// Everything after TERMA: geometry, cones, PVCS accumulation into d_dose.
+int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_dose, BeamGeom* gout) {
+    float red[7];
+    CCCS_TRY(run_crop(ctx, d_terma, b, red));
+    BeamGeom g;
+    CCCS_TRY(beam_geometry(ctx, b, red, g));
+    if (gout) *gout = g;
+    if (!d_dose) return CCCS_OK;
+    const cccs_problem& p = ctx->prob;
+    CCCS_TRY(ctx->rev.ensure(sizeof(float) * g.total_rev));
+    CCCS_TRY(ctx->cones.ensure(sizeof(ConeDev) * kMaxRays));
+    CCCS_TRY(ctx->h_cones.ensure(sizeof(ConeDev) * kMaxRays));
+    ConeDev* hc = ctx->h_cones.as<ConeDev>();
+    dm33 bfwd = dm_beam_fwd(b.azimuth, b.zenith, b.coll);
+    dm33 binv = dm_beam_inv(b.azimuth, b.zenith, b.coll);
+    v3 rv = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    v3 off = mk3((g.lim_min.x - p.start[0]) / p.voxel[0], (g.lim_min.y - p.start[1]) / p.voxel[1],
+                 (g.lim_min.z - p.start[2]) / p.voxel[2]);
+    uint64_t line0 = 0, roff = 0;
+    const int half = (int)p.ntheta / 2;
+    for (int r = 0; r < g.nrays; r++) {
+        float th = ctx->theta[r] * 3.14159265358979323846f / 180.f;
+        float ph = ctx->phi[r] * 3.14159265358979323846f / 180.f;
+        dm33 kf = dm_kernel_fwd(th, ph), ki = dm_kernel_inv(th, ph);
+        ConeDev& c = hc[r];
+        c.kinv = to_f(ki);
+        dm33 a = dm_mul(kf, bfwd);
+        c.a = to_f(a);
+        c.rs = mk3(g.rev_min[r].x / rv.x, g.rev_min[r].y / rv.y, g.rev_min[r].z / rv.z);
+        // REV coordinate of PVCS index 0: a * (-off) - rs
+        v3 t0 = dm_apply_f(a, mk3(-off.x, -off.y, -off.z));
+        c.t = mk3(t0.x - c.rs.x, t0.y - c.rs.y, t0.z - c.rs.z);
+        c.nx = (int)g.rev_size[r][0];
+        c.ny = (int)g.rev_size[r][1];
+        c.nz = (int)g.rev_size[r][2];
+        c.rowf = r % half;
+        c.rowb = (int)p.ntheta - 1 - (r % half);
+        c.line0 = line0;
+        c.off = roff;
+        line0 += (uint64_t)c.ny * c.nz;
+        roff += (uint64_t)c.nx * c.ny * c.nz;
+    }
+    CCCS_HIP(hipMemcpyAsync(ctx->cones.p, hc, sizeof(ConeDev) * g.nrays, hipMemcpyHostToDevice, ctx->stream));
+    ConeArgs ca;
+    ca.dens = ctx->d_dens;
+    ca.terma = d_terma;
+    ca.tables = ctx->d_tables;
+    ca.cones = ctx->cones.as<ConeDev>();
+    ca.rev = ctx->rev.as<float>();
+    ca.ncones = g.nrays;
+    ca.ntheta = (int)p.ntheta;
+    ca.nradii = (int)p.nradii;
+    ca.nphi = (int)p.nphi;
+    ca.delr = p.rev_longspacing;
+    ca.counts = nullptr;
+    ca.vd = ctx->vol;
+    ca.binv = to_f(binv);
+    ca.off = off;
+    ca.bev_size = mk3((float)g.bev_size[0], (float)g.bev_size[1], (float)g.bev_size[2]);
+    ca.bb0 = ldu3(p.bbox_start);
+    ca.bb1 = ca.bb0 + ldu3(p.bbox_size);
+    ca.bbx = (int)p.bbox_size[0]; ca.bby = (int)p.bbox_size[1]; ca.bbz = (int)p.bbox_size[2];
+    if (g.total_lines > 0x7fffffffULL) return set_error(CCCS_EGEOM, "too many REV lines");
+    ca.counts = prof_counter(ctx, g.total_lines);
+    int tok = prof_begin(ctx, K_CONE);
+    hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(256), 0, ctx->stream, ca);
+    prof_end(ctx, tok, 12.0 * (double)g.total_rev, 0.0, g.total_rev);
+    CCCS_HIP(hipGetLastError());
+    if (ca.counts) prof_fold(ctx, K_CONE, g.total_lines);
+    PvcsArgs pa;
+    pa.cones = ctx->cones.as<ConeDev>();
+    pa.rev = ctx->rev.as<float>();
+    pa.dose = d_dose;
+    pa.ncones = g.nrays;
+    pa.nx = (int)p.size[0];
+    pa.ny = (int)p.size[1];
+    pa.x0 = g.pvcs_lo[0]; pa.y0 = g.pvcs_lo[1]; pa.z0 = g.pvcs_lo[2];
+    pa.sx = g.pvcs_hi[0] - g.pvcs_lo[0]; pa.sy = g.pvcs_hi[1] - g.pvcs_lo[1]; pa.sz = g.pvcs_hi[2] - g.pvcs_lo[2];
+    if (pa.sx > 0 && pa.sy > 0 && pa.sz > 0) {
+        dim3 blk(64, 4, 1), grd((pa.sx + 63) / 64, (pa.sy + 3) / 4, pa.sz);
+        const double nbox = (double)pa.sx * pa.sy * pa.sz;
+        int tok = prof_begin(ctx, K_PVCS);
+        hipLaunchKernelGGL(pvcs_kernel, grd, blk, 0, ctx->stream, pa);
+        prof_end(ctx, tok, 4.0 * (double)g.total_rev + 8.0 * nbox, 0.0, (uint64_t)nbox);
+        CCCS_HIP(hipGetLastError());
+    }
+    return CCCS_OK;
+}
+
+int upload_fmap(cccs_ctx* ctx, const cccs_beam& b, const float* host_fmap, const float** d_out) {
+    size_t n = (size_t)b.fmap_size[0] * b.fmap_size[1];
+    CCCS_TRY(ctx->fmap.ensure(n * sizeof(float)));
+    // synchronous copy: the host buffer may be reused by the caller right after the call
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(ctx->fmap.p, host_fmap, n * sizeof(float), hipMemcpyHostToDevice));
+    *d_out = ctx->fmap.as<float>();
+    return CCCS_OK;
+}
+
+}  // namespace
+
+// Shared by the C-ABI entry points below and by the beamlet path.
+int beam_dose_one(cccs_ctx* ctx, const cccs_beam& b, const float* d_fmap, float* d_dose) {
+    CCCS_TRY(check_beam(ctx, b));
+    CCCS_TRY(ctx->terma.ensure(sizeof(float) * bbox_voxels(ctx)));
+    TermaArgs ta = terma_args(ctx, b, d_fmap, ctx->terma.as<float>(), -1);
+    CCCS_TRY(launch_terma(ctx, ta));
+    return run_cones(ctx, b, ctx->terma.as<float>(), d_dose, nullptr);
+}
+
+thread_local std::string g_last_error;
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+}  // namespace cccs
+
+using namespace cccs;
+
+// =================================================================================================
+// C-ABI
+
+extern "C" const char* cccs_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int cccs_ctx_create(const cccs_problem* p, int device, cccs_ctx** out) {
+    clear_error();
+    if (!p || !out) return set_error(CCCS_EINVAL, "null argument");
+    *out = nullptr;
+    if (!p->density || !p->kernel || !p->radii || !p->conv_theta_deg || !p->conv_phi_deg || !p->spec_fluence ||
+        !p->spec_energy || !p->spec_mu || !p->spec_mu_en)
+        return set_error(CCCS_EINVAL, "problem tables must be non-null");
+    if (p->ntheta < 2 || p->ntheta % 2 || p->nphi < 1 || p->nradii < 2 || p->nradii > (uint32_t)kMaxRadii)
+        return set_error(CCCS_EINVAL, "bad ntheta/nphi/nradii");
+    if (p->nphi * (p->ntheta / 2) > (uint32_t)kMaxRays) return set_error(CCCS_EINVAL, "more than 48 cone lines");
+    if (p->nspec < 1 || p->nspec > (uint32_t)kMaxSpec) return set_error(CCCS_EINVAL, "nspec must be in [1, 30]");
+    for (int k = 0; k < 3; k++) {
+        if (p->size[k] == 0 || p->bbox_size[k] == 0 || (uint64_t)p->bbox_start[k] + p->bbox_size[k] > p->size[k])
+            return set_error(CCCS_EINVAL, "calc_bbox must lie inside the volume");
+        if (!(p->voxel[k] > 0.f)) return set_error(CCCS_EINVAL, "voxel size must be > 0");
+    }
+    if (!(p->rev_latspacing > 0.f) || !(p->rev_longspacing > 0.f)) return set_error(CCCS_EINVAL, "REV spacing must be > 0");
+    for (uint32_t r = 1; r < p->nradii; r++)
+        if (!(p->radii[r] > p->radii[r - 1]) || !(p->radii[0] > 0.f)) return set_error(CCCS_EINVAL, "radii must increase");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return set_error(CCCS_EDEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return set_error(CCCS_EINVAL, "device ordinal out of range");
+    CCCS_HIP(hipSetDevice(device));
+    cccs_ctx* c = new cccs_ctx();
+    c->device = device;
+    c->prob = *p;
+    c->prob.density = nullptr; c->prob.kernel = nullptr; c->prob.radii = nullptr;
+    c->prob.conv_theta_deg = nullptr; c->prob.conv_phi_deg = nullptr;
+    c->prob.spec_fluence = c->prob.spec_energy = c->prob.spec_mu = c->prob.spec_mu_en = nullptr;
+    c->nrays = (int)(p->nphi * (p->ntheta / 2));
+    c->kernel.assign(p->kernel, p->kernel + (size_t)p->ntheta * p->nradii);
+    c->radii.assign(p->radii, p->radii + p->nradii);
+    c->theta.assign(p->conv_theta_deg, p->conv_theta_deg + c->nrays);
+    c->phi.assign(p->conv_phi_deg, p->conv_phi_deg + c->nrays);
+    c->vol.nx = (int)p->size[0]; c->vol.ny = (int)p->size[1]; c->vol.nz = (int)p->size[2];
+    c->vol.start = ld3(p->start);
+    c->vol.voxel = ld3(p->voxel);
+    c->vol.end = mk3(p->start[0] + (float)p->size[0] * p->voxel[0], p->start[1] + (float)p->size[1] * p->voxel[1],
+                     p->start[2] + (float)p->size[2] * p->voxel[2]);
+    c->spec.n = (int)p->nspec;
+    for (uint32_t e = 0; e < p->nspec; e++) {
+        c->spec.flu[e] = p->spec_fluence[e];
+        c->spec.en[e] = p->spec_energy[e];
+        c->spec.mu[e] = p->spec_mu[e];
+    }
+    // beam-path REV limit, derived as initCudaConstandTex does (nvbbRayConvolve_host.cu:248-258)
+    {
+        float min_vox = std::min(p->rev_latspacing, p->rev_longspacing);
+        v3 bb = ldu3(p->bbox_size);
+        unsigned diag = (unsigned)std::ceil(len3(bb) * p->voxel[0] / min_vox);
+        unsigned v = (unsigned)(diag + 2 * (0.0f / min_vox + p->penumbra / min_vox) + 100);
+        c->max_rev_beam[0] = c->max_rev_beam[1] = c->max_rev_beam[2] = v;
+    }
+    int st = CCCS_OK;
+    do {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { st = set_error(CCCS_EDEVICE, "stream"); break; }
+        size_t nv = (size_t)p->size[0] * p->size[1] * p->size[2];
+        if (hipMalloc((void**)&c->d_dens, nv * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "density"); break; }
+        if (hipMemcpy(c->d_dens, p->density, nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "density copy"); break; }
+        // tables: kernel | radii | 1/width | mean radii | 1/width of mean radii
+        const uint32_t nr = p->nradii;
+        std::vector<float> tab((size_t)p->ntheta * nr + 4 * nr);
+        std::copy(c->kernel.begin(), c->kernel.end(), tab.begin());
+        float* R = tab.data() + (size_t)p->ntheta * nr;
+        float* IW = R + nr;
+        float* MR = IW + nr;
+        float* MIW = MR + nr;
+        for (uint32_t r = 0; r < nr; r++) {
+            R[r] = p->radii[r];
+            IW[r] = 1.0f / (r == 0 ? p->radii[0] : (p->radii[r] - p->radii[r - 1]));
+            MR[r] = (float)(r == 0 ? 0.5 * p->radii[0] : 0.5 * (p->radii[r] + p->radii[r - 1]));
+        }
+        for (uint32_t r = 0; r < nr; r++) MIW[r] = 1.0f / (r == 0 ? MR[0] : (MR[r] - MR[r - 1]));
+        if (hipMalloc((void**)&c->d_tables, tab.size() * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "tables"); break; }
+        if (hipMemcpy(c->d_tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "tables copy"); break; }
+        c->resident = nv * sizeof(float) + tab.size() * sizeof(float);
+    } while (false);
+    if (st != CCCS_OK) {
+        cccs_ctx_destroy(c);
+        return st;
+    }
+    *out = c;
+    return CCCS_OK;
+}
+
+extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
+    if (!c) return CCCS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_dens) (void)hipFree(c->d_dens);
+    if (c->d_tables) (void)hipFree(c->d_tables);
+    c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
+    c->dose.release(); c->col.release();
+    c->h_small.release(); c->h_cones.release();
+    for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
+    if (c->prof.d_counts) (void)hipFree(c->prof.d_counts);
+    if (c->prof.d_part) (void)hipFree(c->prof.d_part);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CCCS_OK;
+}
+
+extern "C" int cccs_ctx_memory(const cccs_ctx* c, uint64_t* resident, uint64_t* scratch) {
+    if (!c) return CCCS_EINVAL;
+    if (resident) *resident = c->resident;
+    if (scratch)
+        *scratch = c->terma.bytes + c->fmap.bytes + c->partial.bytes + c->rev.bytes + c->cones.bytes + c->dose.bytes +
+                   c->col.bytes;
+    return CCCS_OK;
+}
+
+extern "C" int cccs_beam_dose_device(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, const float* d_fluence,
+                                     float* d_dose) {
+    clear_error();
+    if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || !d_dose || (!d_fluence && nbeams > 0))
+        return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    size_t fo = 0;
+    for (int i = 0; i < nbeams; i++) {
+        CCCS_TRY(beam_dose_one(ctx, beams[i], d_fluence + fo, d_dose));
+        fo += (size_t)beams[i].fmap_size[0] * beams[i].fmap_size[1];
+    }
+    return CCCS_OK;
+}
+
+extern "C" int cccs_beam_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, const float* const* fluence,
+                              float* dose) {
+    clear_error();
+    if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || !dose || (!fluence && nbeams > 0))
+        return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    size_t nv = vol_voxels(ctx);
+    CCCS_TRY(ctx->dose.ensure(nv * sizeof(float)));
+    float* d = ctx->dose.as<float>();
+    CCCS_HIP(hipMemsetAsync(d, 0, nv * sizeof(float), ctx->stream));
+    for (int i = 0; i < nbeams; i++) {
+        if (!fluence[i]) return set_error(CCCS_EINVAL, "null fluence map");
+        const float* dfm = nullptr;
+        CCCS_TRY(upload_fmap(ctx, beams[i], fluence[i], &dfm));
+        CCCS_TRY(beam_dose_one(ctx, beams[i], dfm, d));
+    }
+    std::vector<float> tmp(nv);
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(tmp.data(), d, nv * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nv; i++) dose[i] += tmp[i];
+    return CCCS_OK;
+}
+
+extern "C" int cccs_beam_terma(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int beamletnum,
+                               float* terma_bbox) {
+    clear_error();
+    if (!ctx || !beam || !fluence || !terma_bbox) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_TRY(check_beam(ctx, *beam));
+    const float* dfm = nullptr;
+    CCCS_TRY(upload_fmap(ctx, *beam, fluence, &dfm));
+    size_t n = bbox_voxels(ctx);
+    CCCS_TRY(ctx->terma.ensure(n * sizeof(float)));
+    TermaArgs ta = terma_args(ctx, *beam, dfm, ctx->terma.as<float>(), beamletnum);
+    CCCS_TRY(launch_terma(ctx, ta));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(terma_bbox, ctx->terma.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return CCCS_OK;
+}
+
+extern "C" int cccs_beam_geometry_of(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence,
+                                     cccs_beam_geometry* geom) {
+    clear_error();
+    if (!ctx || !beam || !fluence || !geom) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_TRY(check_beam(ctx, *beam));
+    const float* dfm = nullptr;
+    CCCS_TRY(upload_fmap(ctx, *beam, fluence, &dfm));
+    CCCS_TRY(ctx->terma.ensure(sizeof(float) * bbox_voxels(ctx)));
+    TermaArgs ta = terma_args(ctx, *beam, dfm, ctx->terma.as<float>(), -1);
+    CCCS_TRY(launch_terma(ctx, ta));
+    BeamGeom g;
+    CCCS_TRY(run_cones(ctx, *beam, ctx->terma.as<float>(), nullptr, &g));
+    std::memset(geom, 0, sizeof(*geom));
+    geom->lim_min[0] = g.lim_min.x; geom->lim_min[1] = g.lim_min.y; geom->lim_min[2] = g.lim_min.z;
+    for (int k = 0; k < 3; k++) geom->bev_size[k] = g.bev_size[k];
+    geom->bev_min[0] = g.bev_min.x; geom->bev_min[1] = g.bev_min.y; geom->bev_min[2] = g.bev_min.z;
+    geom->bev_max[0] = g.bev_max.x; geom->bev_max[1] = g.bev_max.y; geom->bev_max[2] = g.bev_max.z;
+    geom->nrays = g.nrays;
+    for (int r = 0; r < g.nrays; r++) {
+        geom->rev_min[r][0] = g.rev_min[r].x; geom->rev_min[r][1] = g.rev_min[r].y; geom->rev_min[r][2] = g.rev_min[r].z;
+        for (int k = 0; k < 3; k++) geom->rev_size[r][k] = g.rev_size[r][k];
+    }
+    return CCCS_OK;
+}
+
+extern "C" int cccs_siddon_paths(cccs_ctx* ctx, const float* source, const float* points, int npoints,
+                                 int max_segments, int32_t* ijk, float* mid, float* len, int32_t* counts, float* rpl) {
+    clear_error();
+    if (!ctx || !source || !points || npoints < 0 || max_segments < 0 || !ijk || !mid || !len || !counts || !rpl)
+        return set_error(CCCS_EINVAL, "null argument");
+    if (npoints == 0) return CCCS_OK;
+    CCCS_HIP(hipSetDevice(ctx->device));
+    const size_t ns = (size_t)npoints * std::max(1, max_segments);
+    DevBuf b;
+    size_t bytes = sizeof(float) * 3 * npoints + ns * (3 * sizeof(int32_t) + 3 * sizeof(float) + sizeof(float)) +
+                   (size_t)npoints * (sizeof(int32_t) + sizeof(float));
+    CCCS_TRY(b.ensure(bytes));
+    char* base = b.as<char>();
+    float* d_pts = (float*)base;
+    int32_t* d_ijk = (int32_t*)(d_pts + 3 * npoints);
+    float* d_mid = (float*)(d_ijk + 3 * ns);
+    float* d_len = d_mid + 3 * ns;
+    int32_t* d_cnt = (int32_t*)(d_len + ns);
+    float* d_rpl = (float*)(d_cnt + npoints);
+    int st = CCCS_OK;
+    do {
+        if (hipMemcpy(d_pts, points, sizeof(float) * 3 * npoints, hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "copy"); break; }
+        // slots past a point's segment count are returned as zeros
+        if (hipMemset(d_ijk, 0, bytes - sizeof(float) * 3 * npoints) != hipSuccess) { st = set_error(CCCS_EDEVICE, "memset"); break; }
+        PathArgs a;
+        a.dens = ctx->d_dens;
+        a.vd = ctx->vol;
+        a.src = ld3(source);
+        a.pts = d_pts;
+        a.n = npoints;
+        a.maxseg = max_segments;
+        a.ijk = d_ijk; a.mid = d_mid; a.len = d_len; a.counts = d_cnt; a.rpl = d_rpl;
+        hipLaunchKernelGGL(siddon_path_kernel, dim3((npoints + 255) / 256), dim3(256), 0, ctx->stream, a);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) { st = set_error(CCCS_EDEVICE, "siddon kernel"); break; }
+        if (max_segments > 0) {
+            if (hipMemcpy(ijk, d_ijk, ns * 3 * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(mid, d_mid, ns * 3 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(len, d_len, ns * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) { st = set_error(CCCS_EDEVICE, "copy back"); break; }
+        }
+        if (hipMemcpy(counts, d_cnt, npoints * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(rpl, d_rpl, npoints * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) { st = set_error(CCCS_EDEVICE, "copy back"); break; }
+    } while (false);
+    b.release();
+    return st;
+}
+
+// ---- memory helpers ----------------------------------------------------------------------------
+extern "C" int cccs_dev_alloc(cccs_ctx* ctx, size_t bytes, void** ptr) {
+    clear_error();
+    if (!ctx || !ptr) return set_error(CCCS_EINVAL, "null argument");
+    *ptr = nullptr;
+    CCCS_HIP(hipSetDevice(ctx->device));
+    if (hipMalloc(ptr, std::max<size_t>(bytes, 4)) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return set_error(CCCS_ENOMEM, "hipMalloc failed");
+    }
+    return CCCS_OK;
+}
+extern "C" int cccs_dev_free(cccs_ctx* ctx, void* ptr) {
+    if (!ctx) return CCCS_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ptr) CCCS_HIP(hipFree(ptr));
+    return CCCS_OK;
+}
+extern "C" int cccs_dev_upload(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    clear_error();
+    if (!ctx || !dst || !src) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return CCCS_OK;
+}
+extern "C" int cccs_dev_download(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    clear_error();
+    if (!ctx || !dst || !src) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CCCS_OK;
+}
+extern "C" int cccs_dev_zero(cccs_ctx* ctx, void* ptr, size_t bytes) {
+    clear_error();
+    if (!ctx || !ptr) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_HIP(hipMemsetAsync(ptr, 0, bytes, ctx->stream));
+    return CCCS_OK;
+}
+extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
+    clear_error();
+    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    return CCCS_OK;
+}
+
+// ---- profiling -----------------------------------------------------------------------------------
+// Algorithmic FP32 operations per counted unit: one line-walk iteration (dens*step, two half-step
+// adds, interval lerp, kernel difference times TERMA: 10 ops) and one Siddon plane crossing (three
+// validity tests, plane choice, segment length, midpoint fetch coordinate, 8-tap trilinear: 40 ops).
+static const double kFlopsPerWalkIter = 10.0;
+static const double kFlopsPerSiddonStep = 40.0;
+
+extern "C" int cccs_ctx_profile(cccs_ctx* ctx, int enable) {
+    clear_error();
+    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    Profiler& P = ctx->prof;
+    if (enable && !P.d_counts) {
+        CCCS_HIP(hipMalloc((void**)&P.d_counts, sizeof(uint64_t) * K_COUNT));
+        CCCS_HIP(hipMemset(P.d_counts, 0, sizeof(uint64_t) * K_COUNT));
+    }
+    P.on = enable != 0;
+    return CCCS_OK;
+}
+extern "C" int cccs_ctx_profile_reset(cccs_ctx* ctx) {
+    clear_error();
+    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->prof.recs.clear();
+    ctx->prof.used = 0;
+    if (ctx->prof.d_counts) CCCS_HIP(hipMemset(ctx->prof.d_counts, 0, sizeof(uint64_t) * K_COUNT));
+    return CCCS_OK;
+}
+extern "C" int cccs_ctx_profile_read(cccs_ctx* ctx, cccs_kernel_stats* out, int max_rows, int* nrows) {
+    clear_error();
+    if (!ctx || !out || !nrows) return set_error(CCCS_EINVAL, "null argument");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    cccs_kernel_stats rows[K_COUNT];
+    std::memset(rows, 0, sizeof(rows));
+    for (const ProfRec& r : ctx->prof.recs) {
+        float ms = 0.f;
+        CCCS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        cccs_kernel_stats& k = rows[r.kind];
+        k.launches++;
+        k.ms_total += ms;
+        k.alg_bytes += r.bytes;
+        k.alg_flops += r.flops;
+        k.work_items += r.items;
+    }
+    uint64_t cnt[K_COUNT] = {0};
+    if (ctx->prof.d_counts) CCCS_HIP(hipMemcpy(cnt, ctx->prof.d_counts, sizeof(cnt), hipMemcpyDeviceToHost));
+    rows[K_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_CONE];
+    rows[K_BL_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_BL_CONE];
+    rows[K_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_TERMA];
+    rows[K_BL_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_TERMA];
+    rows[K_BL_RAY].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_RAY];
+    int n = 0;
+    for (int k = 0; k < K_COUNT && n < max_rows; k++) {
+        if (rows[k].launches == 0) continue;
+        std::snprintf(rows[k].name, sizeof(rows[k].name), "%s", kKernelNames[k]);
+        out[n++] = rows[k];
+    }
+    *nrows = n;
+    return CCCS_OK;
+}

@@ -1,0 +1,211 @@
+// cccs_internal.hpp -- types and math shared by the host pipeline and the gfx950 kernels.
+//
+// Geometry convention (CCCS/CudaUtilities/geometry.cu:6-19): PVCS is the patient volume frame
+// (x fastest in memory, `start` = centre of voxel 0); BEV is the beam frame with the beam along
+// +y; REV is the frame of one convolution (cone) direction, sampled as lines along its y axis.
+// Rotations are applied as 3x3 matrices built once per beam/cone on the host in double
+// precision and rounded to FP32, instead of per-thread Rodrigues evaluations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "../../include/cccs.h"
+
+#define CCCS_HD __host__ __device__ __forceinline__
+
+namespace cccs {
+
+struct v3 {
+    float x, y, z;
+};
+CCCS_HD v3 mk3(float x, float y, float z) { return v3{x, y, z}; }
+CCCS_HD v3 operator+(v3 a, v3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+CCCS_HD v3 operator-(v3 a, v3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+CCCS_HD v3 operator*(v3 a, v3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+CCCS_HD v3 operator/(v3 a, v3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
+CCCS_HD v3 operator*(v3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+CCCS_HD v3 operator*(float s, v3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+CCCS_HD v3 operator+(v3 a, float s) { return mk3(a.x + s, a.y + s, a.z + s); }
+CCCS_HD v3 operator-(v3 a, float s) { return mk3(a.x - s, a.y - s, a.z - s); }
+CCCS_HD float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CCCS_HD float len3(v3 v) { return sqrtf(dot3(v, v)); }
+CCCS_HD v3 norm3(v3 v) {
+    float inv = 1.0f / sqrtf(dot3(v, v));
+    return v * inv;
+}
+inline v3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+inline v3 ldu3(const uint32_t* p) { return mk3((float)p[0], (float)p[1], (float)p[2]); }
+
+// Row-major 3x3, FP32.
+struct m33 {
+    float m[9];
+    CCCS_HD v3 operator()(v3 v) const {
+        return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                   m[6] * v.x + m[7] * v.y + m[8] * v.z);
+    }
+};
+
+// float -> int toward -inf with NaN -> 0 (the convention of the reference's float->int casts)
+CCCS_HD int f2i_rd(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483520.f) return 2147483647;
+    if (x <= -2147483648.f) return (-2147483647 - 1);
+    return (int)floorf(x);
+}
+
+// ---- host: rotation matrices (double precision build) ------------------------------------------
+struct dm33 {
+    double m[9];
+};
+dm33 dm_axis(double ax, double ay, double az, double t);  // right-hand rotation by t about unit axis
+dm33 dm_mul(const dm33& a, const dm33& b);
+dm33 dm_transpose(const dm33& a);
+m33 to_f(const dm33& a);
+v3 dm_apply_f(const dm33& a, v3 v);  // apply in double, round result to float
+
+// BEV <- PVCS and inverse (CudaUtilities/geometry.cu:61-92)
+dm33 dm_beam_fwd(float theta, float phi, float coll);
+dm33 dm_beam_inv(float theta, float phi, float coll);
+// REV <- BEV and inverse (geometry.cu:119-145)
+dm33 dm_kernel_fwd(float theta, float phi);
+dm33 dm_kernel_inv(float theta, float phi);
+// beamlet bev <- BEV and inverse (geometry.cu:95-115)
+dm33 dm_beamlet_fwd(float theta, float phi);
+dm33 dm_beamlet_inv(float theta, float phi);
+
+// ---- trilinear sampling with texture-coordinate conventions ------------------------------------
+// Texel i is centred at coordinate i+0.5; taps outside [0,n) read 0 (border addressing).
+CCCS_HD float fetch_or0(const float* __restrict__ v, int nx, int ny, int nz, int i, int j, int k) {
+    return (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz)
+               ? 0.f
+               : v[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k)];
+}
+CCCS_HD float tex3_border(const float* __restrict__ v, int nx, int ny, int nz, float u, float w, float t) {
+    float xb = u - 0.5f, yb = w - 0.5f, zb = t - 0.5f;
+    if (!(xb > -1.f && yb > -1.f && zb > -1.f && xb < (float)nx && yb < (float)ny && zb < (float)nz)) return 0.f;
+    float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+    int i = (int)fx, j = (int)fy, k = (int)fz;
+    float a = xb - fx, b = yb - fy, c = zb - fz;
+    float c00 = fetch_or0(v, nx, ny, nz, i, j, k), c10 = fetch_or0(v, nx, ny, nz, i + 1, j, k);
+    float c01 = fetch_or0(v, nx, ny, nz, i, j + 1, k), c11 = fetch_or0(v, nx, ny, nz, i + 1, j + 1, k);
+    float d00 = fetch_or0(v, nx, ny, nz, i, j, k + 1), d10 = fetch_or0(v, nx, ny, nz, i + 1, j, k + 1);
+    float d01 = fetch_or0(v, nx, ny, nz, i, j + 1, k + 1), d11 = fetch_or0(v, nx, ny, nz, i + 1, j + 1, k + 1);
+    float e0 = c00 + a * (c10 - c00), e1 = c01 + a * (c11 - c01);
+    float f0 = d00 + a * (d10 - d00), f1 = d01 + a * (d11 - d01);
+    float g0 = e0 + b * (e1 - e0), g1 = f0 + b * (f1 - f0);
+    return g0 + c * (g1 - g0);
+}
+
+// ---- Siddon march (DoseCalcAlgo/cudaSiddon.cu:20-115) -------------------------------------------
+// The float operation sequence is fixed (no contraction: every file is built with
+// -ffp-contract=off) so the segment list is reproducible bit for bit on any IEEE FP32 machine.
+// `seg(fx, fy, fz, len)` is called for every segment of length >= 0.001 cm with the midpoint
+// fetch coordinate (density is sampled at that coordinate without the +0.5 texel shift).
+template <class Seg>
+CCCS_HD float siddon_march(v3 this_voxel, v3 start, v3 end, v3 voxel, v3 src, const float* __restrict__ dens,
+                           int nx, int ny, int nz, Seg&& seg, int* iters) {
+    v3 diff = this_voxel - src;
+    v3 startdiff = start - src;
+    v3 enddiff = end - src;
+    float ax0 = ((startdiff.x) - 0.5f * voxel.x) / (diff.x);
+    float ay0 = ((startdiff.y) - 0.5f * voxel.y) / (diff.y);
+    float az0 = ((startdiff.z) - 0.5f * voxel.z) / (diff.z);
+    float ax1 = ((enddiff.x) + 0.5f * voxel.x) / (diff.x);
+    float ay1 = ((enddiff.y) + 0.5f * voxel.y) / (diff.y);
+    float az1 = ((enddiff.z) + 0.5f * voxel.z) / (diff.z);
+    float a_min = fmaxf(0.f, fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fminf(az0, az1)));
+    float a_max = fminf(1.f, fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1)));
+    float rpl = 0.0f;
+    int iter = 0;
+    if (!(a_min >= a_max)) {
+        float d12 = len3(diff);
+        float step_x = fabsf(voxel.x / diff.x);
+        float step_y = fabsf(voxel.y / diff.y);
+        float step_z = fabsf(voxel.z / diff.z);
+        float alpha = a_min, alpha_x = a_min, alpha_y = a_min, alpha_z = a_min;
+        float nextalpha = 0.f;
+        while (alpha < a_max && ++iter < 5000) {
+            bool vx = (alpha_x >= 0.0f && alpha_x < 1.0f);
+            bool vy = (alpha_y >= 0.0f && alpha_y < 1.0f);
+            bool vz = (alpha_z >= 0.0f && alpha_z < 1.0f);
+            if (!(vx || vy || vz)) break;
+            // ties advance every tied plane (three independent tests, as in cudaSiddon.cu:84-95)
+            bool tx = vx && (!vy || alpha_x <= alpha_y) && (!vz || alpha_x <= alpha_z);
+            bool ty = vy && (!vx || alpha_y <= alpha_x) && (!vz || alpha_y <= alpha_z);
+            bool tz = vz && (!vx || alpha_z <= alpha_x) && (!vy || alpha_z <= alpha_y);
+            if (tx) { nextalpha = alpha_x; alpha_x += step_x; }
+            if (ty) { nextalpha = alpha_y; alpha_y += step_y; }
+            if (tz) { nextalpha = alpha_z; alpha_z += step_z; }
+            float inter = fabsf(d12 * (nextalpha - alpha));
+            if (inter >= 0.001f) {
+                float am = (nextalpha + alpha) * 0.5f;
+                float fx = (src.x + am * diff.x - start.x) / voxel.x;
+                float fy = (src.y + am * diff.y - start.y) / voxel.y;
+                float fz = (src.z + am * diff.z - start.z) / voxel.z;
+                rpl += inter * tex3_border(dens, nx, ny, nz, fx, fy, fz);
+                seg(fx, fy, fz, inter);
+            }
+            alpha = nextalpha;
+        }
+    }
+    if (iters) *iters = iter;
+    return rpl;
+}
+
+// ---- device-side problem constants -------------------------------------------------------------
+struct VolDesc {
+    int nx, ny, nz;
+    v3 start, voxel, end;
+};
+
+struct BeamDev {
+    v3 src, dir_n, iso;
+    float sad;
+    float iso_dist;  // |iso - src|
+    int fmx, fmy;
+    float bsx, bsy;
+    m33 beam_fwd;  // BEV <- PVCS rotation
+};
+
+constexpr int kMaxSpec = 30;
+struct SpecDev {
+    int n;
+    float flu[kMaxSpec], en[kMaxSpec], mu[kMaxSpec];
+};
+
+// TERMA of one point (cudaSiddon_device, cudaSiddon.cu:117-208).  Returns 0 outside the field
+// or for a zero-weight beamlet; beamletnum >= 0 keeps only that beamlet's footprint.
+CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b, const SpecDev& sp,
+                          const VolDesc& vd, const float* __restrict__ dens, float bhc, int beamletnum,
+                          int* iters) {
+    if (iters) *iters = 0;
+    v3 diff = p - b.src;
+    v3 projection = dot3(diff, b.dir_n) * b.dir_n;
+    v3 rejection = diff - projection;
+    float trad = len3(rejection) * fabsf(b.iso_dist / len3(projection));
+    v3 pf = b.beam_fwd(trad * norm3(rejection));
+    float fx = (pf.x / b.bsx) + ((float)b.fmx * 0.5f);
+    float fz = (pf.z / b.bsy) + ((float)b.fmy * 0.5f);
+    // a NaN coordinate (voxel on the central axis) passes these tests, as in the reference
+    if (fx < 0 || fx >= (float)b.fmx || fz < 0 || fz >= (float)b.fmy) return 0.f;
+    int fidx = f2i_rd(fx) + b.fmx * f2i_rd(fz);
+    if (fidx < 0 || fidx >= b.fmx * b.fmy || (beamletnum >= 0 && fidx != beamletnum)) return 0.f;
+    float fluence = fmap[fidx];
+    if (fluence <= 0.f) return 0.f;
+    float rpl = siddon_march(p, vd.start, vd.end, vd.voxel, b.src, dens, vd.nx, vd.ny, vd.nz,
+                             [](float, float, float, float) {}, iters);
+    float tsum = 0.f;
+    for (int e = 0; e < sp.n; e++) {
+        float this_fluence = fluence * sp.flu[e];
+        tsum += this_fluence * sp.en[e] * sp.mu[e] * expf(-1.f * sp.mu[e] * rpl);
+    }
+    float d = len3(p - b.src);
+    float invsq = (b.sad * b.sad) / (d * d);
+    return tsum * bhc * invsq;
+}
+
+}  // namespace cccs

@@ -1,0 +1,121 @@
+// context.hpp -- the resident device context behind cccs_ctx (replaces the per-device static
+// arrays and textures of initCudaConstandTex, dosecalc-beam/nvbbRayConvolve_host.cu:27-54).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "cccs_internal.hpp"
+#include "errors.hpp"
+
+namespace cccs {
+
+// Grow-only device buffer; contents are not preserved across growth.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Grow-only pinned host buffer.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr int kMaxRays = 48;      // ntheta/2 * nphi <= 48 cone lines per beam in the plain path
+constexpr int kMaxLine = 1024;    // REV line limit of the reference (nvbbRayConvolve_host.cu:567)
+constexpr int kMaxRadii = 32;
+
+// Per-cone constants of the plain beam path, consumed by the cone and PVCS kernels.
+struct ConeDev {
+    m33 kinv;          // BEV <- REV (index units)
+    m33 a;             // REV <- PVCS index: kernel_fwd * beam_fwd
+    v3 rs;             // REV origin in REV index units (revStart / rev spacing)
+    v3 t;              // REV coordinate of PVCS index (0,0,0)
+    int nx, ny, nz;    // line length, lines along y, lines along z
+    int rowf, rowb;    // kernel rows of the forward / backward walk
+    uint64_t line0;    // first global line index of this cone
+    uint64_t off;      // float offset of this cone's REV dose grid
+};
+
+struct BeamGeom {
+    v3 lim_min;            // BEV crop-box origin in PVCS [cm]
+    uint32_t bev_size[3];
+    v3 bev_min, bev_max;
+    int nrays;
+    v3 rev_min[kMaxRays];
+    uint32_t rev_size[kMaxRays][3];
+    uint64_t total_lines, total_rev;
+    int pvcs_lo[3], pvcs_hi[3];  // PVCS index box that can receive dose (half-open)
+};
+
+// ---- profiling ---------------------------------------------------------------------------------
+enum KernelKind { K_TERMA = 0, K_CROP, K_CONE, K_PVCS, K_BL_TERMA, K_BL_RAY, K_BL_CONE, K_BL_BEV, K_BL_UNPACK,
+                  K_BL_COMPACT, K_COUNT };
+extern const char* const kKernelNames[K_COUNT];
+
+struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+    double bytes, flops;
+    uint64_t items;
+};
+
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<ProfRec> recs;
+    uint64_t* d_counts = nullptr;  // per-kind device counters (walk iterations, Siddon steps)
+    uint32_t* d_part = nullptr;    // per-block partial counts
+    size_t part_cap = 0;
+};
+
+}  // namespace cccs
+
+struct cccs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cccs_problem prob{};  // scalar copy (table pointers cleared)
+    cccs::VolDesc vol{};
+    cccs::SpecDev spec{};
+    std::vector<float> kernel, radii, theta, phi;
+    uint32_t max_rev_beam[3] = {0, 0, 0};
+    int nrays = 0;
+    float* d_dens = nullptr;
+    float* d_tables = nullptr;  // kernel (ntheta*nradii) | radii | inverse widths | mean radii | inverse widths
+    size_t resident = 0;
+    cccs::DevBuf terma, fmap, partial, rev, cones, dose, col;
+    cccs::HostBuf h_small, h_cones;
+    std::vector<uint8_t> scratch_host;
+    cccs::Profiler prof;
+};
+
+namespace cccs {
+// table offsets inside d_tables
+inline size_t tab_kernel(const cccs_ctx*) { return 0; }
+inline size_t tab_radii(const cccs_ctx* c) { return (size_t)c->prob.ntheta * c->prob.nradii; }
+inline size_t tab_iw(const cccs_ctx* c) { return tab_radii(c) + c->prob.nradii; }
+inline size_t tab_mradii(const cccs_ctx* c) { return tab_iw(c) + c->prob.nradii; }
+inline size_t tab_miw(const cccs_ctx* c) { return tab_mradii(c) + c->prob.nradii; }
+
+BeamDev make_beam_dev(const cccs_beam& b);
+// Profiling hooks: prof_begin returns a token (or -1 when profiling is off); prof_end closes it.
+// prof_counter returns the per-block partial-count array for a launch of `blocks` workgroups (or
+// nullptr when off); prof_fold adds those partials into the kind's device counter.
+int prof_begin(cccs_ctx* c, int kind);
+void prof_end(cccs_ctx* c, int token, double bytes, double flops, uint64_t items);
+uint32_t* prof_counter(cccs_ctx* c, size_t blocks);
+void prof_fold(cccs_ctx* c, int kind, size_t blocks);
+int check_beam(const cccs_ctx* ctx, const cccs_beam& b);
+}  // namespace cccs

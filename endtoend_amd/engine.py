@@ -1,0 +1,302 @@
+"""Python-side handle on the HIP CCCS engine (a thin layer over include/cccs.h).
+
+`ProblemSpec` carries the problem description (the reference's CONSTANTS + SHM_DATA),
+`Engine` owns one resident device context (cccs_ctx), and `make_beam` builds a beam from
+gantry/couch/collimator angles exactly as the reference's beam-list loader does
+(DoseCalcIO/beam.cpp:441-443 then BEAM::reconfigure).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Sequence
+
+import numpy as np
+
+from . import abi
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+PI_F = np.float32(3.14159265358979323846)
+
+
+@dataclasses.dataclass
+class KernelTables:
+    """Spectrum, radial boundaries and the 48x24 poly-energetic kernel of one beam quality."""
+    spectrum: np.ndarray  # (4, nspec): fluence, energy, mu_en, mu
+    radii: np.ndarray     # (24,)
+    poly: np.ndarray      # (48, 24)
+
+
+def load_tables(name: str = "spec_6mv") -> KernelTables:
+    path = os.path.join(DATA_DIR, name + ".txt")
+    with open(path) as f:
+        lines = [ln for ln in f.read().splitlines() if ln and not ln.startswith("#")]
+    ns = int(lines[0].split()[1])
+    spec = np.array([[float(v) for v in ln.split()] for ln in lines[1:1 + ns]], np.float32).T.copy()
+    radii = np.array(lines[2 + ns].split(), np.float32)
+    poly = np.array([[float(v) for v in ln.split()] for ln in lines[4 + ns:4 + ns + 48]], np.float32)
+    return KernelTables(spec, radii, poly)
+
+
+def cumulative_kernel(t: KernelTables, nphi: int, ntheta: int):
+    """make_cumulative_kernel (kernel.cpp:260-431) through the library: (kernel, theta, phi)."""
+    lib = abi.load()
+    nr = t.poly.shape[1]
+    kern = np.zeros((ntheta, nr), np.float32)
+    nrays = nphi * ntheta // 2
+    theta = np.zeros(nrays, np.float32)
+    phi = np.zeros(nrays, np.float32)
+    poly = np.ascontiguousarray(t.poly, np.float32)
+    abi.check(lib.cccs_cumulative_kernel(abi.fptr(poly), nphi, ntheta, abi.fptr(kern), abi.fptr(theta),
+                                         abi.fptr(phi)), "cccs_cumulative_kernel")
+    return kern, theta, phi
+
+
+@dataclasses.dataclass
+class ProblemSpec:
+    density: np.ndarray            # (nz, ny, nx) float32
+    voxel: tuple
+    start: tuple                   # centre of voxel 0 [cm]
+    kernel: np.ndarray             # (ntheta, nradii) cumulative
+    radii: np.ndarray
+    theta: np.ndarray
+    phi: np.ndarray
+    spectrum: np.ndarray           # (4, nspec)
+    nphi: int
+    ntheta: int
+    rev_latspacing: float
+    rev_longspacing: float
+    penumbra: float = 1.0
+    kernel_extent: float = 2.0
+    ss_factor: int = 1
+    bbox_start: tuple = (0, 0, 0)
+    bbox_size: tuple | None = None
+    max_rev_size: tuple = (0, 0, 0)
+    beamhard_correct: float = 1.0
+
+    def __post_init__(self):
+        self.density = np.ascontiguousarray(self.density, np.float32)
+        nz, ny, nx = self.density.shape
+        if self.bbox_size is None:
+            self.bbox_size = (nx, ny, nz)
+        for name in ("kernel", "radii", "theta", "phi", "spectrum"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), np.float32))
+        self._keep = []
+
+    @property
+    def size(self):
+        nz, ny, nx = self.density.shape
+        return (nx, ny, nz)
+
+    @property
+    def bbox_shape(self):
+        """(z, y, x) shape of calc_bbox arrays."""
+        bx, by, bz = self.bbox_size
+        return (bz, by, bx)
+
+    def struct(self) -> abi.Problem:
+        p = abi.Problem()
+        p.size[:] = self.size
+        p.voxel[:] = [float(v) for v in self.voxel]
+        p.start[:] = [float(v) for v in self.start]
+        p.bbox_start[:] = [int(v) for v in self.bbox_start]
+        p.bbox_size[:] = [int(v) for v in self.bbox_size]
+        p.max_rev_size[:] = [int(v) for v in self.max_rev_size]
+        p.rev_latspacing = self.rev_latspacing
+        p.rev_longspacing = self.rev_longspacing
+        p.kernel_extent = self.kernel_extent
+        p.penumbra = self.penumbra
+        p.ss_factor = self.ss_factor
+        p.nphi = self.nphi
+        p.ntheta = self.ntheta
+        p.nradii = int(self.radii.shape[0])
+        p.beamhard_correct = self.beamhard_correct
+        spec = [np.ascontiguousarray(self.spectrum[k]) for k in range(4)]
+        self._keep = spec
+        p.density = abi.fptr(self.density)
+        p.kernel = abi.fptr(self.kernel)
+        p.radii = abi.fptr(self.radii)
+        p.conv_theta_deg = abi.fptr(self.theta)
+        p.conv_phi_deg = abi.fptr(self.phi)
+        p.nspec = int(self.spectrum.shape[1])
+        p.spec_fluence = abi.fptr(spec[0])
+        p.spec_energy = abi.fptr(spec[1])
+        p.spec_mu_en = abi.fptr(spec[2])
+        p.spec_mu = abi.fptr(spec[3])
+        return p
+
+
+def deg2rad_f(deg: float) -> np.float32:
+    """(float(deg) * PI) / 180.0f as the beam-list loader computes it (beam.cpp:441-442)."""
+    return np.float32(np.float32(np.float32(deg) * PI_F) / np.float32(180.0))
+
+
+def make_beam(gantry_deg: float, couch_deg: float = 0.0, coll_deg: float = 0.0, iso=(0.0, 0.0, 0.0),
+              sad: float = 100.0, fmap_size=(20, 20), beamlet_size=(0.5, 0.5)) -> abi.Beam:
+    b = abi.Beam()
+    b.azimuth = deg2rad_f(gantry_deg)
+    b.zenith = deg2rad_f(couch_deg)
+    b.coll = deg2rad_f(coll_deg)
+    b.isocenter[:] = [float(v) for v in iso]
+    b.sad = sad
+    b.fmap_size[:] = [int(v) for v in fmap_size]
+    b.beamlet_size[:] = [float(v) for v in beamlet_size]
+    abi.check(abi.load().cccs_beam_reconfigure(C.byref(b), 0), "cccs_beam_reconfigure")
+    return b
+
+
+def beams_array(beams: Sequence[abi.Beam]):
+    arr = (abi.Beam * len(beams))()
+    for i, b in enumerate(beams):
+        C.memmove(C.addressof(arr[i]), C.addressof(b), C.sizeof(abi.Beam))
+    return arr
+
+
+class Engine:
+    """One resident cccs_ctx on a HIP device."""
+
+    def __init__(self, spec: ProblemSpec, device: int = 0):
+        self.spec = spec
+        self.lib = abi.load()
+        self._p = spec.struct()
+        h = C.c_void_p()
+        abi.check(self.lib.cccs_ctx_create(C.byref(self._p), device, C.byref(h)), "cccs_ctx_create")
+        self.ctx = h
+
+    def close(self):
+        if self.ctx:
+            self.lib.cccs_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memory(self):
+        r, s = C.c_uint64(), C.c_uint64()
+        abi.check(self.lib.cccs_ctx_memory(self.ctx, C.byref(r), C.byref(s)), "cccs_ctx_memory")
+        return r.value, s.value
+
+    def beam_dose(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray]) -> np.ndarray:
+        dose = np.zeros(self.spec.density.shape, np.float32)
+        fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]
+        ptrs = (abi._fp * len(fm))(*[abi.fptr(f) for f in fm])
+        arr = beams_array(beams)
+        abi.check(self.lib.cccs_beam_dose(self.ctx, arr, len(beams), ptrs, abi.fptr(dose)), "cccs_beam_dose")
+        return dose
+
+    # ---- device-resident inputs (for throughput measurements) ----
+    def dev_alloc(self, nbytes: int) -> int:
+        ptr = C.c_void_p()
+        abi.check(self.lib.cccs_dev_alloc(self.ctx, nbytes, C.byref(ptr)), "cccs_dev_alloc")
+        return ptr.value
+
+    def dev_free(self, ptr: int):
+        abi.check(self.lib.cccs_dev_free(self.ctx, C.c_void_p(ptr)), "cccs_dev_free")
+
+    def dev_upload(self, ptr: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        abi.check(self.lib.cccs_dev_upload(self.ctx, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p), a.nbytes),
+                  "cccs_dev_upload")
+
+    def dev_download(self, ptr: int, out: np.ndarray):
+        abi.check(self.lib.cccs_dev_download(self.ctx, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), out.nbytes),
+                  "cccs_dev_download")
+
+    def dev_zero(self, ptr: int, nbytes: int):
+        abi.check(self.lib.cccs_dev_zero(self.ctx, C.c_void_p(ptr), nbytes), "cccs_dev_zero")
+
+    def sync(self):
+        abi.check(self.lib.cccs_ctx_sync(self.ctx), "cccs_ctx_sync")
+
+    def beam_dose_device(self, beams: Sequence[abi.Beam], d_fluence: int, d_dose: int):
+        arr = beams_array(beams)
+        abi.check(self.lib.cccs_beam_dose_device(self.ctx, arr, len(beams), C.c_void_p(d_fluence),
+                                                 C.c_void_p(d_dose)), "cccs_beam_dose_device")
+
+    def profile(self, enable: bool = True):
+        abi.check(self.lib.cccs_ctx_profile(self.ctx, int(enable)), "cccs_ctx_profile")
+
+    def profile_reset(self):
+        abi.check(self.lib.cccs_ctx_profile_reset(self.ctx), "cccs_ctx_profile_reset")
+
+    def profile_read(self) -> dict:
+        rows = (abi.KernelStats * 16)()
+        n = C.c_int(0)
+        abi.check(self.lib.cccs_ctx_profile_read(self.ctx, rows, 16, C.byref(n)), "cccs_ctx_profile_read")
+        return {rows[i].name.decode(): dict(launches=rows[i].launches, ms=rows[i].ms_total,
+                                            bytes=rows[i].alg_bytes, flops=rows[i].alg_flops,
+                                            items=rows[i].work_items) for i in range(n.value)}
+
+    def beam_terma(self, beam: abi.Beam, fmap: np.ndarray, beamletnum: int = -1) -> np.ndarray:
+        out = np.zeros(self.spec.bbox_shape, np.float32)
+        f = np.ascontiguousarray(fmap, np.float32)
+        abi.check(self.lib.cccs_beam_terma(self.ctx, C.byref(beam), abi.fptr(f), beamletnum, abi.fptr(out)),
+                  "cccs_beam_terma")
+        return out
+
+    def geometry(self, beam: abi.Beam, fmap: np.ndarray) -> abi.BeamGeometry:
+        g = abi.BeamGeometry()
+        f = np.ascontiguousarray(fmap, np.float32)
+        abi.check(self.lib.cccs_beam_geometry_of(self.ctx, C.byref(beam), abi.fptr(f), C.byref(g)),
+                  "cccs_beam_geometry_of")
+        return g
+
+    def siddon_paths(self, source, points: np.ndarray, max_segments: int = 1024):
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        n = pts.shape[0]
+        src = np.ascontiguousarray(source, np.float32)
+        ijk = np.zeros((n, max_segments, 3), np.int32)
+        mid = np.zeros((n, max_segments, 3), np.float32)
+        ln = np.zeros((n, max_segments), np.float32)
+        cnt = np.zeros(n, np.int32)
+        rpl = np.zeros(n, np.float32)
+        abi.check(self.lib.cccs_siddon_paths(self.ctx, abi.fptr(src), abi.fptr(pts), n, max_segments,
+                                             ijk.ctypes.data_as(C.POINTER(C.c_int32)), abi.fptr(mid), abi.fptr(ln),
+                                             cnt.ctypes.data_as(C.POINTER(C.c_int32)), abi.fptr(rpl)),
+                  "cccs_siddon_paths")
+        return ijk, mid, ln, cnt, rpl
+
+    def beamlet_dense(self, beam: abi.Beam, fmap: np.ndarray, mode: int = abi.CCCS_BEAMLET_FAITHFUL,
+                      max_beamlets: int | None = None) -> np.ndarray:
+        f = np.ascontiguousarray(fmap, np.float32)
+        nact = int((f > 0).sum())
+        cap = max_beamlets or max(1, nact)
+        out = np.zeros((cap,) + self.spec.bbox_shape, np.float32)
+        n = C.c_int(0)
+        abi.check(self.lib.cccs_beamlet_dense(self.ctx, C.byref(beam), abi.fptr(f), mode, abi.fptr(out), cap,
+                                              C.byref(n)), "cccs_beamlet_dense")
+        return out[:n.value]
+
+    def beamlet_dose(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray], threshold: float = 1e-4,
+                     mode: int = abi.CCCS_BEAMLET_FAST, keep: bool = True):
+        """Sparse columns of every active beamlet: list of (beam, beamlet, keys, values) or counts."""
+        cols = []
+        stats = {"columns": 0, "nnz": 0}
+
+        def sink(user, colp):
+            c = colp.contents
+            stats["columns"] += 1
+            stats["nnz"] += c.nnz
+            if keep:
+                k = np.ctypeslib.as_array(c.keys, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.uint64)
+                v = np.ctypeslib.as_array(c.values, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.float32)
+                cols.append((c.beam_index, c.beamlet_index, k, v))
+            return 0
+
+        cb = abi.SPARSE_SINK(sink)
+        fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]
+        ptrs = (abi._fp * len(fm))(*[abi.fptr(f) for f in fm])
+        arr = beams_array(beams)
+        abi.check(self.lib.cccs_beamlet_dose(self.ctx, arr, len(beams), ptrs, threshold, mode, cb, None),
+                  "cccs_beamlet_dose")
+        return cols if keep else stats

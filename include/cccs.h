@@ -1,0 +1,224 @@
+/*
+ * cccs.h -- C-ABI of the MI355X collapsed-cone convolution/superposition (CCCS) dose engine.
+ *
+ * Plain C types only: pointers, sizes and POD structs.  Every entry point returns an int status
+ * (CCCS_OK == 0, negative on failure) and never throws across the boundary.  Host buffers are
+ * owned by the caller; device memory is owned by the context unless a call says otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the xuqifan897/EndtoEnd tree):
+ *   problem description   CONSTANTS / SHM_DATA          CCCS/include/server/brain_defs.h:14-90
+ *   beam description      BEAM                          CCCS/DoseCalcIO/beam.h:16-95
+ *   context set-up        initCudaConstandTex            CCCS/dosecalc-beam/nvbbRayConvolve_host.cu:158-293
+ *                                                        CCCS/dosecalc-beamlet/nvbbRayConvolve_host.cu:283-433
+ *   full-beam dose        radconvolveTexture (beam)      CCCS/dosecalc-beam/nvbbRayConvolve_host.cu:296-742
+ *   beamlet coefficients  radconvolveTexture (beamlet)   CCCS/dosecalc-beamlet/nvbbRayConvolve_host.cu:444-1409
+ *   sparse columns        _sparsify                      CCCS/DoseCalcIO/sparse_io.cpp:22-58
+ *   tear-down             freeCudaTexture                CCCS/dosecalc-beam/nvbbRayConvolve_host.cu:745-771
+ *   dose read-back        cuda_sum_device_float_dose     CCCS/dosecalc-beam/nvbbRayConvolve_host.cu:774-837
+ */
+#ifndef CCCS_H_
+#define CCCS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+#define CCCS_OK          0
+#define CCCS_EINVAL     -1  /* bad argument or inconsistent problem description                */
+#define CCCS_EGEOM      -2  /* geometry outside limits: REV grid > max_rev_size, line > limit,
+                               empty/oversized crop box (the reference throws ArraySizeError /
+                               runtime_error here: nvbbRayConvolve_host.cu:148-154, 494-498, 567) */
+#define CCCS_ENOMEM     -3  /* device allocation failed or request exceeds the HBM budget      */
+#define CCCS_EIO        -4  /* file could not be read or parsed                                */
+#define CCCS_EDEVICE    -5  /* HIP runtime error                                               */
+#define CCCS_EBUSY      -6  /* serving queue full / request not finished yet                   */
+
+/* ---- problem description (CONSTANTS + SHM_DATA, brain_defs.h:14-90) ---------------------- */
+typedef struct cccs_problem {
+    uint32_t size[3];          /* PVCS volume dims Nx Ny Nz (x fastest in memory)              */
+    float    voxel[3];         /* voxel size [cm]                                               */
+    float    start[3];         /* coords of the CENTRE of voxel 0 [cm]                          */
+    uint32_t bbox_start[3];    /* calc_bbox start (voxels)                                      */
+    uint32_t bbox_size[3];     /* calc_bbox size (voxels)                                       */
+    uint32_t max_rev_size[3];  /* REV grid limit; 0 = derive as the beam path does (:248-258)  */
+    float    rev_latspacing;   /* convolution lateral ray spacing [cm]                          */
+    float    rev_longspacing;  /* convolution step [cm]                                         */
+    float    kernel_extent;    /* kernel truncation used for pillar sizing [cm]                 */
+    float    penumbra;         /* crop-box margin [cm]                                          */
+    uint32_t ss_factor;        /* TERMA supersampling factor (beam path)                        */
+    uint32_t nphi;             /* azimuthal cone count                                          */
+    uint32_t ntheta;           /* polar cone count (even, divides 48)                           */
+    uint32_t nradii;           /* radial samples of the cumulative kernel (24)                  */
+    float    beamhard_correct; /* TERMA scale (the reference fixes it to 1)                     */
+    /* host tables, copied by cccs_ctx_create */
+    const float* density;        /* Nz*Ny*Nx, g/cc                                               */
+    const float* kernel;         /* cumulative kernel, ntheta x nradii, r fastest                */
+    const float* radii;          /* nradii radial boundaries [cm]                                */
+    const float* conv_theta_deg; /* nphi*ntheta/2 cone polar angles                              */
+    const float* conv_phi_deg;   /* nphi*ntheta/2 cone azimuth angles                            */
+    uint32_t     nspec;          /* spectrum bins (<= 30)                                        */
+    const float* spec_fluence;   /* normalised fluence per bin                                   */
+    const float* spec_energy;    /* MeV                                                          */
+    const float* spec_mu_en;     /* cm^2/g                                                       */
+    const float* spec_mu;        /* cm^2/g                                                       */
+} cccs_problem;
+
+/* ---- beam description (BEAM, beam.h:16-95) ------------------------------------------------- */
+typedef struct cccs_beam {
+    float    azimuth;          /* gantry [rad]                                                  */
+    float    zenith;           /* couch  [rad]                                                  */
+    float    coll;             /* collimator [rad]                                              */
+    float    source[3];        /* x-ray source [cm]                                             */
+    float    direction[3];     /* unit beam direction                                           */
+    float    isocenter[3];     /* [cm]                                                          */
+    float    sad;              /* source-axis distance [cm]                                     */
+    uint32_t fmap_size[2];     /* fluence map dims (x fastest)                                  */
+    float    beamlet_size[2];  /* [cm]                                                          */
+} cccs_beam;
+
+/* Per-beam geometry the plain path derives on the host (crop box + REV grids), exposed so
+ * tests can compare it with the oracle.  REV arrays hold one entry per cone direction. */
+typedef struct cccs_beam_geometry {
+    float    lim_min[3];       /* BEV crop-box origin in PVCS [cm]                              */
+    uint32_t bev_size[3];      /* BEV crop-box size (index units)                               */
+    float    bev_min[3];       /* BEV-index min of non-zero TERMA minus penumbra                */
+    float    bev_max[3];       /* BEV-index max of non-zero TERMA plus penumbra                 */
+    int      nrays;
+    float    rev_min[48][3];   /* REV min coords per cone (<= 48 cone lines)                    */
+    uint32_t rev_size[48][3];  /* REV grid dims per cone (line length first)                    */
+} cccs_beam_geometry;
+
+/* Sparse beamlet column: full-volume linear keys + FP32 values, in the reference's emission
+ * order (i outer, j, k inner; sparse_io.cpp:34-53). */
+typedef struct cccs_sparse_column {
+    int       beam_index;      /* position of the beam in the request                          */
+    int       beamlet_index;   /* linear fluence-map index of the beamlet                      */
+    uint64_t  nnz;
+    const uint64_t* keys;      /* valid only during the sink callback                          */
+    const float*    values;
+    float     perc_nonzero;
+    float     perc_dropped;
+} cccs_sparse_column;
+
+/* Called once per beamlet column; return 0 to continue, non-zero to abort the call. */
+typedef int (*cccs_sparse_sink)(void* user, const cccs_sparse_column* column);
+
+typedef struct cccs_ctx cccs_ctx;
+
+/* Selects how a beamlet call computes per-beamlet TERMA at REV samples.
+ *   CCCS_BEAMLET_FAITHFUL : full Siddon march per REV sample per cone, exactly the reference
+ *                           dataflow (dosecalc-beamlet/nvbbRayConvolve_device.cu:170-211).
+ *   CCCS_BEAMLET_FAST     : Siddon once per context voxel, shared by every cone.            */
+#define CCCS_BEAMLET_FAITHFUL 0
+#define CCCS_BEAMLET_FAST     1
+
+/* ---- context: replaces initCudaConstandTex / freeCudaTexture --------------------------------
+ * Uploads density, kernel table, radii, spectrum and cone angles to the device and keeps them
+ * resident until cccs_ctx_destroy.  `device` is the HIP ordinal. */
+int cccs_ctx_create(const cccs_problem* problem, int device, cccs_ctx** out);
+int cccs_ctx_destroy(cccs_ctx* ctx);
+/* HBM bytes held by the context (resident state plus cached work buffers). */
+int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scratch_bytes);
+
+/* ---- full-beam dose: replaces radconvolveTexture (dosecalc-beam) + cuda_sum_device_float_dose.
+ * Computes the dose of `nbeams` beams and adds their sum into `dose` (Nz*Ny*Nx, host memory).
+ * fluence[b] points at beam b's fluence map (fmap_size[1] rows of fmap_size[0]). */
+int cccs_beam_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
+                   const float* const* fluence, float* dose);
+
+/* Device-resident variant: d_fluence holds the maps back to back (beam order), d_dose is a
+ * device buffer of Nz*Ny*Nx floats that is accumulated into.  Runs on the context stream;
+ * returns after the work is enqueued and the host-side geometry is resolved. */
+int cccs_beam_dose_device(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
+                          const float* d_fluence, float* d_dose);
+
+/* Stage outputs of the beam path, for stage-wise parity checks. */
+int cccs_beam_terma(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int beamletnum,
+                    float* terma_bbox);
+int cccs_beam_geometry_of(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence,
+                          cccs_beam_geometry* geom);
+
+/* Siddon voxel-index lists (bit-exact artefact).  For each of `npoints` end points traces the
+ * ray from `source`, writing up to `max_segments` segments per point: ijk[3] voxel index of the
+ * segment midpoint, mid[3] the midpoint fetch coordinate, len the segment length [cm].
+ * counts[p] receives the segment count (may exceed max_segments; extra segments are dropped)
+ * and rpl[p] the radiological path length. */
+int cccs_siddon_paths(cccs_ctx* ctx, const float* source, const float* points, int npoints,
+                      int max_segments, int32_t* ijk, float* mid, float* len, int32_t* counts,
+                      float* rpl);
+
+/* ---- beamlet coefficients: replaces radconvolveTexture (dosecalc-beamlet) + SparsifyManager.
+ * Every active beamlet (fluence > 0) of every beam yields one sparse column, passed to `sink`
+ * in beam order then active-beamlet order.  threshold: keep v > threshold * max(column).
+ * mode: CCCS_BEAMLET_FAITHFUL or CCCS_BEAMLET_FAST. */
+int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
+                      const float* const* fluence, float threshold, int mode,
+                      cccs_sparse_sink sink, void* user);
+
+/* Dense variant for parity tests: writes the bbox-sized dense column of each active beamlet of
+ * ONE beam into out (nactive x bbox voxels, active-beamlet order). */
+int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int mode,
+                       float* out, int max_beamlets, int* nactive);
+
+/* ---- host-side helpers restating the reference's input preparation ------------------------
+ * Beam angles -> source/direction (BEAM::reconfigure, beam.cpp:300-331). orientation 0 = from
+ * angles, 1 = from direction. */
+int cccs_beam_reconfigure(cccs_beam* beam, int orientation);
+
+/* Spectrum + mono-energetic kernels -> cumulative poly kernel and cone angles
+ * (read_spectrum_file spectra.cpp:7-62, read_kernel kernel.cpp:27-122, make_poly_kernel
+ * kernel.cpp:123-258, make_cumulative_kernel kernel.cpp:260-431).
+ * kernel_out: ntheta*nradii; theta_out/phi_out: nphi*ntheta/2; spec_out: 4*nspec rows
+ * (fluence, energy, mu_en, mu); radii_out: nradii. */
+int cccs_build_kernel_from_files(const char* spectrum_file, const char* kernel_dir,
+                                 const char* radii_file, int nphi, int ntheta,
+                                 float* kernel_out, float* radii_out, float* theta_out,
+                                 float* phi_out, float* spec_out, int* nspec);
+
+/* First half of the above: the poly-energetic kernel (48 polar bins x 24 radii total weights,
+ * r fastest), the radial boundaries and the normalised spectrum (4*nspec rows as above). */
+int cccs_poly_kernel_from_files(const char* spectrum_file, const char* kernel_dir,
+                                const char* radii_file, float* poly_total_out, float* radii_out,
+                                float* spec_out, int* nspec);
+
+/* Same, from an already assembled poly-energetic kernel (48 x 24 total weights). */
+int cccs_cumulative_kernel(const float* poly_total, int nphi, int ntheta,
+                           float* kernel_out, float* theta_out, float* phi_out);
+
+/* ---- device memory helpers (plain pointers on the context's device) ------------------------ */
+int cccs_dev_alloc(cccs_ctx* ctx, size_t bytes, void** ptr);
+int cccs_dev_free(cccs_ctx* ctx, void* ptr);
+int cccs_dev_upload(cccs_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cccs_dev_download(cccs_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cccs_dev_zero(cccs_ctx* ctx, void* ptr, size_t bytes);
+/* Waits for all work the context has enqueued. */
+int cccs_ctx_sync(cccs_ctx* ctx);
+
+/* ---- per-kernel profiling --------------------------------------------------------------------
+ * When enabled, every kernel launch of the context is bracketed by HIP events on the stream it
+ * runs on, and its algorithmic HBM bytes / FLOPs (from the launch's geometry and the kernel's own
+ * work counters) are recorded.  cccs_ctx_profile_read synchronises and returns one row per
+ * kernel name, summed over launches since the last reset. */
+typedef struct cccs_kernel_stats {
+    char     name[32];
+    uint64_t launches;
+    double   ms_total;         /* sum of event-measured launch durations                         */
+    double   alg_bytes;        /* algorithmic HBM bytes (sum over launches)                      */
+    double   alg_flops;        /* algorithmic FP32 operations (sum over launches)                */
+    uint64_t work_items;       /* kernel-specific count: voxels, samples or walk iterations      */
+} cccs_kernel_stats;
+int cccs_ctx_profile(cccs_ctx* ctx, int enable);
+int cccs_ctx_profile_reset(cccs_ctx* ctx);
+int cccs_ctx_profile_read(cccs_ctx* ctx, cccs_kernel_stats* out, int max_rows, int* nrows);
+
+/* Last error message of the calling thread ("" when none). */
+const char* cccs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCCS_H_ */

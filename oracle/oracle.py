@@ -1,0 +1,200 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/libcccs_oracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, always as the
+checker -- never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from endtoend_amd import abi
+from endtoend_amd.engine import ProblemSpec
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libcccs_oracle.so")
+_fp = C.POINTER(C.c_float)
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("terma_voxels", "terma_active", "siddon_steps", "rev_samples", "rev_valid",
+                                          "cone_iters", "cone_iters_all", "pvcs_hits")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def build() -> str:
+    r = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P, B, G = C.POINTER(abi.Problem), C.POINTER(abi.Beam), C.POINTER(abi.BeamGeometry)
+    i32p = C.POINTER(C.c_int32)
+    sig = {
+        "orc_reconfigure": [B, C.c_int],
+        "orc_kernel_tables": [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp,
+                              C.POINTER(C.c_int), _fp, _fp],
+        "orc_cumulative_kernel": [_fp, C.c_int, C.c_int, _fp, _fp, _fp],
+        "orc_terma": [P, B, _fp, C.c_int, _fp, C.POINTER(Stats)],
+        "orc_siddon_paths": [P, _fp, _fp, C.c_int, C.c_int, i32p, _fp, _fp, i32p, _fp],
+        "orc_beam_geometry": [P, B, _fp, G],
+        "orc_beam_dose": [P, B, _fp, _fp, C.POINTER(Stats)],
+        "orc_beam_cone_stage": [P, B, _fp, C.c_int, _fp, _fp, _fp],
+        "orc_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
+        "orc_pack_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
+        "orc_beamlet_dense": [P, B, _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int), C.POINTER(Stats)],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.orc_sparsify.argtypes = [_fp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_float,
+                                 C.POINTER(C.c_uint64), _fp, C.c_int64, _fp, _fp]
+    lib.orc_sparsify.restype = C.c_int64
+    _lib = lib
+    return lib
+
+
+def _chk(code, where):
+    if code != 0:
+        raise RuntimeError(f"oracle {where} returned {code}")
+
+
+def fp(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_fp)
+
+
+def reconfigure(beam: abi.Beam, orientation: int = 0) -> abi.Beam:
+    b = beam.copy()
+    _chk(load().orc_reconfigure(C.byref(b), orientation), "reconfigure")
+    return b
+
+
+def kernel_tables(data_dir: str, spectrum: str, nphi: int, ntheta: int):
+    kern = np.zeros((ntheta, 24), np.float32)
+    radii = np.zeros(24, np.float32)
+    th = np.zeros(nphi * ntheta // 2, np.float32)
+    ph = np.zeros(nphi * ntheta // 2, np.float32)
+    spec = np.zeros(4 * 30, np.float32)
+    poly = np.zeros((48, 24), np.float32)
+    cat = np.zeros((5, ntheta, 24), np.float32)
+    n = C.c_int(0)
+    _chk(load().orc_kernel_tables(os.path.join(data_dir, "spectra", spectrum + ".spec").encode(),
+                                  os.path.join(data_dir, "dsa").encode(),
+                                  os.path.join(data_dir, "dsa", "radii.dat").encode(), nphi, ntheta, fp(kern),
+                                  fp(radii), fp(th), fp(ph), fp(spec), C.byref(n), fp(poly), fp(cat)), "kernel_tables")
+    ns = n.value
+    return dict(kernel=kern, radii=radii, theta=th, phi=ph, spectrum=spec[:4 * ns].reshape(4, ns), poly=poly, cat=cat)
+
+
+def cumulative(poly: np.ndarray, nphi: int, ntheta: int):
+    kern = np.zeros((ntheta, 24), np.float32)
+    th = np.zeros(nphi * ntheta // 2, np.float32)
+    ph = np.zeros(nphi * ntheta // 2, np.float32)
+    _chk(load().orc_cumulative_kernel(fp(np.ascontiguousarray(poly, np.float32)), nphi, ntheta, fp(kern), fp(th),
+                                      fp(ph)), "cumulative")
+    return kern, th, ph
+
+
+def terma(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, beamletnum: int = -1):
+    p = spec.struct()
+    out = np.zeros(spec.bbox_shape, np.float32)
+    st = Stats()
+    _chk(load().orc_terma(C.byref(p), C.byref(beam), fp(np.ascontiguousarray(fmap, np.float32)), beamletnum, fp(out),
+                          C.byref(st)), "terma")
+    return out, st.as_dict()
+
+
+def siddon_paths(spec: ProblemSpec, source, points: np.ndarray, max_segments: int = 1024):
+    p = spec.struct()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = pts.shape[0]
+    ijk = np.zeros((n, max_segments, 3), np.int32)
+    mid = np.zeros((n, max_segments, 3), np.float32)
+    ln = np.zeros((n, max_segments), np.float32)
+    cnt = np.zeros(n, np.int32)
+    rpl = np.zeros(n, np.float32)
+    i32 = C.POINTER(C.c_int32)
+    _chk(load().orc_siddon_paths(C.byref(p), fp(np.ascontiguousarray(source, np.float32)), fp(pts), n, max_segments,
+                                 ijk.ctypes.data_as(i32), fp(mid), fp(ln), cnt.ctypes.data_as(i32), fp(rpl)), "siddon")
+    return ijk, mid, ln, cnt, rpl
+
+
+def geometry(spec: ProblemSpec, beam: abi.Beam, terma_bbox: np.ndarray):
+    p = spec.struct()
+    g = abi.BeamGeometry()
+    code = load().orc_beam_geometry(C.byref(p), C.byref(beam), fp(np.ascontiguousarray(terma_bbox, np.float32)),
+                                    C.byref(g))
+    return code, g
+
+
+def beam_dose(spec: ProblemSpec, beams, fmaps):
+    p = spec.struct()
+    dose = np.zeros(spec.density.shape, np.float32)
+    st = Stats()
+    for b, f in zip(beams, fmaps):
+        _chk(load().orc_beam_dose(C.byref(p), C.byref(b), fp(np.ascontiguousarray(f, np.float32)), fp(dose),
+                                  C.byref(st)), "beam_dose")
+    return dose, st.as_dict()
+
+
+def cone_stage(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, ray: int, rev_size):
+    p = spec.struct()
+    n = int(np.prod(rev_size))
+    shape = (rev_size[2], rev_size[1], rev_size[0])
+    d, t, o = (np.zeros(n, np.float32) for _ in range(3))
+    _chk(load().orc_beam_cone_stage(C.byref(p), C.byref(beam), fp(np.ascontiguousarray(fmap, np.float32)), ray, fp(d),
+                                    fp(t), fp(o)), "cone_stage")
+    return d.reshape(shape), t.reshape(shape), o.reshape(shape)
+
+
+def row_convolve(dens, terma_line, kernel, radii, P, delr, nphi, packed=False):
+    d = np.ascontiguousarray(dens, np.float32)
+    t = np.ascontiguousarray(terma_line, np.float32)
+    k = np.ascontiguousarray(kernel, np.float32)
+    r = np.ascontiguousarray(radii, np.float32)
+    out = np.zeros_like(d)
+    fn = load().orc_pack_row_convolve if packed else load().orc_row_convolve
+    _chk(fn(fp(d), fp(t), d.size, fp(k), k.shape[0], k.shape[1], fp(r), P, delr, nphi, fp(out)), "row_convolve")
+    return out
+
+
+def beamlet_dense(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, max_beamlets: int | None = None):
+    p = spec.struct()
+    f = np.ascontiguousarray(fmap, np.float32)
+    cap = max_beamlets or max(1, int((f > 0).sum()))
+    out = np.zeros((cap,) + spec.bbox_shape, np.float32)
+    n = C.c_int(0)
+    st = Stats()
+    _chk(load().orc_beamlet_dense(C.byref(p), C.byref(beam), fp(f), 0, fp(out), cap, C.byref(n), C.byref(st)),
+         "beamlet_dense")
+    return out[:n.value], st.as_dict()
+
+
+def sparsify(dense: np.ndarray, size, crop_start, crop_size, thresh: float):
+    d = np.ascontiguousarray(dense, np.float32).ravel()
+    cap = d.size
+    keys = np.zeros(cap, np.uint64)
+    vals = np.zeros(cap, np.float32)
+    u3 = C.c_uint32 * 3
+    pn, pd = C.c_float(), C.c_float()
+    n = load().orc_sparsify(fp(d), u3(*size), u3(*crop_start), u3(*crop_size), thresh,
+                            keys.ctypes.data_as(C.POINTER(C.c_uint64)), fp(vals), cap, C.byref(pn), C.byref(pd))
+    return keys[:n], vals[:n], pn.value, pd.value
