@@ -161,7 +161,7 @@ BeamDev make_beam_dev(const cccs_beam& b) {
     d.fmy = (int)b.fmap_size[1];
     d.bsx = b.beamlet_size[0];
     d.bsy = b.beamlet_size[1];
-    d.beam_fwd = to_f(dm_beam_fwd(b.azimuth, b.zenith, b.coll));
+    d.beam_fwd = rot2_beam_fwd(b.azimuth, b.zenith, b.coll);
     return d;
 }
 
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) siddon_path_kernel(PathArgs a) {
 // Pass 1 writes one partial (min xyz, max xyz, count) per workgroup.
 constexpr int kCropBlocks = 1024;
 __global__ void __launch_bounds__(256) crop_partial_kernel(const float* __restrict__ terma, int bx, int by, int bz,
-                                                           m33 rot, float* __restrict__ part) {
+                                                           Rot2 rot, float* __restrict__ part) {
     __shared__ float s[7][256];
     float mn0 = FLT_MAX, mn1 = FLT_MAX, mn2 = FLT_MAX, mx0 = -FLT_MAX, mx1 = -FLT_MAX, mx2 = -FLT_MAX, cnt = 0.f;
     const size_t n = (size_t)bx * by * bz;
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256) crop_partial_kernel(const float* __restri
             int x = (int)(v % bx);
             int y = (int)((v / bx) % by);
             int z = (int)(v / ((size_t)bx * by));
-            v3 b = rot(mk3((float)x, (float)y, (float)z));
+            v3 b = rot2(rot, mk3((float)x, (float)y, (float)z));
             mn0 = fminf(mn0, b.x); mn1 = fminf(mn1, b.y); mn2 = fminf(mn2, b.z);
             mx0 = fmaxf(mx0, b.x); mx1 = fmaxf(mx1, b.y); mx2 = fmaxf(mx2, b.z);
             cnt += 1.f;
@@ -535,7 +535,7 @@ int beam_geometry(cccs_ctx* ctx, const cccs_beam& b, const float* red, BeamGeom&
     g.bev_min = bmin;
     g.bev_max = bmax;
     dm33 binv = dm_beam_inv(b.azimuth, b.zenith, b.coll);
-    v3 tmin = dm_apply_f(binv, bmin);
+    v3 tmin = rot2(rot2_beam_inv(b.azimuth, b.zenith, b.coll), bmin);
     g.lim_min = mk3(p.start[0] + (tmin.x + (float)p.bbox_start[0]) * p.voxel[0],
                     p.start[1] + (tmin.y + (float)p.bbox_start[1]) * p.voxel[1],
                     p.start[2] + (tmin.z + (float)p.bbox_start[2]) * p.voxel[2]);
@@ -547,12 +547,14 @@ int beam_geometry(cccs_ctx* ctx, const cccs_beam& b, const float* red, BeamGeom&
     for (int r = 0; r < ctx->nrays; r++) {
         float th = ctx->theta[r] * 3.14159265358979323846f / 180.f;
         float ph = ctx->phi[r] * 3.14159265358979323846f / 180.f;
-        dm33 kf = dm_kernel_fwd(th, ph);
+        Rot2 kf = rot2_kernel_fwd(th, ph);
         v3 mn = mk3(FLT_MAX, FLT_MAX, FLT_MAX), mx = mk3(FLT_MIN, FLT_MIN, FLT_MIN);
         for (int c = 0; c < 8; c++) {
-            v3 in = mk3(lim.x * (c & 1), lim.y * ((c >> 1) & 1), lim.z * ((c >> 2) & 1));
-            v3 o = dm_apply_f(kf, in);
-            mn = mk3(std::min(mn.x, o.x), std::min(mn.y, o.y), std::min(mn.z, o.z));
+            v3 in = mk3(lim.x * (float)(c & 1), lim.y * (float)((c >> 1) & 1), lim.z * (float)((c >> 2) & 1));
+            v3 o = rot2(kf, in);
+            if (o.x < mn.x) mn.x = o.x;
+            if (o.y < mn.y) mn.y = o.y;
+            if (o.z < mn.z) mn.z = o.z;
             if (o.x > mx.x) mx.x = o.x;
             if (o.y > mx.y) mx.y = o.y;
             if (o.z > mx.z) mx.z = o.z;
@@ -592,7 +594,7 @@ int run_crop(cccs_ctx* ctx, const float* d_terma, const cccs_beam& b, float* red
     CCCS_TRY(ctx->partial.ensure(sizeof(float) * (kCropBlocks * 8 + 8)));
     CCCS_TRY(ctx->h_small.ensure(64));
     float* part = ctx->partial.as<float>();
-    m33 rot = to_f(dm_beam_fwd(b.azimuth, b.zenith, b.coll));
+    Rot2 rot = rot2_beam_fwd(b.azimuth, b.zenith, b.coll);
     const cccs_problem& p = ctx->prob;
     size_t n = bbox_voxels(ctx);
     int blocks = (int)std::min<size_t>(kCropBlocks, (n + 255) / 256);

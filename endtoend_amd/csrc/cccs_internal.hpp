@@ -49,6 +49,28 @@ struct m33 {
     }
 };
 
+// Two successive axis rotations in the Rodrigues form of CudaUtilities/geometry.cu:46-58, with the
+// sines/cosines computed once on the host (libm sincosf).  Used where a rounding difference would
+// flip a discrete decision (fluence-cell lookup, crop-box size, REV grid size), so those decisions
+// are reproducible against a host evaluation of the same formula.
+struct Rot2 {
+    v3 a1;
+    float s1, c1;
+    v3 a2;
+    float s2, c2;
+};
+CCCS_HD v3 rodrigues(v3 p, v3 r, float s, float c) {
+    return mk3((-r.x * (-r.x * p.x - r.y * p.y - r.z * p.z)) * (1 - c) + p.x * c + (-r.z * p.y + r.y * p.z) * s,
+               (-r.y * (-r.x * p.x - r.y * p.y - r.z * p.z)) * (1 - c) + p.y * c + (+r.z * p.x - r.x * p.z) * s,
+               (-r.z * (-r.x * p.x - r.y * p.y - r.z * p.z)) * (1 - c) + p.z * c + (-r.y * p.x + r.x * p.y) * s);
+}
+CCCS_HD v3 rot2(const Rot2& R, v3 p) { return rodrigues(rodrigues(p, R.a1, R.s1, R.c1), R.a2, R.s2, R.c2); }
+// host builders (CudaUtilities/geometry.cu:69-75, 85-92, 126-131, 140-145)
+Rot2 rot2_beam_fwd(float theta, float phi, float coll);
+Rot2 rot2_beam_inv(float theta, float phi, float coll);
+Rot2 rot2_kernel_fwd(float theta, float phi);
+Rot2 rot2_kernel_inv(float theta, float phi);
+
 // float -> int toward -inf with NaN -> 0 (the convention of the reference's float->int casts)
 CCCS_HD int f2i_rd(float x) {
     if (!(x == x)) return 0;
@@ -133,13 +155,12 @@ CCCS_HD float siddon_march(v3 this_voxel, v3 start, v3 end, v3 voxel, v3 src, co
             bool vy = (alpha_y >= 0.0f && alpha_y < 1.0f);
             bool vz = (alpha_z >= 0.0f && alpha_z < 1.0f);
             if (!(vx || vy || vz)) break;
-            // ties advance every tied plane (three independent tests, as in cudaSiddon.cu:84-95)
-            bool tx = vx && (!vy || alpha_x <= alpha_y) && (!vz || alpha_x <= alpha_z);
-            bool ty = vy && (!vx || alpha_y <= alpha_x) && (!vz || alpha_y <= alpha_z);
-            bool tz = vz && (!vx || alpha_z <= alpha_x) && (!vy || alpha_z <= alpha_y);
-            if (tx) { nextalpha = alpha_x; alpha_x += step_x; }
-            if (ty) { nextalpha = alpha_y; alpha_y += step_y; }
-            if (tz) { nextalpha = alpha_z; alpha_z += step_z; }
+            // the three plane tests run in sequence and each sees the alphas already advanced in
+            // this step (cudaSiddon.cu:84-95): ties advance every tied plane, and a crossing that
+            // lies between the old and the advanced alpha_x can merge into this segment
+            if (vx && (!vy || alpha_x <= alpha_y) && (!vz || alpha_x <= alpha_z)) { nextalpha = alpha_x; alpha_x += step_x; }
+            if (vy && (!vx || alpha_y <= alpha_x) && (!vz || alpha_y <= alpha_z)) { nextalpha = alpha_y; alpha_y += step_y; }
+            if (vz && (!vx || alpha_z <= alpha_x) && (!vy || alpha_z <= alpha_y)) { nextalpha = alpha_z; alpha_z += step_z; }
             float inter = fabsf(d12 * (nextalpha - alpha));
             if (inter >= 0.001f) {
                 float am = (nextalpha + alpha) * 0.5f;
@@ -168,7 +189,7 @@ struct BeamDev {
     float iso_dist;  // |iso - src|
     int fmx, fmy;
     float bsx, bsy;
-    m33 beam_fwd;  // BEV <- PVCS rotation
+    Rot2 beam_fwd;  // BEV <- PVCS rotation
 };
 
 constexpr int kMaxSpec = 30;
@@ -187,7 +208,7 @@ CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b
     v3 projection = dot3(diff, b.dir_n) * b.dir_n;
     v3 rejection = diff - projection;
     float trad = len3(rejection) * fabsf(b.iso_dist / len3(projection));
-    v3 pf = b.beam_fwd(trad * norm3(rejection));
+    v3 pf = rot2(b.beam_fwd, trad * norm3(rejection));
     float fx = (pf.x / b.bsx) + ((float)b.fmx * 0.5f);
     float fz = (pf.z / b.bsy) + ((float)b.fmy * 0.5f);
     // a NaN coordinate (voxel on the central axis) passes these tests, as in the reference

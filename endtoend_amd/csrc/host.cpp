@@ -75,6 +75,35 @@ dm33 dm_beamlet_inv(float theta, float phi) {
 }
 dm33 dm_beamlet_fwd(float theta, float phi) { return dm_transpose(dm_beamlet_inv(theta, phi)); }
 
+static Rot2 make_rot2(v3 a1, float t1, v3 a2, float t2) {
+    Rot2 r;
+    r.a1 = a1;
+    r.a2 = a2;
+    sincosf(t1, &r.s1, &r.c1);
+    sincosf(t2, &r.s2, &r.c2);
+    return r;
+}
+Rot2 rot2_beam_fwd(float theta, float phi, float coll) {
+    float s, c;
+    sincosf(-phi, &s, &c);
+    return make_rot2(mk3(s, 0.f, c), -theta, mk3(0.f, 1.f, 0.f), phi + coll);
+}
+Rot2 rot2_beam_inv(float theta, float phi, float coll) {
+    float s, c;
+    sincosf(-phi, &s, &c);
+    return make_rot2(mk3(0.f, 1.f, 0.f), -(phi + coll), mk3(s, 0.f, c), theta);
+}
+Rot2 rot2_kernel_fwd(float theta, float phi) {
+    float s, c;
+    sincosf(phi, &s, &c);
+    return make_rot2(mk3(c, 0.f, -s), -theta, mk3(0.f, 1.f, 0.f), -phi);
+}
+Rot2 rot2_kernel_inv(float theta, float phi) {
+    float s, c;
+    sincosf(phi, &s, &c);
+    return make_rot2(mk3(0.f, 1.f, 0.f), phi, mk3(c, 0.f, -s), theta);
+}
+
 namespace {
 
 // make_cumulative_kernel (DoseCalcIO/kernel.cpp:260-431): aggregate the 48 polar bins of the

@@ -924,6 +924,13 @@ int orc_beam_dose(const cccs_problem* p, const cccs_beam* beam, const float* fma
     return CCCS_OK;
 }
 
+int orc_rev_to_pvcs(const cccs_problem* p, const cccs_beam* beam, const cccs_beam_geometry* g, int ray,
+                    const float* rev_dose, float* dose) {
+    if (!problem_ok(p) || !beam || !g || !rev_dose || !dose || ray < 0 || ray >= g->nrays) return CCCS_EINVAL;
+    beam_rev_to_pvcs(p, *beam, *g, ray, rev_dose, dose, nullptr);
+    return CCCS_OK;
+}
+
 int orc_row_convolve(const float* dens, const float* terma, int n, const float* kernel, int ntheta, int nradii,
                      const float* radii, int P, float delr, int nphi, float* out) {
     for (int t = 0; t < n; t++)

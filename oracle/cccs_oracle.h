@@ -67,6 +67,11 @@ int orc_beam_dose(const cccs_problem* p, const cccs_beam* beam, const float* fma
 int orc_beam_cone_stage(const cccs_problem* p, const cccs_beam* beam, const float* fmap, int ray,
                         float* rev_dens, float* rev_terma, float* rev_dose);
 
+/* A7 alone: REV2XYZdose of a caller-supplied REV dose grid of cone `ray` (REV-grid sized, line
+ * fastest) into dose (full volume, +=), with the geometry from orc_beam_geometry. */
+int orc_rev_to_pvcs(const cccs_problem* p, const cccs_beam* beam, const cccs_beam_geometry* geom, int ray,
+                    const float* rev_dose, float* dose);
+
 /* A6 on one line (RowConvolve body, dosecalc-beam/nvbbRayConvolve_device.cu:113-217):
  * dens/terma of length n, kernel row P forward / ntheta-1-P backward; radii = KERN_RADII. */
 int orc_row_convolve(const float* dens, const float* terma, int n, const float* kernel, int ntheta,

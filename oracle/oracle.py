@@ -56,6 +56,7 @@ def load() -> C.CDLL:
         "orc_beam_geometry": [P, B, _fp, G],
         "orc_beam_dose": [P, B, _fp, _fp, C.POINTER(Stats)],
         "orc_beam_cone_stage": [P, B, _fp, C.c_int, _fp, _fp, _fp],
+        "orc_rev_to_pvcs": [P, B, G, C.c_int, _fp, _fp],
         "orc_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
         "orc_pack_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
         "orc_beamlet_dense": [P, B, _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int), C.POINTER(Stats)],
@@ -163,6 +164,14 @@ def cone_stage(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, ray: int, re
     _chk(load().orc_beam_cone_stage(C.byref(p), C.byref(beam), fp(np.ascontiguousarray(fmap, np.float32)), ray, fp(d),
                                     fp(t), fp(o)), "cone_stage")
     return d.reshape(shape), t.reshape(shape), o.reshape(shape)
+
+
+def rev_to_pvcs(spec: ProblemSpec, beam: abi.Beam, geom: abi.BeamGeometry, ray: int, rev_dose: np.ndarray):
+    p = spec.struct()
+    dose = np.zeros(spec.density.shape, np.float32)
+    _chk(load().orc_rev_to_pvcs(C.byref(p), C.byref(beam), C.byref(geom), ray,
+                                fp(np.ascontiguousarray(rev_dose, np.float32)), fp(dose)), "rev_to_pvcs")
+    return dose
 
 
 def row_convolve(dens, terma_line, kernel, radii, P, delr, nphi, packed=False):
