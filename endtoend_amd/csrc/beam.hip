@@ -338,98 +338,164 @@ struct ConeArgs {
     int bbx, bby, bbz;        // calc_bbox size
 };
 
-// Walks one direction along the line from destination i, accumulating the cumulative-kernel
-// differences weighted by TERMA (the body of RowConvolve's do/while loops).
-__device__ __forceinline__ float walk(const float* __restrict__ sD, const float* __restrict__ sT, int i, int step,
-                                      int n, const float* __restrict__ R, const float* __restrict__ IW,
-                                      const float* __restrict__ row, int nr, float delr, uint32_t& iters) {
-    float sum = 0.f, r = 0.f, last = 0.f;
-    int rs = 0;
-    float lo = 0.f, hi = R[0], klo = 0.f, khi = row[0], iw = IW[0];
+// The line's samples live in LDS as {half step, TERMA}: x = 0.5*delr*density (negative marks a
+// sample outside the crop box or the calc_bbox), y = TERMA.
+//
+// walk_both runs the two walks of destination j (RowConvolve's do/while loops) back to back over
+// the line's valid segment [vlo, vhi]: first toward lower indices with the forward kernel row,
+// then toward higher indices with the backward row.  Every destination takes exactly vhi - vlo
+// steps, so the trip count is the same on every lane of a wave (a scalar loop, no divergence);
+// the direction switch and the reference's early exits (radiological depth beyond the last radius,
+// reaching the last radial interval) become per-lane flags.  The cumulative kernel is linear
+// between two radii, so a lane keeps its current interval in registers and reads the LDS tables
+// only when the radiological depth crosses a radius.
+__device__ __forceinline__ float walk_both(const float2* __restrict__ s, int j, int vlo, int vhi,
+                                           const float* __restrict__ R, const float* __restrict__ IW,
+                                           const float* __restrict__ rowF, const float* __restrict__ rowB, int nr,
+                                           bool active, uint32_t& iters) {
     const float rmax = R[nr - 1];
-    for (int k = i + step; k >= 0 && k < n; k += step) {
-        const float d = sD[k];
-        if (d < 0.f) break;  // left the crop-box / calc_bbox segment (TERMA is 0 beyond it)
-        iters++;
-        const float de = delr * d;
-        r += 0.5f * de;
-        if (r > rmax) break;
-        while (hi < r && rs < nr - 1) {
-            rs++;
-            lo = hi;
-            hi = R[rs];
-            klo = khi;
-            khi = row[rs];
-            iw = IW[rs];
+    const int total = vhi - vlo;
+    const int nfwd = j - vlo;
+    const float* row = rowF;
+    float r = 0.f, last = 0.f, lo = 0.f, hi = R[0], klo = 0.f, slope = rowF[0] * IW[0], sum = 0.f;
+    int rs = 0, k = j, dir = -1;
+    bool alive = active && nfwd > 0;
+    for (int it = 0; it < total; ++it) {
+        if (it == nfwd) {  // switch to the backward walk
+            dir = 1;
+            k = j;
+            row = rowB;
+            r = 0.f; last = 0.f; lo = 0.f; hi = R[0]; klo = 0.f; slope = rowB[0] * IW[0];
+            rs = 0;
+            alive = active;
         }
-        const float cum = klo + ((r - lo) * iw) * (khi - klo);
-        sum += (cum - last) * sT[k];
+        k += dir;
+        const float2 e = s[k];
+        r += e.x;
+        bool ok = alive && e.x >= 0.f && r <= rmax;
+        if (ok && hi < r) {
+            do {
+                rs++;
+                lo = hi;
+                hi = R[rs];
+                klo = row[rs - 1];
+                slope = (row[rs] - klo) * IW[rs];
+            } while (hi < r && rs < nr - 1);
+        }
+        const float cum = klo + (r - lo) * slope;
+        sum += ok ? (cum - last) * e.y : 0.f;
         last = cum;
-        r += 0.5f * de;
-        if (rs >= nr - 1) break;
+        r += e.x;
+        iters += ok ? 1u : 0u;
+        alive = ok && rs < nr - 1;
     }
     return sum;
 }
 
-// One workgroup per REV line of any cone.  Phase 1 resamples density and TERMA at the line's
-// samples into LDS; phase 2 runs both walks for every valid sample from LDS and stores the
-// line's cone dose contiguously.
-__global__ void __launch_bounds__(256) cone_kernel(ConeArgs a) {
-    __shared__ float sD[kMaxLine];
-    __shared__ float sT[kMaxLine];
-    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
-    __shared__ int s_cone;
-    const uint64_t line = blockIdx.x;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = a.ncones - 1;
-        while (lo < hi) {  // last cone whose first line <= this line
-            int mid = (lo + hi + 1) >> 1;
-            if (a.cones[mid].line0 <= line) lo = mid; else hi = mid - 1;
-        }
-        s_cone = lo;
+// Narrows [t0, t1] to the parameters where o + t*d lies in [lo, hi) (widened by a margin that
+// covers FP32 rounding; the exact per-sample test decides membership).
+__device__ __forceinline__ void clip_slab(float o, float d, float lo, float hi, float& t0, float& t1) {
+    if (fabsf(d) < 1e-3f) {
+        // the coordinate moves by less than one index unit over a whole line (<= 1024 samples)
+        if (!(o >= lo - 2.f && o < hi + 2.f)) { t0 = 1.f; t1 = 0.f; }
+        return;
     }
-    __syncthreads();
-    const ConeDev& c = a.cones[s_cone];
+    float ta = (lo - o) / d, tb = (hi - o) / d;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+}
+
+// One wave per REV line of any cone.  The line's parameter interval inside the crop box and the
+// calc_bbox is found analytically; only samples in it are resampled (density and TERMA, trilinear)
+// into LDS, with the exact per-sample test of cudaRaytrace; then every valid sample runs both
+// walks from LDS and the line's cone dose is stored contiguously (zeros outside the interval).
+__global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
+    extern __shared__ float2 s_line[];
+    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    __shared__ int s_lim[2];
+    const uint64_t line = blockIdx.x;
+    const int tid = threadIdx.x;
+    int lo = 0, hi = a.ncones - 1;
+    while (lo < hi) {  // last cone whose first line <= this line (same on every lane)
+        int mid = (lo + hi + 1) >> 1;
+        if (a.cones[mid].line0 <= line) lo = mid; else hi = mid - 1;
+    }
+    const ConeDev& c = a.cones[lo];
     const int nr = a.nradii;
-    if (threadIdx.x < nr) {
+    if (tid < nr) {
         const float* K = a.tables;
         const float* R = a.tables + (size_t)a.ntheta * nr;
-        sR[threadIdx.x] = R[threadIdx.x];
-        sIW[threadIdx.x] = R[nr + threadIdx.x];
-        sF[threadIdx.x] = K[(size_t)c.rowf * nr + threadIdx.x];
-        sB[threadIdx.x] = K[(size_t)c.rowb * nr + threadIdx.x];
+        sR[tid] = R[tid];
+        sIW[tid] = R[nr + tid];
+        sF[tid] = K[(size_t)c.rowf * nr + tid];
+        sB[tid] = K[(size_t)c.rowb * nr + tid];
     }
     const uint64_t l = line - c.line0;
     const int tY = (int)(l % (uint64_t)c.ny);
     const int tZ = (int)(l / (uint64_t)c.ny);
     const int n = c.nx;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        float d = -1.f, t = 0.f;
-        v3 bev = c.kinv(mk3(c.rs.x + (float)tZ, c.rs.y + (float)i, c.rs.z + (float)tY));
-        if (bev.x >= 0.f && bev.y >= 0.f && bev.z >= 0.f && bev.x < a.bev_size.x && bev.y < a.bev_size.y &&
-            bev.z < a.bev_size.z) {
-            v3 g = a.binv(bev) + a.off;
-            if (g.x >= a.bb0.x && g.x < a.bb1.x && g.y >= a.bb0.y && g.y < a.bb1.y && g.z >= a.bb0.z && g.z < a.bb1.z) {
-                d = tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, g.x + 0.5f, g.y + 0.5f, g.z + 0.5f);
-                v3 tq = g - a.bb0;
-                t = tex3_border(a.terma, a.bbx, a.bby, a.bbz, tq.x + 0.5f, tq.y + 0.5f, tq.z + 0.5f);
-            }
-        }
-        sD[i] = d;
-        sT[i] = t;
-    }
-    __syncthreads();
     float* out = a.rev + c.off + (size_t)n * ((size_t)tY + (size_t)c.ny * tZ);
-    const float nphi_f = (float)a.nphi;
+    // BEV and PVCS positions along the line are affine in the sample index
+    const v3 p0 = mk3(c.rs.x + (float)tZ, c.rs.y, c.rs.z + (float)tY);
+    const v3 b0 = c.kinv(p0);
+    const v3 kd = mk3(c.kinv.m[1], c.kinv.m[4], c.kinv.m[7]);
+    const v3 g0 = a.binv(b0) + a.off;
+    const v3 gd = a.binv(kd);
+    float t0 = 0.f, t1 = (float)n;
+    clip_slab(b0.x, kd.x, 0.f, a.bev_size.x, t0, t1);
+    clip_slab(b0.y, kd.y, 0.f, a.bev_size.y, t0, t1);
+    clip_slab(b0.z, kd.z, 0.f, a.bev_size.z, t0, t1);
+    clip_slab(g0.x, gd.x, a.bb0.x, a.bb1.x, t0, t1);
+    clip_slab(g0.y, gd.y, a.bb0.y, a.bb1.y, t0, t1);
+    clip_slab(g0.z, gd.z, a.bb0.z, a.bb1.z, t0, t1);
+    int i0 = 0, i1 = 0;
+    if (t0 <= t1) {
+        i0 = max(0, (int)floorf(t0) - 2);
+        i1 = min(n, (int)ceilf(t1) + 3);
+    }
+    for (int i = tid; i < n; i += 64)
+        if (i < i0 || i >= i1) out[i] = 0.f;
     uint32_t iters = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        float v = 0.f;
-        if (sD[i] >= 0.f) {
-            float s = walk(sD, sT, i, -1, n, sR, sIW, sF, nr, a.delr, iters);
-            s += walk(sD, sT, i, +1, n, sR, sIW, sB, nr, a.delr, iters);
-            v = s / nphi_f;
+    if (i0 < i1) {
+        const int m = i1 - i0;
+        const float halfstep = 0.5f;
+        for (int j = tid; j < m; j += 64) {
+            const int i = i0 + j;
+            float2 e = make_float2(-1.f, 0.f);
+            v3 bev = c.kinv(mk3(c.rs.x + (float)tZ, c.rs.y + (float)i, c.rs.z + (float)tY));
+            if (bev.x >= 0.f && bev.y >= 0.f && bev.z >= 0.f && bev.x < a.bev_size.x && bev.y < a.bev_size.y &&
+                bev.z < a.bev_size.z) {
+                v3 g = a.binv(bev) + a.off;
+                if (g.x >= a.bb0.x && g.x < a.bb1.x && g.y >= a.bb0.y && g.y < a.bb1.y && g.z >= a.bb0.z &&
+                    g.z < a.bb1.z) {
+                    float d = tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, g.x + 0.5f, g.y + 0.5f, g.z + 0.5f);
+                    v3 tq = g - a.bb0;
+                    e.x = halfstep * (a.delr * d);
+                    e.y = tex3_border(a.terma, a.bbx, a.bby, a.bbz, tq.x + 0.5f, tq.y + 0.5f, tq.z + 0.5f);
+                }
+            }
+            s_line[j] = e;
         }
-        out[i] = v;
+        __syncthreads();
+        // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
+        if (tid == 0) { s_lim[0] = m; s_lim[1] = -1; }
+        __syncthreads();
+        for (int j = tid; j < m; j += 64)
+            if (s_line[j].x >= 0.f) {
+                atomicMin(&s_lim[0], j);
+                atomicMax(&s_lim[1], j);
+            }
+        __syncthreads();
+        const int vlo = s_lim[0], vhi = s_lim[1];
+        const float nphi_f = (float)a.nphi;
+        for (int j = tid; j < m; j += 64)
+            if (j < vlo || j > vhi) out[i0 + j] = 0.f;
+        for (int base = vlo; base <= vhi; base += 64) {
+            const int j = base + tid;
+            const bool act = j <= vhi && s_line[min(j, vhi)].x >= 0.f;
+            const float v = walk_both(s_line, min(j, vhi), vlo, vhi, sR, sIW, sF, sB, nr, act, iters);
+            if (j <= vhi) out[i0 + j] = act ? v / nphi_f : 0.f;
+        }
     }
     if (a.counts) block_count(iters, a.counts, blockIdx.x);
 }
@@ -675,7 +741,9 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     if (g.total_lines > 0x7fffffffULL) return set_error(CCCS_EGEOM, "too many REV lines");
     ca.counts = prof_counter(ctx, g.total_lines);
     int tok = prof_begin(ctx, K_CONE);
-    hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(256), 0, ctx->stream, ca);
+    uint32_t maxn = 1;
+    for (int r = 0; r < g.nrays; r++) maxn = std::max(maxn, g.rev_size[r][0]);
+    hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), sizeof(float2) * maxn, ctx->stream, ca);
     prof_end(ctx, tok, 12.0 * (double)g.total_rev, 0.0, g.total_rev);
     CCCS_HIP(hipGetLastError());
     if (ca.counts) prof_fold(ctx, K_CONE, g.total_lines);
