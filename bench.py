@@ -84,6 +84,23 @@ def cpu_baseline(w, beams_idx, d_gpu_of):
             "sample": f"{len(beams_idx)} of {len(w.beams)} beams ({nb} beamlet-doses), {dt:.1f} s"}, max(errs)
 
 
+def copy_calibration(w, mib: int, reps: int = 5):
+    """Streams mib MiB through the library's copy kernel `reps` times (reads = writes = mib MiB per
+    launch), so rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    from endtoend_amd.engine import Engine
+    eng = Engine(w.spec)
+    n = mib << 20
+    a, b = eng.dev_alloc(n), eng.dev_alloc(n)
+    eng.dev_zero(a, n)
+    for _ in range(reps):
+        eng.dev_copy(b, a, n)
+    eng.sync()
+    eng.dev_free(a)
+    eng.dev_free(b)
+    eng.close()
+    print(json.dumps({"copy_calibration_mib": mib, "launches": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +108,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-beams", type=int, default=2, help="beams in the oracle sample (0 = skip)")
     ap.add_argument("--latency-reps", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run the streaming copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (beams shard across GPUs "
@@ -101,6 +120,9 @@ def main():
     from endtoend_amd.engine import Engine
 
     w = W.cfg2()
+    if args.copy_mib > 0:
+        copy_calibration(w, args.copy_mib)
+        return
     beams = w.beams
     nv = w.spec.density.size
     fl = np.concatenate([f.ravel() for f in w.fmaps]).astype(np.float32)

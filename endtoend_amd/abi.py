@@ -139,6 +139,7 @@ EXPORTS = [
     ("cccs_dev_upload", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("cccs_dev_download", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("cccs_dev_zero", C.c_int, [_P, _P, C.c_size_t]),
+    ("cccs_dev_copy", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("cccs_ctx_sync", C.c_int, [_P]),
     ("cccs_ctx_profile", C.c_int, [_P, C.c_int]),
     ("cccs_ctx_profile_reset", C.c_int, [_P]),

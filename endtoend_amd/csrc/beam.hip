@@ -1087,6 +1087,21 @@ extern "C" int cccs_dev_zero(cccs_ctx* ctx, void* ptr, size_t bytes) {
     CCCS_HIP(hipMemsetAsync(ptr, 0, bytes, ctx->stream));
     return CCCS_OK;
 }
+__global__ void __launch_bounds__(256) copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+extern "C" int cccs_dev_copy(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    clear_error();
+    if (!ctx || !dst || !src || bytes % 16 != 0) return set_error(CCCS_EINVAL, "bad copy arguments");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    size_t n = bytes / 16;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    if (n) hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (float4*)dst, (const float4*)src, n);
+    CCCS_HIP(hipGetLastError());
+    return CCCS_OK;
+}
+
 extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
     clear_error();
     if (!ctx) return set_error(CCCS_EINVAL, "null argument");

@@ -215,6 +215,9 @@ class Engine:
     def dev_zero(self, ptr: int, nbytes: int):
         abi.check(self.lib.cccs_dev_zero(self.ctx, C.c_void_p(ptr), nbytes), "cccs_dev_zero")
 
+    def dev_copy(self, dst: int, src: int, nbytes: int):
+        abi.check(self.lib.cccs_dev_copy(self.ctx, C.c_void_p(dst), C.c_void_p(src), nbytes), "cccs_dev_copy")
+
     def sync(self):
         abi.check(self.lib.cccs_ctx_sync(self.ctx), "cccs_ctx_sync")
 

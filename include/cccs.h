@@ -195,6 +195,9 @@ int cccs_dev_free(cccs_ctx* ctx, void* ptr);
 int cccs_dev_upload(cccs_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cccs_dev_download(cccs_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cccs_dev_zero(cccs_ctx* ctx, void* ptr, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (bytes must be a multiple of 16); also the
+ * calibration kernel for HBM byte counters. */
+int cccs_dev_copy(cccs_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Waits for all work the context has enqueued. */
 int cccs_ctx_sync(cccs_ctx* ctx);
 
