@@ -19,8 +19,8 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcccs.so")
 
-SOURCES = ["host.cpp", "beam.hip", "beamlet.hip"]
-HEADERS = ["cccs_internal.hpp", "context.hpp", "errors.hpp"]
+SOURCES = ["host.cpp", "context.hip", "beam.hip", "beamlet.hip"]
+HEADERS = ["cccs_internal.hpp", "context.hpp", "device_common.hpp", "errors.hpp"]
 
 ARCH = os.environ.get("CCCS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

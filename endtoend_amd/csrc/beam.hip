@@ -26,129 +26,9 @@
 #include <vector>
 
 #include "context.hpp"
+#include "device_common.hpp"
 
 namespace cccs {
-
-// ------------------------------------------------------------------------------------------------
-// buffers
-int DevBuf::ensure(size_t need) {
-    if (need <= bytes) return CCCS_OK;
-    if (p) {
-        (void)hipDeviceSynchronize();  // queued kernels may still read the old buffer
-        (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    size_t want = std::max(need, bytes + bytes / 4);
-    if (hipMalloc(&p, want) != hipSuccess) {
-        p = nullptr;
-        (void)hipGetLastError();
-        return set_error(CCCS_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
-    }
-    bytes = want;
-    return CCCS_OK;
-}
-void DevBuf::release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-}
-int HostBuf::ensure(size_t need) {
-    if (need <= bytes) return CCCS_OK;
-    if (p) { (void)hipHostFree(p); p = nullptr; bytes = 0; }
-    size_t want = std::max(need, bytes + bytes / 4);
-    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
-        p = nullptr;
-        (void)hipGetLastError();
-        return set_error(CCCS_ENOMEM, "hipHostMalloc failed");
-    }
-    bytes = want;
-    return CCCS_OK;
-}
-void HostBuf::release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    bytes = 0;
-}
-
-const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_terma", "bl_raytrace",
-                                            "bl_cone", "bl_rev2bev", "bl_unpack", "bl_compact"};
-
-// Sums one uint32 per thread over the workgroup (LDS tree) and stores it at part[block].
-__device__ __forceinline__ void block_count(uint32_t v, uint32_t* __restrict__ part, size_t block) {
-    __shared__ uint32_t s_cnt[256];
-    const int t = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
-    const int nt = blockDim.x * blockDim.y * blockDim.z;
-    s_cnt[t] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w && t + w < nt) s_cnt[t] += s_cnt[t + w];
-        __syncthreads();
-    }
-    if (t == 0) part[block] = s_cnt[0];
-}
-
-__global__ void __launch_bounds__(256) count_fold_kernel(const uint32_t* __restrict__ part, size_t n,
-                                                         uint64_t* __restrict__ acc) {
-    __shared__ uint64_t s[256];
-    uint64_t v = 0;
-    for (size_t i = threadIdx.x; i < n; i += 256) v += part[i];
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) acc[0] += s[0];
-}
-
-int prof_begin(cccs_ctx* c, int kind) {
-    Profiler& P = c->prof;
-    if (!P.on) return -1;
-    if (P.used + 2 > P.pool.size()) {
-        for (int i = 0; i < 64; i++) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return -1;
-            P.pool.push_back(e);
-        }
-    }
-    ProfRec r;
-    r.kind = kind;
-    r.a = P.pool[P.used++];
-    r.b = P.pool[P.used++];
-    r.bytes = r.flops = 0;
-    r.items = 0;
-    (void)hipEventRecord(r.a, c->stream);
-    P.recs.push_back(r);
-    return (int)P.recs.size() - 1;
-}
-void prof_end(cccs_ctx* c, int token, double bytes, double flops, uint64_t items) {
-    if (token < 0) return;
-    ProfRec& r = c->prof.recs[token];
-    (void)hipEventRecord(r.b, c->stream);
-    r.bytes = bytes;
-    r.flops = flops;
-    r.items = items;
-}
-uint32_t* prof_counter(cccs_ctx* c, size_t blocks) {
-    Profiler& P = c->prof;
-    if (!P.on) return nullptr;
-    if (blocks > P.part_cap) {
-        // the previous array may still be read by queued kernels: wait before replacing it
-        (void)hipStreamSynchronize(c->stream);
-        if (P.d_part) (void)hipFree(P.d_part);
-        P.d_part = nullptr;
-        size_t cap = std::max(blocks, (size_t)1 << 16);
-        if (hipMalloc((void**)&P.d_part, cap * sizeof(uint32_t)) != hipSuccess) { P.part_cap = 0; return nullptr; }
-        P.part_cap = cap;
-    }
-    return P.d_part;
-}
-void prof_fold(cccs_ctx* c, int kind, size_t blocks) {
-    Profiler& P = c->prof;
-    if (!P.on || !P.d_part || !P.d_counts) return;
-    hipLaunchKernelGGL(count_fold_kernel, dim3(1), dim3(256), 0, c->stream, P.d_part, blocks, P.d_counts + kind);
-}
 
 BeamDev make_beam_dev(const cccs_beam& b) {
     BeamDev d;
@@ -325,11 +205,13 @@ struct ConeArgs {
     const float* terma;
     const float* tables;      // kernel | radii | inverse widths
     const ConeDev* cones;
-    float* rev;               // REV dose output, all cones back to back
+    LineMeta* lines;          // one entry per REV line of every cone
+    float* rev;               // compact REV dose store
+    uint64_t nlines;
     int ncones;
     int ntheta, nradii, nphi;
     float delr;
-    uint32_t* counts;         // per-line walk iteration counts (profiling) or nullptr
+    uint32_t* counts;         // per-workgroup walk iteration counts (profiling) or nullptr
     VolDesc vd;
     m33 binv;                 // PVCS <- BEV rotation (index units)
     v3 off;                   // (bevStart - start) / voxel
@@ -338,104 +220,22 @@ struct ConeArgs {
     int bbx, bby, bbz;        // calc_bbox size
 };
 
-// The line's samples live in LDS as {half step, TERMA}: x = 0.5*delr*density (negative marks a
-// sample outside the crop box or the calc_bbox), y = TERMA.
-//
-// walk_both runs the two walks of destination j (RowConvolve's do/while loops) back to back over
-// the line's valid segment [vlo, vhi]: first toward lower indices with the forward kernel row,
-// then toward higher indices with the backward row.  Every destination takes exactly vhi - vlo
-// steps, so the trip count is the same on every lane of a wave (a scalar loop, no divergence);
-// the direction switch and the reference's early exits (radiological depth beyond the last radius,
-// reaching the last radial interval) become per-lane flags.  The cumulative kernel is linear
-// between two radii, so a lane keeps its current interval in registers and reads the LDS tables
-// only when the radiological depth crosses a radius.
-__device__ __forceinline__ float walk_both(const float2* __restrict__ s, int j, int vlo, int vhi,
-                                           const float* __restrict__ R, const float* __restrict__ IW,
-                                           const float* __restrict__ rowF, const float* __restrict__ rowB, int nr,
-                                           bool active, uint32_t& iters) {
-    const float rmax = R[nr - 1];
-    const int total = vhi - vlo;
-    const int nfwd = j - vlo;
-    const float* row = rowF;
-    float r = 0.f, last = 0.f, lo = 0.f, hi = R[0], klo = 0.f, slope = rowF[0] * IW[0], sum = 0.f;
-    int rs = 0, k = j, dir = -1;
-    bool alive = active && nfwd > 0;
-    for (int it = 0; it < total; ++it) {
-        if (it == nfwd) {  // switch to the backward walk
-            dir = 1;
-            k = j;
-            row = rowB;
-            r = 0.f; last = 0.f; lo = 0.f; hi = R[0]; klo = 0.f; slope = rowB[0] * IW[0];
-            rs = 0;
-            alive = active;
-        }
-        k += dir;
-        const float2 e = s[k];
-        r += e.x;
-        bool ok = alive && e.x >= 0.f && r <= rmax;
-        if (ok && hi < r) {
-            do {
-                rs++;
-                lo = hi;
-                hi = R[rs];
-                klo = row[rs - 1];
-                slope = (row[rs] - klo) * IW[rs];
-            } while (hi < r && rs < nr - 1);
-        }
-        const float cum = klo + (r - lo) * slope;
-        sum += ok ? (cum - last) * e.y : 0.f;
-        last = cum;
-        r += e.x;
-        iters += ok ? 1u : 0u;
-        alive = ok && rs < nr - 1;
-    }
-    return sum;
-}
-
-// Narrows [t0, t1] to the parameters where o + t*d lies in [lo, hi) (widened by a margin that
-// covers FP32 rounding; the exact per-sample test decides membership).
-__device__ __forceinline__ void clip_slab(float o, float d, float lo, float hi, float& t0, float& t1) {
-    if (fabsf(d) < 1e-3f) {
-        // the coordinate moves by less than one index unit over a whole line (<= 1024 samples)
-        if (!(o >= lo - 2.f && o < hi + 2.f)) { t0 = 1.f; t1 = 0.f; }
-        return;
-    }
-    float ta = (lo - o) / d, tb = (hi - o) / d;
-    t0 = fmaxf(t0, fminf(ta, tb));
-    t1 = fminf(t1, fmaxf(ta, tb));
-}
-
-// One wave per REV line of any cone.  The line's parameter interval inside the crop box and the
-// calc_bbox is found analytically; only samples in it are resampled (density and TERMA, trilinear)
-// into LDS, with the exact per-sample test of cudaRaytrace; then every valid sample runs both
-// walks from LDS and the line's cone dose is stored contiguously (zeros outside the interval).
-__global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
-    extern __shared__ float2 s_line[];
-    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
-    __shared__ int s_lim[2];
-    const uint64_t line = blockIdx.x;
-    const int tid = threadIdx.x;
-    int lo = 0, hi = a.ncones - 1;
-    while (lo < hi) {  // last cone whose first line <= this line (same on every lane)
+__device__ __forceinline__ int cone_of_line(const ConeDev* __restrict__ cones, int ncones, uint64_t line) {
+    int lo = 0, hi = ncones - 1;
+    while (lo < hi) {  // last cone whose first line <= line
         int mid = (lo + hi + 1) >> 1;
-        if (a.cones[mid].line0 <= line) lo = mid; else hi = mid - 1;
+        if (cones[mid].line0 <= line) lo = mid; else hi = mid - 1;
     }
-    const ConeDev& c = a.cones[lo];
-    const int nr = a.nradii;
-    if (tid < nr) {
-        const float* K = a.tables;
-        const float* R = a.tables + (size_t)a.ntheta * nr;
-        sR[tid] = R[tid];
-        sIW[tid] = R[nr + tid];
-        sF[tid] = K[(size_t)c.rowf * nr + tid];
-        sB[tid] = K[(size_t)c.rowb * nr + tid];
-    }
-    const uint64_t l = line - c.line0;
-    const int tY = (int)(l % (uint64_t)c.ny);
-    const int tZ = (int)(l / (uint64_t)c.ny);
+    return lo;
+}
+
+// Sample interval [i0, i1) of REV line (tY, tZ) of cone c that can lie inside the crop box and the
+// calc_bbox: the BEV and PVCS positions along a line are affine in the sample index, so each box
+// clips the line to a slab interval.  Widened by 2/3 samples; the exact per-sample test of
+// cudaRaytrace (nvbbRayConvolve_device.cu:50-70) still decides membership.
+__device__ __forceinline__ void line_interval(const ConeArgs& a, const ConeDev& c, int tY, int tZ, int& i0,
+                                              int& i1) {
     const int n = c.nx;
-    float* out = a.rev + c.off + (size_t)n * ((size_t)tY + (size_t)c.ny * tZ);
-    // BEV and PVCS positions along the line are affine in the sample index
     const v3 p0 = mk3(c.rs.x + (float)tZ, c.rs.y, c.rs.z + (float)tY);
     const v3 b0 = c.kinv(p0);
     const v3 kd = mk3(c.kinv.m[1], c.kinv.m[4], c.kinv.m[7]);
@@ -448,17 +248,60 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
     clip_slab(g0.x, gd.x, a.bb0.x, a.bb1.x, t0, t1);
     clip_slab(g0.y, gd.y, a.bb0.y, a.bb1.y, t0, t1);
     clip_slab(g0.z, gd.z, a.bb0.z, a.bb1.z, t0, t1);
-    int i0 = 0, i1 = 0;
+    i0 = 0;
+    i1 = 0;
     if (t0 <= t1) {
         i0 = max(0, (int)floorf(t0) - 2);
         i1 = min(n, (int)ceilf(t1) + 3);
+        if (i1 < i0) i1 = i0;
     }
-    for (int i = tid; i < n; i += 64)
-        if (i < i0 || i >= i1) out[i] = 0.f;
+}
+
+// Pass 1 of the cone pipeline: the stored interval of every REV line (offsets come from a scan).
+__global__ void __launch_bounds__(256) line_interval_kernel(ConeArgs a) {
+    const uint64_t line = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (line >= a.nlines) return;
+    const int q = cone_of_line(a.cones, a.ncones, line);
+    const ConeDev& c = a.cones[q];
+    const uint64_t l = line - c.line0;
+    int i0, i1;
+    line_interval(a, c, (int)(l % (uint64_t)c.ny), (int)(l / (uint64_t)c.ny), i0, i1);
+    LineMeta m;
+    m.i0 = i0;
+    m.len = i1 - i0;
+    m.off = 0;
+    m.pad = 0;
+    a.lines[line] = m;
+}
+
+// Pass 2: one wave per REV line of any cone (XCD-aware order).  The stored interval of the line is
+// resampled (density and TERMA, trilinear) into LDS with the exact per-sample test of
+// cudaRaytrace; then every valid sample runs both collapsed-cone walks from LDS and the interval's
+// cone dose is written to the compact REV store.
+__global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
+    extern __shared__ float2 s_line[];
+    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    __shared__ int s_lim[2];
+    const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    const LineMeta meta = a.lines[line];
     uint32_t iters = 0;
-    if (i0 < i1) {
-        const int m = i1 - i0;
-        const float halfstep = 0.5f;
+    if (meta.len > 0) {
+        const ConeDev& c = a.cones[cone_of_line(a.cones, a.ncones, line)];
+        const int nr = a.nradii;
+        if (tid < nr) {
+            const float* K = a.tables;
+            const float* R = a.tables + (size_t)a.ntheta * nr;
+            sR[tid] = R[tid];
+            sIW[tid] = R[nr + tid];
+            sF[tid] = K[(size_t)c.rowf * nr + tid];
+            sB[tid] = K[(size_t)c.rowb * nr + tid];
+        }
+        const uint64_t l = line - c.line0;
+        const int tY = (int)(l % (uint64_t)c.ny);
+        const int tZ = (int)(l / (uint64_t)c.ny);
+        const int i0 = meta.i0, m = meta.len;
+        float* out = a.rev + meta.off;
         for (int j = tid; j < m; j += 64) {
             const int i = i0 + j;
             float2 e = make_float2(-1.f, 0.f);
@@ -470,14 +313,12 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
                     g.z < a.bb1.z) {
                     float d = tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, g.x + 0.5f, g.y + 0.5f, g.z + 0.5f);
                     v3 tq = g - a.bb0;
-                    e.x = halfstep * (a.delr * d);
+                    e.x = 0.5f * (a.delr * d);
                     e.y = tex3_border(a.terma, a.bbx, a.bby, a.bbz, tq.x + 0.5f, tq.y + 0.5f, tq.z + 0.5f);
                 }
             }
             s_line[j] = e;
         }
-        __syncthreads();
-        // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
         if (tid == 0) { s_lim[0] = m; s_lim[1] = -1; }
         __syncthreads();
         for (int j = tid; j < m; j += 64)
@@ -486,15 +327,16 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
                 atomicMax(&s_lim[1], j);
             }
         __syncthreads();
+        // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
         const int vlo = s_lim[0], vhi = s_lim[1];
         const float nphi_f = (float)a.nphi;
         for (int j = tid; j < m; j += 64)
-            if (j < vlo || j > vhi) out[i0 + j] = 0.f;
+            if (j < vlo || j > vhi) out[j] = 0.f;
         for (int base = vlo; base <= vhi; base += 64) {
             const int j = base + tid;
             const bool act = j <= vhi && s_line[min(j, vhi)].x >= 0.f;
             const float v = walk_both(s_line, min(j, vhi), vlo, vhi, sR, sIW, sF, sB, nr, act, iters);
-            if (j <= vhi) out[i0 + j] = act ? v / nphi_f : 0.f;
+            if (j <= vhi) out[j] = act ? v / nphi_f : 0.f;
         }
     }
     if (a.counts) block_count(iters, a.counts, blockIdx.x);
@@ -502,32 +344,44 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
 
 struct PvcsArgs {
     const ConeDev* cones;
+    const LineMeta* lines;
     const float* rev;
     float* dose;
     int ncones;
     int nx, ny;                 // full volume dims (x, y)
     int x0, y0, z0, sx, sy, sz; // PVCS index box
+    int tx, ty, tz;             // tiles per axis
+    m33 bfwd;                   // BEV <- PVCS rotation (index units)
+    v3 off;                     // BEV origin in PVCS index units
+    v3 bev_lo, bev_hi;          // dilated crop box (BEV index units)
 };
 
-// One lane per PVCS voxel of the reachable box; sums every cone's trilinear REV dose in a
-// register and adds it to the dose volume once.
+// One lane per PVCS voxel, 16x4x4 voxel tiles in XCD-aware order.  Voxels outside the dilated
+// crop box receive nothing from any cone and leave at once; the others sum every cone's trilinear
+// REV dose (compact store, REV2XYZdose's fetch) in a register and add it to the dose volume once.
 __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
     __shared__ ConeDev sc[kMaxRays];
-    const int tid = threadIdx.x + threadIdx.y * 64;
+    const int tid = threadIdx.x;
     for (int q = tid; q < a.ncones; q += 256) sc[q] = a.cones[q];
     __syncthreads();
-    const int X = blockIdx.x * 64 + threadIdx.x;
-    const int Y = blockIdx.y * 4 + threadIdx.y;
-    const int Z = blockIdx.z;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int txi = (int)(tile % (uint32_t)a.tx);
+    const int tyi = (int)((tile / (uint32_t)a.tx) % (uint32_t)a.ty);
+    const int tzi = (int)(tile / ((uint32_t)a.tx * (uint32_t)a.ty));
+    const int X = txi * 16 + (tid & 15), Y = tyi * 4 + ((tid >> 4) & 3), Z = tzi * 4 + (tid >> 6);
     if (X >= a.sx || Y >= a.sy || Z >= a.sz) return;
     const int gx = a.x0 + X, gy = a.y0 + Y, gz = a.z0 + Z;
     const v3 g = mk3((float)gx, (float)gy, (float)gz);
+    const v3 bv = a.bfwd(g - a.off);
+    if (!(bv.x >= a.bev_lo.x && bv.y >= a.bev_lo.y && bv.z >= a.bev_lo.z && bv.x < a.bev_hi.x && bv.y < a.bev_hi.y &&
+          bv.z < a.bev_hi.z))
+        return;
     float sum = 0.f;
     for (int q = 0; q < a.ncones; q++) {
         const ConeDev& c = sc[q];
-        v3 r = c.a(g) + c.t;
+        const v3 r = c.a(g) + c.t;
         if (r.y >= (float)c.nx || r.z >= (float)c.ny || r.x >= (float)c.nz || r.x < 0.f || r.y < 0.f || r.z < 0.f) continue;
-        sum += tex3_border(a.rev + c.off, c.nx, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
+        sum += rev_fetch(a.lines, a.rev, c.line0, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
     }
     if (sum != 0.f) a.dose[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] += sum;
 }
@@ -576,7 +430,6 @@ int launch_terma(cccs_ctx* ctx, const TermaArgs& a0) {
     hipLaunchKernelGGL(terma_kernel, grd, blk, 0, ctx->stream, a);
     prof_end(ctx, tok, 4.0 * nct + 4.0 * nbb, 0.0, (uint64_t)nbb);
     CCCS_HIP(hipGetLastError());
-    if (a.counts) prof_fold(ctx, K_TERMA, blocks);
     return CCCS_OK;
 }
 
@@ -685,9 +538,12 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     if (gout) *gout = g;
     if (!d_dose) return CCCS_OK;
     const cccs_problem& p = ctx->prob;
-    CCCS_TRY(ctx->rev.ensure(sizeof(float) * g.total_rev));
+    if (g.total_lines > 0x7fffffffULL || g.total_rev > 0xffffffffULL)
+        return set_error(CCCS_EGEOM, "REV grids exceed 2^32 samples");
     CCCS_TRY(ctx->cones.ensure(sizeof(ConeDev) * kMaxRays));
     CCCS_TRY(ctx->h_cones.ensure(sizeof(ConeDev) * kMaxRays));
+    CCCS_TRY(ctx->lines.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, g.total_lines)));
+    CCCS_TRY(ctx->h_small.ensure(64));
     ConeDev* hc = ctx->h_cones.as<ConeDev>();
     dm33 bfwd = dm_beam_fwd(b.azimuth, b.zenith, b.coll);
     dm33 binv = dm_beam_inv(b.azimuth, b.zenith, b.coll);
@@ -724,7 +580,9 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     ca.terma = d_terma;
     ca.tables = ctx->d_tables;
     ca.cones = ctx->cones.as<ConeDev>();
-    ca.rev = ctx->rev.as<float>();
+    ca.lines = ctx->lines.as<LineMeta>();
+    ca.rev = nullptr;
+    ca.nlines = g.total_lines;
     ca.ncones = g.nrays;
     ca.ntheta = (int)p.ntheta;
     ca.nradii = (int)p.nradii;
@@ -738,18 +596,33 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     ca.bb0 = ldu3(p.bbox_start);
     ca.bb1 = ca.bb0 + ldu3(p.bbox_size);
     ca.bbx = (int)p.bbox_size[0]; ca.bby = (int)p.bbox_size[1]; ca.bbz = (int)p.bbox_size[2];
-    if (g.total_lines > 0x7fffffffULL) return set_error(CCCS_EGEOM, "too many REV lines");
+    if (g.total_lines == 0) return CCCS_OK;
+    // pass 1: stored interval of every line, then offsets into the compact store
+    hipLaunchKernelGGL(line_interval_kernel, dim3((unsigned)((g.total_lines + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ca);
+    CCCS_HIP(hipGetLastError());
+    CCCS_TRY(launch_line_scan(ctx, ca.lines, g.total_lines));
+    CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, ca.lines + (g.total_lines - 1), sizeof(LineMeta), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    LineMeta last;
+    std::memcpy(&last, ctx->h_small.p, sizeof(LineMeta));
+    const uint64_t stored = (uint64_t)last.off + (uint64_t)last.len;
+    CCCS_TRY(ctx->rev.ensure(sizeof(float) * std::max<uint64_t>(1, stored)));
+    ca.rev = ctx->rev.as<float>();
+    // pass 2: per-line resample + walks into the compact store
     ca.counts = prof_counter(ctx, g.total_lines);
-    int tok = prof_begin(ctx, K_CONE);
     uint32_t maxn = 1;
     for (int r = 0; r < g.nrays; r++) maxn = std::max(maxn, g.rev_size[r][0]);
+    int tok = prof_begin(ctx, K_CONE);
     hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), sizeof(float2) * maxn, ctx->stream, ca);
-    prof_end(ctx, tok, 12.0 * (double)g.total_rev, 0.0, g.total_rev);
+    // algorithmic traffic: density + TERMA in, cone dose out, per stored sample
+    prof_end(ctx, tok, 12.0 * (double)stored, 0.0, stored);
     CCCS_HIP(hipGetLastError());
-    if (ca.counts) prof_fold(ctx, K_CONE, g.total_lines);
     PvcsArgs pa;
     pa.cones = ctx->cones.as<ConeDev>();
-    pa.rev = ctx->rev.as<float>();
+    pa.lines = ca.lines;
+    pa.rev = ca.rev;
     pa.dose = d_dose;
     pa.ncones = g.nrays;
     pa.nx = (int)p.size[0];
@@ -757,11 +630,22 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     pa.x0 = g.pvcs_lo[0]; pa.y0 = g.pvcs_lo[1]; pa.z0 = g.pvcs_lo[2];
     pa.sx = g.pvcs_hi[0] - g.pvcs_lo[0]; pa.sy = g.pvcs_hi[1] - g.pvcs_lo[1]; pa.sz = g.pvcs_hi[2] - g.pvcs_lo[2];
     if (pa.sx > 0 && pa.sy > 0 && pa.sz > 0) {
-        dim3 blk(64, 4, 1), grd((pa.sx + 63) / 64, (pa.sy + 3) / 4, pa.sz);
+        pa.tx = (pa.sx + 15) / 16;
+        pa.ty = (pa.sy + 3) / 4;
+        pa.tz = (pa.sz + 3) / 4;
+        pa.bfwd = to_f(bfwd);
+        pa.off = off;
+        // a voxel gets dose only within the trilinear support (< sqrt(3) index units) of a stored
+        // sample, and stored samples lie inside the crop box
+        pa.bev_lo = mk3(-2.f, -2.f, -2.f);
+        pa.bev_hi = mk3((float)g.bev_size[0] + 2.f, (float)g.bev_size[1] + 2.f, (float)g.bev_size[2] + 2.f);
+        const uint64_t tiles = (uint64_t)pa.tx * pa.ty * pa.tz;
+        if (tiles > 0xffffffffULL) return set_error(CCCS_EGEOM, "PVCS box too large");
         const double nbox = (double)pa.sx * pa.sy * pa.sz;
-        int tok = prof_begin(ctx, K_PVCS);
-        hipLaunchKernelGGL(pvcs_kernel, grd, blk, 0, ctx->stream, pa);
-        prof_end(ctx, tok, 4.0 * (double)g.total_rev + 8.0 * nbox, 0.0, (uint64_t)nbox);
+        int tk = prof_begin(ctx, K_PVCS);
+        hipLaunchKernelGGL(pvcs_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, pa);
+        // algorithmic traffic: every stored sample read once, the box's dose read and written once
+        prof_end(ctx, tk, 4.0 * (double)stored + 8.0 * nbox, 0.0, (uint64_t)nbox);
         CCCS_HIP(hipGetLastError());
     }
     return CCCS_OK;
@@ -788,135 +672,12 @@ int beam_dose_one(cccs_ctx* ctx, const cccs_beam& b, const float* d_fmap, float*
     return run_cones(ctx, b, ctx->terma.as<float>(), d_dose, nullptr);
 }
 
-thread_local std::string g_last_error;
-int set_error(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-void clear_error() { g_last_error.clear(); }
-
 }  // namespace cccs
 
 using namespace cccs;
 
 // =================================================================================================
 // C-ABI
-
-extern "C" const char* cccs_last_error(void) { return g_last_error.c_str(); }
-
-extern "C" int cccs_ctx_create(const cccs_problem* p, int device, cccs_ctx** out) {
-    clear_error();
-    if (!p || !out) return set_error(CCCS_EINVAL, "null argument");
-    *out = nullptr;
-    if (!p->density || !p->kernel || !p->radii || !p->conv_theta_deg || !p->conv_phi_deg || !p->spec_fluence ||
-        !p->spec_energy || !p->spec_mu || !p->spec_mu_en)
-        return set_error(CCCS_EINVAL, "problem tables must be non-null");
-    if (p->ntheta < 2 || p->ntheta % 2 || p->nphi < 1 || p->nradii < 2 || p->nradii > (uint32_t)kMaxRadii)
-        return set_error(CCCS_EINVAL, "bad ntheta/nphi/nradii");
-    if (p->nphi * (p->ntheta / 2) > (uint32_t)kMaxRays) return set_error(CCCS_EINVAL, "more than 48 cone lines");
-    if (p->nspec < 1 || p->nspec > (uint32_t)kMaxSpec) return set_error(CCCS_EINVAL, "nspec must be in [1, 30]");
-    for (int k = 0; k < 3; k++) {
-        if (p->size[k] == 0 || p->bbox_size[k] == 0 || (uint64_t)p->bbox_start[k] + p->bbox_size[k] > p->size[k])
-            return set_error(CCCS_EINVAL, "calc_bbox must lie inside the volume");
-        if (!(p->voxel[k] > 0.f)) return set_error(CCCS_EINVAL, "voxel size must be > 0");
-    }
-    if (!(p->rev_latspacing > 0.f) || !(p->rev_longspacing > 0.f)) return set_error(CCCS_EINVAL, "REV spacing must be > 0");
-    for (uint32_t r = 1; r < p->nradii; r++)
-        if (!(p->radii[r] > p->radii[r - 1]) || !(p->radii[0] > 0.f)) return set_error(CCCS_EINVAL, "radii must increase");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        (void)hipGetLastError();
-        return set_error(CCCS_EDEVICE, "no HIP device available");
-    }
-    if (device < 0 || device >= ndev) return set_error(CCCS_EINVAL, "device ordinal out of range");
-    CCCS_HIP(hipSetDevice(device));
-    cccs_ctx* c = new cccs_ctx();
-    c->device = device;
-    c->prob = *p;
-    c->prob.density = nullptr; c->prob.kernel = nullptr; c->prob.radii = nullptr;
-    c->prob.conv_theta_deg = nullptr; c->prob.conv_phi_deg = nullptr;
-    c->prob.spec_fluence = c->prob.spec_energy = c->prob.spec_mu = c->prob.spec_mu_en = nullptr;
-    c->nrays = (int)(p->nphi * (p->ntheta / 2));
-    c->kernel.assign(p->kernel, p->kernel + (size_t)p->ntheta * p->nradii);
-    c->radii.assign(p->radii, p->radii + p->nradii);
-    c->theta.assign(p->conv_theta_deg, p->conv_theta_deg + c->nrays);
-    c->phi.assign(p->conv_phi_deg, p->conv_phi_deg + c->nrays);
-    c->vol.nx = (int)p->size[0]; c->vol.ny = (int)p->size[1]; c->vol.nz = (int)p->size[2];
-    c->vol.start = ld3(p->start);
-    c->vol.voxel = ld3(p->voxel);
-    c->vol.end = mk3(p->start[0] + (float)p->size[0] * p->voxel[0], p->start[1] + (float)p->size[1] * p->voxel[1],
-                     p->start[2] + (float)p->size[2] * p->voxel[2]);
-    c->spec.n = (int)p->nspec;
-    for (uint32_t e = 0; e < p->nspec; e++) {
-        c->spec.flu[e] = p->spec_fluence[e];
-        c->spec.en[e] = p->spec_energy[e];
-        c->spec.mu[e] = p->spec_mu[e];
-    }
-    // beam-path REV limit, derived as initCudaConstandTex does (nvbbRayConvolve_host.cu:248-258)
-    {
-        float min_vox = std::min(p->rev_latspacing, p->rev_longspacing);
-        v3 bb = ldu3(p->bbox_size);
-        unsigned diag = (unsigned)std::ceil(len3(bb) * p->voxel[0] / min_vox);
-        unsigned v = (unsigned)(diag + 2 * (0.0f / min_vox + p->penumbra / min_vox) + 100);
-        c->max_rev_beam[0] = c->max_rev_beam[1] = c->max_rev_beam[2] = v;
-    }
-    int st = CCCS_OK;
-    do {
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { st = set_error(CCCS_EDEVICE, "stream"); break; }
-        size_t nv = (size_t)p->size[0] * p->size[1] * p->size[2];
-        if (hipMalloc((void**)&c->d_dens, nv * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "density"); break; }
-        if (hipMemcpy(c->d_dens, p->density, nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "density copy"); break; }
-        // tables: kernel | radii | 1/width | mean radii | 1/width of mean radii
-        const uint32_t nr = p->nradii;
-        std::vector<float> tab((size_t)p->ntheta * nr + 4 * nr);
-        std::copy(c->kernel.begin(), c->kernel.end(), tab.begin());
-        float* R = tab.data() + (size_t)p->ntheta * nr;
-        float* IW = R + nr;
-        float* MR = IW + nr;
-        float* MIW = MR + nr;
-        for (uint32_t r = 0; r < nr; r++) {
-            R[r] = p->radii[r];
-            IW[r] = 1.0f / (r == 0 ? p->radii[0] : (p->radii[r] - p->radii[r - 1]));
-            MR[r] = (float)(r == 0 ? 0.5 * p->radii[0] : 0.5 * (p->radii[r] + p->radii[r - 1]));
-        }
-        for (uint32_t r = 0; r < nr; r++) MIW[r] = 1.0f / (r == 0 ? MR[0] : (MR[r] - MR[r - 1]));
-        if (hipMalloc((void**)&c->d_tables, tab.size() * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "tables"); break; }
-        if (hipMemcpy(c->d_tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "tables copy"); break; }
-        c->resident = nv * sizeof(float) + tab.size() * sizeof(float);
-    } while (false);
-    if (st != CCCS_OK) {
-        cccs_ctx_destroy(c);
-        return st;
-    }
-    *out = c;
-    return CCCS_OK;
-}
-
-extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
-    if (!c) return CCCS_OK;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->d_dens) (void)hipFree(c->d_dens);
-    if (c->d_tables) (void)hipFree(c->d_tables);
-    c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
-    c->dose.release(); c->col.release();
-    c->h_small.release(); c->h_cones.release();
-    for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
-    if (c->prof.d_counts) (void)hipFree(c->prof.d_counts);
-    if (c->prof.d_part) (void)hipFree(c->prof.d_part);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
-    return CCCS_OK;
-}
-
-extern "C" int cccs_ctx_memory(const cccs_ctx* c, uint64_t* resident, uint64_t* scratch) {
-    if (!c) return CCCS_EINVAL;
-    if (resident) *resident = c->resident;
-    if (scratch)
-        *scratch = c->terma.bytes + c->fmap.bytes + c->partial.bytes + c->rev.bytes + c->cones.bytes + c->dose.bytes +
-                   c->col.bytes;
-    return CCCS_OK;
-}
 
 extern "C" int cccs_beam_dose_device(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, const float* d_fluence,
                                      float* d_dose) {
@@ -1044,131 +805,3 @@ extern "C" int cccs_siddon_paths(cccs_ctx* ctx, const float* source, const float
     return st;
 }
 
-// ---- memory helpers ----------------------------------------------------------------------------
-extern "C" int cccs_dev_alloc(cccs_ctx* ctx, size_t bytes, void** ptr) {
-    clear_error();
-    if (!ctx || !ptr) return set_error(CCCS_EINVAL, "null argument");
-    *ptr = nullptr;
-    CCCS_HIP(hipSetDevice(ctx->device));
-    if (hipMalloc(ptr, std::max<size_t>(bytes, 4)) != hipSuccess) {
-        (void)hipGetLastError();
-        *ptr = nullptr;
-        return set_error(CCCS_ENOMEM, "hipMalloc failed");
-    }
-    return CCCS_OK;
-}
-extern "C" int cccs_dev_free(cccs_ctx* ctx, void* ptr) {
-    if (!ctx) return CCCS_EINVAL;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    if (ptr) CCCS_HIP(hipFree(ptr));
-    return CCCS_OK;
-}
-extern "C" int cccs_dev_upload(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
-    clear_error();
-    if (!ctx || !dst || !src) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    CCCS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-    return CCCS_OK;
-}
-extern "C" int cccs_dev_download(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
-    clear_error();
-    if (!ctx || !dst || !src) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    CCCS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return CCCS_OK;
-}
-extern "C" int cccs_dev_zero(cccs_ctx* ctx, void* ptr, size_t bytes) {
-    clear_error();
-    if (!ctx || !ptr) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipMemsetAsync(ptr, 0, bytes, ctx->stream));
-    return CCCS_OK;
-}
-__global__ void __launch_bounds__(256) copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
-}
-
-extern "C" int cccs_dev_copy(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
-    clear_error();
-    if (!ctx || !dst || !src || bytes % 16 != 0) return set_error(CCCS_EINVAL, "bad copy arguments");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    size_t n = bytes / 16;
-    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-    if (n) hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (float4*)dst, (const float4*)src, n);
-    CCCS_HIP(hipGetLastError());
-    return CCCS_OK;
-}
-
-extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
-    clear_error();
-    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    return CCCS_OK;
-}
-
-// ---- profiling -----------------------------------------------------------------------------------
-// Algorithmic FP32 operations per counted unit: one line-walk iteration (dens*step, two half-step
-// adds, interval lerp, kernel difference times TERMA: 10 ops) and one Siddon plane crossing (three
-// validity tests, plane choice, segment length, midpoint fetch coordinate, 8-tap trilinear: 40 ops).
-static const double kFlopsPerWalkIter = 10.0;
-static const double kFlopsPerSiddonStep = 40.0;
-
-extern "C" int cccs_ctx_profile(cccs_ctx* ctx, int enable) {
-    clear_error();
-    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    Profiler& P = ctx->prof;
-    if (enable && !P.d_counts) {
-        CCCS_HIP(hipMalloc((void**)&P.d_counts, sizeof(uint64_t) * K_COUNT));
-        CCCS_HIP(hipMemset(P.d_counts, 0, sizeof(uint64_t) * K_COUNT));
-    }
-    P.on = enable != 0;
-    return CCCS_OK;
-}
-extern "C" int cccs_ctx_profile_reset(cccs_ctx* ctx) {
-    clear_error();
-    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->prof.recs.clear();
-    ctx->prof.used = 0;
-    if (ctx->prof.d_counts) CCCS_HIP(hipMemset(ctx->prof.d_counts, 0, sizeof(uint64_t) * K_COUNT));
-    return CCCS_OK;
-}
-extern "C" int cccs_ctx_profile_read(cccs_ctx* ctx, cccs_kernel_stats* out, int max_rows, int* nrows) {
-    clear_error();
-    if (!ctx || !out || !nrows) return set_error(CCCS_EINVAL, "null argument");
-    CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    cccs_kernel_stats rows[K_COUNT];
-    std::memset(rows, 0, sizeof(rows));
-    for (const ProfRec& r : ctx->prof.recs) {
-        float ms = 0.f;
-        CCCS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-        cccs_kernel_stats& k = rows[r.kind];
-        k.launches++;
-        k.ms_total += ms;
-        k.alg_bytes += r.bytes;
-        k.alg_flops += r.flops;
-        k.work_items += r.items;
-    }
-    uint64_t cnt[K_COUNT] = {0};
-    if (ctx->prof.d_counts) CCCS_HIP(hipMemcpy(cnt, ctx->prof.d_counts, sizeof(cnt), hipMemcpyDeviceToHost));
-    rows[K_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_CONE];
-    rows[K_BL_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_BL_CONE];
-    rows[K_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_TERMA];
-    rows[K_BL_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_TERMA];
-    rows[K_BL_RAY].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_RAY];
-    int n = 0;
-    for (int k = 0; k < K_COUNT && n < max_rows; k++) {
-        if (rows[k].launches == 0) continue;
-        std::snprintf(rows[k].name, sizeof(rows[k].name), "%s", kKernelNames[k]);
-        out[n++] = rows[k];
-    }
-    *nrows = n;
-    return CCCS_OK;
-}

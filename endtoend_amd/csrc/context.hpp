@@ -69,18 +69,25 @@ struct ProfRec {
     hipEvent_t a, b;
     double bytes, flops;
     uint64_t items;
+    uint32_t* cnt;  // per-workgroup work counts of this launch (device), or nullptr
+    size_t ncnt;
 };
+
+constexpr size_t kCountChunk = size_t(1) << 24;  // counter entries per device chunk (64 MiB)
 
 struct Profiler {
     bool on = false;
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     std::vector<ProfRec> recs;
-    uint64_t* d_counts = nullptr;  // per-kind device counters (walk iterations, Siddon steps)
-    uint32_t* d_part = nullptr;    // per-block partial counts
-    size_t part_cap = 0;
+    std::vector<uint32_t*> chunks;  // per-launch counter slices, summed on the host at read time
+    size_t cur = 0;                 // chunk being filled
+    size_t chunk_used = 0;
+    uint32_t* pending = nullptr;
+    size_t pending_n = 0;
 };
 
+struct LineMeta;
 }  // namespace cccs
 
 struct cccs_ctx {
@@ -95,7 +102,7 @@ struct cccs_ctx {
     float* d_dens = nullptr;
     float* d_tables = nullptr;  // kernel (ntheta*nradii) | radii | inverse widths | mean radii | inverse widths
     size_t resident = 0;
-    cccs::DevBuf terma, fmap, partial, rev, cones, dose, col;
+    cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan;
     cccs::HostBuf h_small, h_cones;
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
@@ -110,12 +117,15 @@ inline size_t tab_mradii(const cccs_ctx* c) { return tab_iw(c) + c->prob.nradii;
 inline size_t tab_miw(const cccs_ctx* c) { return tab_mradii(c) + c->prob.nradii; }
 
 BeamDev make_beam_dev(const cccs_beam& b);
-// Profiling hooks: prof_begin returns a token (or -1 when profiling is off); prof_end closes it.
-// prof_counter returns the per-block partial-count array for a launch of `blocks` workgroups (or
-// nullptr when off); prof_fold adds those partials into the kind's device counter.
+// Profiling hooks.  prof_counter reserves a per-workgroup count slice for the next launch (nullptr
+// when profiling is off); prof_begin records a start event and attaches that slice; prof_end
+// records the stop event and the launch's algorithmic bytes/FLOPs.  Nothing runs in the stream
+// besides the events: the counts are summed on the host by cccs_ctx_profile_read.
+uint32_t* prof_counter(cccs_ctx* c, size_t blocks);
 int prof_begin(cccs_ctx* c, int kind);
 void prof_end(cccs_ctx* c, int token, double bytes, double flops, uint64_t items);
-uint32_t* prof_counter(cccs_ctx* c, size_t blocks);
-void prof_fold(cccs_ctx* c, int kind, size_t blocks);
+
+// Exclusive scan of LineMeta::len into LineMeta::off over n lines (on the context stream).
+int launch_line_scan(cccs_ctx* c, LineMeta* meta, uint64_t n);
 int check_beam(const cccs_ctx* ctx, const cccs_beam& b);
 }  // namespace cccs

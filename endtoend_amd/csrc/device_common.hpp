@@ -1,0 +1,140 @@
+// device_common.hpp -- device helpers shared by the beam and beamlet pipelines: XCD-aware work
+// mapping, per-workgroup counters, the REV line table, and the collapsed-cone line walk.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cccs_internal.hpp"
+
+namespace cccs {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the part (workgroup b runs on XCD
+// b % 8).  xcd_remap(b, nb) gives the workgroups of one XCD one contiguous range of work items,
+// so neighbouring REV lines / voxel tiles share that XCD's L2.  A bijection on [0, nb).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+    const uint32_t nx = 8;
+    const uint32_t per = nb / nx, rem = nb % nx;
+    const uint32_t x = b % nx, idx = b / nx;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
+// Sums one uint32 per thread over the workgroup (LDS tree, <= 256 threads) into part[slot].
+__device__ __forceinline__ void block_count(uint32_t v, uint32_t* __restrict__ part, size_t slot) {
+    __shared__ uint32_t s_cnt[256];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    s_cnt[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && t + w < nt) s_cnt[t] += s_cnt[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[slot] = s_cnt[0];
+}
+
+// One REV line of a cone grid in the compact store: samples [i0, i0 + len) of the line are kept at
+// rev[off ...]; every other sample of the line is zero.
+struct LineMeta {
+    int i0;
+    int len;
+    uint32_t off;
+    int pad;
+};
+
+// Trilinear fetch from a compactly stored REV grid (nx samples per line, ny x nz lines) at texture
+// coordinates (u, v, w) = (sample, line y, line z), texel centres at +0.5, border 0: the same
+// operation sequence as tex3_border on the dense grid, so both give identical results.
+__device__ __forceinline__ float rev_fetch(const LineMeta* __restrict__ meta, const float* __restrict__ rev,
+                                           uint64_t line0, int ny, int nz, float u, float v, float w) {
+    const float ub = u - 0.5f, vb = v - 0.5f, wb = w - 0.5f;
+    const float fu = floorf(ub), fv = floorf(vb), fw = floorf(wb);
+    const int i = (int)fu, j = (int)fv, k = (int)fw;
+    const float a = ub - fu, b = vb - fv, c = wb - fw;
+    float t[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int jj = j + (q & 1), kk = k + (q >> 1);
+        float lo = 0.f, hi = 0.f;
+        if (jj >= 0 && jj < ny && kk >= 0 && kk < nz) {
+            const LineMeta m = meta[line0 + (uint64_t)jj + (uint64_t)ny * kk];
+            const int d = i - m.i0;
+            if (d >= 0 && d < m.len) lo = rev[m.off + d];
+            if (d + 1 >= 0 && d + 1 < m.len) hi = rev[m.off + d + 1];
+        }
+        t[q] = lo + a * (hi - lo);
+    }
+    const float e0 = t[0] + b * (t[1] - t[0]);
+    const float e1 = t[2] + b * (t[3] - t[2]);
+    return e0 + c * (e1 - e0);
+}
+
+// The line's samples live in LDS as {half step, TERMA}: x = 0.5*delr*density (negative marks a
+// sample outside the crop box or the calc_bbox), y = TERMA.
+//
+// walk_both runs the two walks of destination j (RowConvolve's do/while loops) back to back over
+// the line's valid segment [vlo, vhi]: first toward lower indices with the forward kernel row,
+// then toward higher indices with the backward row.  Every destination takes exactly vhi - vlo
+// steps, so the trip count is the same on every lane of a wave (a scalar loop, no divergence);
+// the direction switch and the reference's early exits (radiological depth beyond the last radius,
+// reaching the last radial interval) become per-lane flags.  The cumulative kernel is linear
+// between two radii, so a lane keeps its current interval in registers and reads the LDS tables
+// only when the radiological depth crosses a radius.
+__device__ __forceinline__ float walk_both(const float2* __restrict__ s, int j, int vlo, int vhi,
+                                           const float* __restrict__ R, const float* __restrict__ IW,
+                                           const float* __restrict__ rowF, const float* __restrict__ rowB, int nr,
+                                           bool active, uint32_t& iters) {
+    const float rmax = R[nr - 1];
+    const int total = vhi - vlo;
+    const int nfwd = j - vlo;
+    const float* row = rowF;
+    float r = 0.f, last = 0.f, lo = 0.f, hi = R[0], klo = 0.f, slope = rowF[0] * IW[0], sum = 0.f;
+    int rs = 0, k = j, dir = -1;
+    bool alive = active && nfwd > 0;
+    for (int it = 0; it < total; ++it) {
+        if (it == nfwd) {  // switch to the backward walk
+            dir = 1;
+            k = j;
+            row = rowB;
+            r = 0.f; last = 0.f; lo = 0.f; hi = R[0]; klo = 0.f; slope = rowB[0] * IW[0];
+            rs = 0;
+            alive = active;
+        }
+        k += dir;
+        const float2 e = s[k];
+        r += e.x;
+        bool ok = alive && e.x >= 0.f && r <= rmax;
+        if (ok && hi < r) {
+            do {
+                rs++;
+                lo = hi;
+                hi = R[rs];
+                klo = row[rs - 1];
+                slope = (row[rs] - klo) * IW[rs];
+            } while (hi < r && rs < nr - 1);
+        }
+        const float cum = klo + (r - lo) * slope;
+        sum += ok ? (cum - last) * e.y : 0.f;
+        last = cum;
+        r += e.x;
+        iters += ok ? 1u : 0u;
+        alive = ok && rs < nr - 1;
+    }
+    return sum;
+}
+
+// Narrows [t0, t1] to the parameters where o + t*d lies in [lo, hi) (widened by a margin that
+// covers FP32 rounding; the exact per-sample test decides membership).
+__device__ __forceinline__ void clip_slab(float o, float d, float lo, float hi, float& t0, float& t1) {
+    if (fabsf(d) < 1e-3f) {
+        // the coordinate moves by less than one index unit over a whole line (<= 1024 samples)
+        if (!(o >= lo - 2.f && o < hi + 2.f)) { t0 = 1.f; t1 = 0.f; }
+        return;
+    }
+    float ta = (lo - o) / d, tb = (hi - o) / d;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+}
+
+}  // namespace cccs
