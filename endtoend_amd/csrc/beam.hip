@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
         const ConeDev& c = sc[q];
         const v3 r = c.a(g) + c.t;
         if (r.y >= (float)c.nx || r.z >= (float)c.ny || r.x >= (float)c.nz || r.x < 0.f || r.y < 0.f || r.z < 0.f) continue;
-        sum += rev_fetch(a.lines, a.rev, c.line0, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
+        sum += rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
     }
     if (sum != 0.f) a.dose[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] += sum;
 }

@@ -1,16 +1,1170 @@
-// beamlet.hip -- beamlet dose coefficients (dosecalc-beamlet path).  Entry points are filled in
-// by the beamlet pipeline; until then they report CCCS_EINVAL.
+// beamlet.hip -- beamlet dose coefficients (dosecalc-beamlet: nvbbRayConvolve_host.cu:444-1409,
+// nvbbRayConvolve_device.cu:19-521) on gfx950.
+//
+// Every active beamlet of a beam becomes a pillar (its beamlet's path through the calc_bbox plus a
+// kernel_extent margin and a wall), and the pillars of a batch are tiled side by side in one packed
+// BEV grid.  For each cone direction, every REV line of every batch in the job is resampled into
+// LDS (pillar mapping, per-beamlet TERMA, density), walked inside its pillar segment, and the cone
+// dose is gathered into the packed grid.  Each beamlet's pillar is then resampled back onto the
+// calc_bbox and compacted on the device into a sparse column.
+//
+// A job is a list of batches, possibly from several beams: one launch of each kernel covers every
+// batch of the job, so small beams and large beams share launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
 #include "context.hpp"
+#include "device_common.hpp"
+
+namespace cccs {
+
+constexpr int kWall = 2;        // pillar wall, REV samples (PILLAR_GRID, io_data_structs.h:139-160)
+constexpr int kBuffer = 2;      // pillar rim excluded from unpacking (same struct)
+constexpr int kMaxBatches = 100;  // batch-count retry limit of the reference (host.cu:692-834)
+
+// One Rodrigues rotation with host-evaluated sine/cosine.
+struct Rot1 {
+    v3 a;
+    float s, c;
+};
+CCCS_HD v3 rot1(const Rot1& R, v3 p) { return rodrigues(p, R.a, R.s, R.c); }
+
+// invRotateBeamlet / rotateBeamlet (CudaUtilities/geometry.cu:99-115): a rotation about
+// (sin(-phi), 0, cos(-phi)) by -theta (beamlet frame -> BEV) or +theta (BEV -> beamlet frame).
+static Rot1 beamlet_rot(float theta, float phi, bool to_bev) {
+    Rot1 r;
+    float s, c;
+    sincosf(-phi, &s, &c);
+    r.a = mk3(s, 0.f, c);
+    sincosf(to_bev ? -theta : theta, &r.s, &r.c);
+    return r;
+}
+
+struct BeamletDev {
+    v3 pst;          // pillar origin [cm]
+    Rot1 inv;        // beamlet frame -> BEV
+    Rot1 fwd;        // BEV -> beamlet frame
+    int fidx;        // fluence-map index of the beamlet
+    int batch;       // batch of the job
+    int pix, piy;    // pillar column / row inside the batch grid
+    int box0[3];     // calc_bbox-relative voxel box that can receive its dose
+    int boxn[3];
+    uint64_t arena;  // first element of its box in the column arena (i outer, k inner)
+};
+
+struct BatchDev {
+    BeamDev beam;
+    Rot2 beam_inv;         // BEV -> PVCS
+    const float* fmap;     // fluence map of the beam (device)
+    const float* tgrid;    // unit-TERMA grid of the beam (fast mode)
+    int n, npx, npy;       // beamlets, pillars per row, rows
+    int gd[3];             // packed grid dims (pX, pY, pZ)
+    int pd[3];             // pillar dims
+    int beamlet0;          // first beamlet of the batch in the job's beamlet table
+    uint64_t pdose;        // float offset of its packed dose grid ([pX][pZ][pY], pY fastest)
+    uint32_t tile0;        // first gather tile
+    int tiles[3];          // gather tiles along pY (16), pZ (4), pX (4)
+};
+
+// One (cone, batch) REV grid.
+struct BlConeDev {
+    Rot2 kinv;       // REV -> packed BEV (invRotateKernel)
+    Rot2 kfwd;       // packed BEV -> REV (rotateKernel)
+    m33 kinv_m;      // matrix form of kinv, for line intervals only
+    v3 minc;         // REV grid origin [cm]
+    int nx, ny, nz;  // line length, lines along y, lines along z
+    int batch;
+    uint64_t line0;  // first line of this grid among the launch's lines
+};
+
+struct BlArgs {
+    const float* dens;
+    const float* tables;        // kernel rows | radii | 1/width | mean radii | 1/width of mean radii
+    const BatchDev* batches;
+    const BlConeDev* cones;     // the grids of one cone direction, one per batch
+    const BeamletDev* beamlets;
+    LineMeta* lines;
+    float* rev;
+    float* pdose;
+    int nbatch;
+    uint64_t nlines;
+    int ntheta, nradii, nphi, rowf, rowb;
+    float delr;
+    v3 rvs, rrv;                // REV spacings (lat, long, lat) and reciprocals
+    VolDesc vd;
+    SpecDev spec;
+    float bhc;
+    int fast;                   // unit-TERMA grid instead of a Siddon march per sample
+    int tg[3];                  // unit-TERMA grid dims (CT dims + 2)
+    uint32_t* counts;
+};
+
+__device__ __forceinline__ int find_cone(const BlConeDev* __restrict__ c, int n, uint64_t line) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (c[mid].line0 <= line) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// REV sample (tX, tY, tZ) of a grid -> packed-BEV coordinate (cudaBeamletRaytrace,
+// nvbbRayConvolve_device.cu:33-41).
+__device__ __forceinline__ v3 rev_to_packed(const BlConeDev& c, v3 rrv, int tX, int tY, int tZ) {
+    v3 q = mk3(fmaf(c.minc.x, rrv.x, (float)tZ), fmaf(c.minc.y, rrv.y, (float)tX), fmaf(c.minc.z, rrv.z, (float)tY));
+    return rot2(c.kinv, q) + 0.5f;
+}
+
+// Interval of stored samples of every line of one cone direction: the line clipped to the packed
+// grid box (packed coordinates are affine in the sample index), widened by 2/3 samples.
+__global__ void __launch_bounds__(256) bl_interval_kernel(BlArgs a) {
+    const uint64_t line = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (line >= a.nlines) return;
+    const BlConeDev& c = a.cones[find_cone(a.cones, a.nbatch, line)];
+    const BatchDev& B = a.batches[c.batch];
+    const uint64_t l = line - c.line0;
+    const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
+    v3 q0 = mk3(fmaf(c.minc.x, a.rrv.x, (float)tZ), c.minc.y * a.rrv.y, fmaf(c.minc.z, a.rrv.z, (float)tY));
+    v3 p0 = c.kinv_m(q0) + 0.5f;
+    v3 d = mk3(c.kinv_m.m[1], c.kinv_m.m[4], c.kinv_m.m[7]);
+    float t0 = 0.f, t1 = (float)c.nx;
+    clip_slab(p0.x, d.x, 0.f, (float)B.gd[0], t0, t1);
+    clip_slab(p0.y, d.y, 0.f, (float)B.gd[1], t0, t1);
+    clip_slab(p0.z, d.z, 0.f, (float)B.gd[2], t0, t1);
+    LineMeta m;
+    m.i0 = 0;
+    m.len = 0;
+    m.off = 0;
+    m.pad = 0;
+    if (t0 <= t1) {
+        m.i0 = max(0, (int)floorf(t0) - 2);
+        const int i1 = min(c.nx, (int)ceilf(t1) + 3);
+        m.len = max(0, i1 - m.i0);
+    }
+    a.lines[line] = m;
+}
+
+// Unit TERMA from the beam's grid (CT voxel centres with a one-voxel rim), trilinear with clamping.
+__device__ __forceinline__ float tgrid_fetch(const float* __restrict__ g, const int* tg, v3 gi) {
+    float x = fminf(fmaxf(gi.x + 1.f, 0.f), (float)(tg[0] - 1));
+    float y = fminf(fmaxf(gi.y + 1.f, 0.f), (float)(tg[1] - 1));
+    float z = fminf(fmaxf(gi.z + 1.f, 0.f), (float)(tg[2] - 1));
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const int i = (int)fx, j = (int)fy, k = (int)fz;
+    const int i1 = min(i + 1, tg[0] - 1), j1 = min(j + 1, tg[1] - 1), k1 = min(k + 1, tg[2] - 1);
+    const float u = x - fx, v = y - fy, w = z - fz;
+    const size_t sx = 1, sy = (size_t)tg[0], sz = (size_t)tg[0] * tg[1];
+    auto at = [&](int ii, int jj, int kk) { return g[ii * sx + jj * sy + kk * sz]; };
+    const float c00 = at(i, j, k) + u * (at(i1, j, k) - at(i, j, k));
+    const float c10 = at(i, j1, k) + u * (at(i1, j1, k) - at(i, j1, k));
+    const float c01 = at(i, j, k1) + u * (at(i1, j, k1) - at(i, j, k1));
+    const float c11 = at(i, j1, k1) + u * (at(i1, j1, k1) - at(i, j1, k1));
+    const float c0 = c00 + v * (c10 - c00), c1 = c01 + v * (c11 - c01);
+    return c0 + w * (c1 - c0);
+}
+
+// Pillar mapping of one REV sample (cudaBeamletRaytrace, nvbbRayConvolve_device.cu:33-68):
+// false for a wall, a sample outside the packed grid or past the last pillar; otherwise the
+// beamlet (pidx, batch-local) and the pillar-local coordinate inp (already shifted by -0.5).
+__device__ __forceinline__ bool bl_locate(const BlArgs& a, const BlConeDev& c, const BatchDev& B, int tX, int tY,
+                                          int tZ, int& pidx, v3& inp) {
+    const v3 pb = rev_to_packed(c, a.rrv, tX, tY, tZ);
+    const v3 gdf = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]);
+    if (pb.x < 0.f || pb.y < 0.f || pb.z < 0.f || pb.x >= gdf.x || pb.y >= gdf.y || pb.z >= gdf.z) return false;
+    const v3 pdf = mk3((float)B.pd[0], (float)B.pd[1], (float)B.pd[2]);
+    const int pix = f2i_rd(pb.x / pdf.x), piy = f2i_rd(pb.z / pdf.z);
+    pidx = pix + piy * B.npx;
+    if (pidx >= B.n) return false;
+    inp = mk3(fmodf(pb.x, pdf.x), fmodf(pb.y, pdf.y), fmodf(pb.z, pdf.z));
+    if (inp.x < (float)kWall || inp.y < (float)kWall || inp.z < (float)kWall) return false;
+    inp = inp - 0.5f;
+    return true;
+}
+
+// PVCS point [cm] where the sample's TERMA is evaluated (device.cu:170-180).
+__device__ __forceinline__ v3 bl_terma_point(const BlArgs& a, const BatchDev& B, const BeamletDev& bl, v3 inp) {
+    const v3 g = rot2(B.beam_inv, rot1(bl.inv, inp));
+    return mk3(fmaf(g.x, a.rvs.x, bl.pst.x), fmaf(g.y, a.rvs.y, bl.pst.y), fmaf(g.z, a.rvs.z, bl.pst.z));
+}
+
+// Density at the sample (device.cu:213-223): trilinear, border 0.
+__device__ __forceinline__ float bl_density(const BlArgs& a, const BatchDev& B, const BeamletDev& bl, v3 inp) {
+    const v3 g2 = rot2(B.beam_inv, rot1(bl.inv, inp * a.rvs)) + bl.pst;
+    const v3 gi = (g2 - a.vd.start) / a.vd.voxel;
+    return tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, gi.x + 0.5f, gi.y + 0.5f, gi.z + 0.5f);
+}
+
+// PackRowConvolve for destination j (nvbbRayConvolve_device.cu:284-417): walks toward lower and
+// then higher indices inside the pillar segment [slo, shi] that holds j, each walk starting at j
+// itself (the self sample adds only its first half step).  Both walks together take shi-slo+2
+// steps, the same count for every destination of a segment.  The cumulative kernel is linear
+// between two mean radii, so the current interval stays in registers.
+__device__ __forceinline__ float walk_pack(const float2* __restrict__ s, int j, int slo, int shi,
+                                           const float* __restrict__ R, const float* __restrict__ IW,
+                                           const float* __restrict__ rowF, const float* __restrict__ rowB, int nr,
+                                           bool active, uint32_t& iters) {
+    const float rmax = R[nr - 1];
+    const int total = shi - slo + 2;
+    const int nfwd = j - slo + 1;
+    const float* row = rowF;
+    float r = 0.f, last = 0.f, lo = 0.f, hi = R[0], klo = 0.f, slope = rowF[0] * IW[0], sum = 0.f;
+    int rs = 0, k = j + 1, dir = -1;
+    bool alive = active;
+    for (int it = 0; it < total; ++it) {
+        if (it == nfwd) {
+            dir = 1;
+            k = j - 1;
+            row = rowB;
+            r = 0.f; last = 0.f; lo = 0.f; hi = R[0]; klo = 0.f; slope = rowB[0] * IW[0];
+            rs = 0;
+            alive = active;
+        }
+        k += dir;
+        const float2 e = s[k];
+        r += e.x;
+        const bool ok = alive && r <= rmax;
+        if (ok && hi < r) {
+            do {
+                rs++;
+                lo = hi;
+                hi = R[rs];
+                klo = row[rs - 1];
+                slope = (row[rs] - klo) * IW[rs];
+            } while (hi < r && rs < nr - 1);
+        }
+        const float cum = klo + (r - lo) * slope;
+        sum += ok ? (cum - last) * e.y : 0.f;
+        last = cum;
+        if (k != j) r += e.x;
+        iters += ok ? 1u : 0u;
+        alive = ok && rs < nr - 1;
+    }
+    return sum;
+}
+
+// One wave per REV line of one cone direction (all batches of the job), XCD-aware order.
+__global__ void __launch_bounds__(64) bl_cone_kernel(BlArgs a) {
+    extern __shared__ float2 s_line[];
+    __shared__ short s_lo[kMaxLine], s_hi[kMaxLine];
+    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    const LineMeta meta = a.lines[line];
+    uint32_t iters = 0;
+    if (meta.len > 0) {
+        const BlConeDev& c = a.cones[find_cone(a.cones, a.nbatch, line)];
+        const BatchDev& B = a.batches[c.batch];
+        const int nr = a.nradii;
+        if (tid < nr) {
+            const float* K = a.tables;
+            const float* MR = a.tables + (size_t)a.ntheta * nr + 2 * nr;
+            sR[tid] = MR[tid];
+            sIW[tid] = MR[nr + tid];
+            sF[tid] = K[(size_t)a.rowf * nr + tid];
+            sB[tid] = K[(size_t)a.rowb * nr + tid];
+        }
+        const uint64_t l = line - c.line0;
+        const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
+        const int i0 = meta.i0, m = meta.len;
+        // pass 1: pillar mapping and density of every stored sample; samples inside their
+        // beamlet's footprint are queued for TERMA (fast mode evaluates it here)
+        short* s_req = s_lo;  // reused: the segment bounds are computed after the TERMA pass
+        int nreq = 0;
+        for (int base = 0; base < m; base += 64) {
+            const int j = base + tid;
+            float2 e = make_float2(-1.f, 0.f);
+            bool need = false;
+            int pidx;
+            v3 inp;
+            if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp)) {
+                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+                e.x = 0.5f * (a.delr * bl_density(a, B, bl, inp));
+                const v3 gcs = bl_terma_point(a, B, bl, inp);
+                const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
+                if (fl > 0.f) {
+                    if (a.fast) {
+                        const v3 gi = (gcs - a.vd.start) / a.vd.voxel;
+                        e.y = fl * tgrid_fetch(B.tgrid, a.tg, gi) * a.bhc * inv_square(gcs, B.beam);
+                    } else {
+                        need = true;
+                    }
+                }
+            }
+            if (j < m) s_line[j] = e;
+            const uint64_t mask = __ballot(need);
+            if (need) s_req[nreq + __popcll(mask & ((1ull << tid) - 1ull))] = (short)j;
+            nreq += __popcll(mask);
+        }
+        __syncthreads();
+        // pass 2: the queued samples' Siddon TERMA, one sample per lane
+        for (int r = 0; r < nreq; r += 64) {
+            if (r + tid < nreq) {
+                const int j = s_req[r + tid];
+                int pidx;
+                v3 inp;
+                bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp);
+                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+                const v3 gcs = bl_terma_point(a, B, bl, inp);
+                s_line[j].y = point_terma(gcs, B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, nullptr);
+            }
+        }
+        __syncthreads();
+        // pillar segments: for each sample the nearest wall on either side (chunked wave scans)
+        const int per = (m + 63) / 64;
+        const int ca = min(m, tid * per), cb = min(m, ca + per);
+        int last_wall = -1, first_wall = m;
+        for (int k = ca; k < cb; k++)
+            if (s_line[k].x < 0.f) last_wall = k;
+        for (int k = cb - 1; k >= ca; k--)
+            if (s_line[k].x < 0.f) first_wall = k;
+        int pre = last_wall, suf = first_wall;  // inclusive prefix max / suffix min over chunks
+        for (int o = 1; o < 64; o <<= 1) {
+            const int up = __shfl_up(pre, o, 64);
+            const int dn = __shfl_down(suf, o, 64);
+            if (tid >= o) pre = max(pre, up);
+            if (tid + o < 64) suf = min(suf, dn);
+        }
+        int left = __shfl_up(pre, 1, 64);
+        int right = __shfl_down(suf, 1, 64);
+        if (tid == 0) left = -1;
+        if (tid == 63) right = m;
+        for (int k = ca; k < cb; k++) {
+            if (s_line[k].x < 0.f) left = k;
+            s_lo[k] = (short)(left + 1);
+        }
+        for (int k = cb - 1; k >= ca; k--) {
+            if (s_line[k].x < 0.f) right = k;
+            s_hi[k] = (short)(right - 1);
+        }
+        __syncthreads();
+        float* out = a.rev + meta.off;
+        const float nphi_f = (float)a.nphi;
+        for (int base = 0; base < m; base += 64) {
+            const int j = min(base + tid, m - 1);
+            const bool act = base + tid < m && s_line[j].x >= 0.f;
+            const int slo = act ? (int)s_lo[j] : j, shi = act ? (int)s_hi[j] : j;
+            const float v = walk_pack(s_line, j, slo, shi, sR, sIW, sF, sB, nr, act, iters);
+            if (base + tid < m) out[j] = act ? v / nphi_f : 0.f;
+        }
+    }
+    if (a.counts) block_count(iters, a.counts, blockIdx.x);
+}
+
+// PackedREVtoBEVdose (nvbbRayConvolve_device.cu:427-452) for one cone direction over every batch's
+// packed grid: 16x4x4 voxel tiles, dose += cone dose where it is positive.
+__global__ void __launch_bounds__(256) bl_gather_kernel(BlArgs a, uint32_t ntiles) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    int lo = 0, hi = a.nbatch - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.batches[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+    }
+    const BatchDev& B = a.batches[lo];
+    const BlConeDev& c = a.cones[lo];
+    const uint32_t t = tile - B.tile0;
+    const int ty = (int)(t % (uint32_t)B.tiles[0]);
+    const int tz = (int)((t / (uint32_t)B.tiles[0]) % (uint32_t)B.tiles[1]);
+    const int tx = (int)(t / ((uint32_t)B.tiles[0] * (uint32_t)B.tiles[1]));
+    const int tid = threadIdx.x;
+    const int pY = ty * 16 + (tid & 15), pZ = tz * 4 + ((tid >> 4) & 3), pX = tx * 4 + (tid >> 6);
+    if (pX >= B.gd[0] || pY >= B.gd[1] || pZ >= B.gd[2]) return;
+    const v3 rk = rot2(c.kfwd, mk3((float)pX, (float)pY, (float)pZ));
+    const v3 ri = mk3(fmaf(-c.minc.x, a.rrv.x, rk.x), fmaf(-c.minc.y, a.rrv.y, rk.y), fmaf(-c.minc.z, a.rrv.z, rk.z));
+    const float dc = rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
+    if (dc > 0.f) a.pdose[B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX)] += dc;
+}
+
+__global__ void __launch_bounds__(256) bl_fill_kernel(float* __restrict__ p, uint64_t n, float v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = v;
+}
+
+// Unit TERMA at every CT voxel centre plus a one-voxel rim: sum_e phi_e E_e mu_e exp(-mu_e rpl)
+// with rpl from the same Siddon march as the per-sample TERMA.
+__global__ void __launch_bounds__(256) bl_tgrid_kernel(BlArgs a, BeamDev beam, float* __restrict__ g,
+                                                       uint32_t* counts) {
+    const uint64_t n = (uint64_t)a.tg[0] * a.tg[1] * a.tg[2];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t steps = 0;
+    if (i < n) {
+        const int x = (int)(i % (uint64_t)a.tg[0]);
+        const int y = (int)((i / (uint64_t)a.tg[0]) % (uint64_t)a.tg[1]);
+        const int z = (int)(i / ((uint64_t)a.tg[0] * a.tg[1]));
+        const v3 p = mk3(a.vd.start.x + (float)(x - 1) * a.vd.voxel.x, a.vd.start.y + (float)(y - 1) * a.vd.voxel.y,
+                         a.vd.start.z + (float)(z - 1) * a.vd.voxel.z);
+        int it = 0;
+        const float rpl = siddon_march(p, a.vd.start, a.vd.end, a.vd.voxel, beam.src, a.dens, a.vd.nx, a.vd.ny,
+                                       a.vd.nz, [](float, float, float, float) {}, &it);
+        g[i] = unit_terma(rpl, a.spec);
+        steps = (uint32_t)it;
+    }
+    if (counts) block_count(steps, counts, blockIdx.x);
+}
+
+struct UnpackArgs {
+    const BatchDev* batches;
+    const BeamletDev* beamlets;
+    const float* pdose;
+    float* arena;
+    int nbeamlets;
+    uint64_t narena;
+    v3 start, voxel, rvs;
+    int bbs[3];   // calc_bbox start
+};
+
+// UnpackBEVDosePillar (nvbbRayConvolve_device.cu:459-521) over each beamlet's voxel box: the
+// calc_bbox voxel goes to its pillar (beam then beamlet rotation), rim and walls read nothing, and
+// the packed dose is fetched trilinearly.  Arena order inside a box: i outer, j, k inner.
+__global__ void __launch_bounds__(256) bl_unpack_kernel(UnpackArgs a) {
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.narena) return;
+    int lo = 0, hi = a.nbeamlets - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.beamlets[mid].arena <= e) lo = mid; else hi = mid - 1;
+    }
+    const BeamletDev& bl = a.beamlets[lo];
+    const BatchDev& B = a.batches[bl.batch];
+    const uint64_t le = e - bl.arena;
+    const int k = (int)(le % (uint64_t)bl.boxn[2]);
+    const int j = (int)((le / (uint64_t)bl.boxn[2]) % (uint64_t)bl.boxn[1]);
+    const int i = (int)(le / ((uint64_t)bl.boxn[2] * bl.boxn[1]));
+    const int tX = bl.box0[0] + i, tY = bl.box0[1] + j, tZ = bl.box0[2] + k;
+    const v3 gc = mk3(((float)tX + (float)a.bbs[0]) * a.voxel.x + a.start.x,
+                      ((float)tY + (float)a.bbs[1]) * a.voxel.y + a.start.y,
+                      ((float)tZ + (float)a.bbs[2]) * a.voxel.z + a.start.z);
+    v3 pidx = rot1(bl.fwd, rot2(B.beam.beam_fwd, gc - bl.pst)) / a.rvs + 0.5f;
+    float v = 0.f;
+    const float lo_f = (float)(kWall + kBuffer);
+    if (!(pidx.x < lo_f || pidx.y < lo_f || pidx.z < lo_f || pidx.x >= (float)(B.pd[0] - kBuffer) ||
+          pidx.y >= (float)(B.pd[1] - kBuffer) || pidx.z >= (float)(B.pd[2] - kBuffer))) {
+        pidx = mk3(pidx.x + (float)(bl.pix * B.pd[0]), pidx.y, pidx.z + (float)(bl.piy * B.pd[2]));
+        const float dc = tex3_border(a.pdose + B.pdose, B.gd[1], B.gd[2], B.gd[0], pidx.y, pidx.z, pidx.x);
+        if (dc > 0.f) v = dc;
+    }
+    a.arena[e] = v;
+}
+
+// Per-beamlet column statistics (one workgroup per beamlet): max, kept (v > t * max) and dropped
+// (0 < v <= t * max) counts, as _sparsify (DoseCalcIO/sparse_io.cpp:22-58) defines them.
+__global__ void __launch_bounds__(256) bl_colstat_kernel(const BeamletDev* __restrict__ bls, const float* __restrict__ arena,
+                                                         float thresh, uint32_t* __restrict__ stat) {
+    __shared__ float s_max[256];
+    __shared__ uint32_t s_a[256], s_b[256];
+    const BeamletDev& bl = bls[blockIdx.x];
+    const uint64_t n = (uint64_t)bl.boxn[0] * bl.boxn[1] * bl.boxn[2];
+    const float* col = arena + bl.arena;
+    const int t = threadIdx.x;
+    float mx = -FLT_MAX;
+    for (uint64_t i = t; i < n; i += 256) mx = fmaxf(mx, col[i]);
+    s_max[t] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s_max[t] = fmaxf(s_max[t], s_max[t + w]);
+        __syncthreads();
+    }
+    // the column outside the box is zero, so its max is at least 0
+    const float cmax = fmaxf(s_max[0], 0.f);
+    const float cut = thresh * cmax;
+    uint32_t kept = 0, dropped = 0;
+    for (uint64_t i = t; i < n; i += 256) {
+        const float v = col[i];
+        if (v > cut) kept++;
+        else if (v > 0.f) dropped++;
+    }
+    s_a[t] = kept;
+    s_b[t] = dropped;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) { s_a[t] += s_a[t + w]; s_b[t] += s_b[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        stat[4 * blockIdx.x + 0] = s_a[0];
+        stat[4 * blockIdx.x + 1] = s_b[0];
+        stat[4 * blockIdx.x + 2] = __float_as_uint(cmax);
+        stat[4 * blockIdx.x + 3] = 0;
+    }
+}
+
+// Exclusive scan of the kept counts over the job's beamlets (one workgroup; n is small).
+__global__ void __launch_bounds__(1024) bl_offsets_kernel(uint32_t* __restrict__ stat, int n, uint64_t* __restrict__ off) {
+    __shared__ uint64_t s[1024];
+    uint64_t carry = 0;
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const uint64_t v = i < n ? stat[4 * i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint64_t add = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) off[i] = carry + s[threadIdx.x] - v;
+        carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) off[n] = carry;
+}
+
+// Writes each beamlet's kept entries (one workgroup per beamlet, order-preserving block scans):
+// key = full-volume linear index, value = dose.
+__global__ void __launch_bounds__(256) bl_compact_kernel(const BeamletDev* __restrict__ bls, const float* __restrict__ arena,
+                                                         float thresh, const uint32_t* __restrict__ stat,
+                                                         const uint64_t* __restrict__ off, int nx, int ny, int bb0,
+                                                         int bb1, int bb2, uint64_t* __restrict__ keys,
+                                                         float* __restrict__ vals) {
+    __shared__ uint32_t s[256];
+    const BeamletDev& bl = bls[blockIdx.x];
+    const uint64_t n = (uint64_t)bl.boxn[0] * bl.boxn[1] * bl.boxn[2];
+    const float* col = arena + bl.arena;
+    const float cut = thresh * __uint_as_float(stat[4 * blockIdx.x + 2]);
+    uint64_t pos = off[blockIdx.x];
+    const int t = threadIdx.x;
+    for (uint64_t base = 0; base < n; base += 256) {
+        const uint64_t i = base + t;
+        const float v = i < n ? col[i] : 0.f;
+        const uint32_t keep = (i < n && v > cut) ? 1u : 0u;
+        s[t] = keep;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const uint32_t add = t >= o ? s[t - o] : 0u;
+            __syncthreads();
+            s[t] += add;
+            __syncthreads();
+        }
+        if (keep) {
+            const int k = (int)(i % (uint64_t)bl.boxn[2]);
+            const int j = (int)((i / (uint64_t)bl.boxn[2]) % (uint64_t)bl.boxn[1]);
+            const int ii = (int)(i / ((uint64_t)bl.boxn[2] * bl.boxn[1]));
+            const uint64_t key = ((uint64_t)(bl.box0[2] + k + bb2) * (uint64_t)ny + (uint64_t)(bl.box0[1] + j + bb1)) *
+                                     (uint64_t)nx +
+                                 (uint64_t)(bl.box0[0] + ii + bb0);
+            const uint64_t p = pos + s[t] - 1;
+            keys[p] = key;
+            vals[p] = v;
+        }
+        const uint32_t tot = s[255];
+        __syncthreads();
+        pos += tot;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// host planning
+
+namespace {
+
+struct Rev {
+    v3 minc;
+    uint32_t size[3];
+};
+
+// findREV (dosecalc-beamlet/nvbbRayConvolve_host.cu:104-173): REV grid holding the rotated box
+// [0, lim]; false when it exceeds max_rev (the reference's ArraySizeError).
+bool find_rev(const cccs_problem& p, const uint32_t* max_rev, v3 lim, float theta, float phi, Rev& out) {
+    Rot2 kf = rot2_kernel_fwd(theta, phi);
+    v3 mn = mk3(FLT_MAX, FLT_MAX, FLT_MAX), mx = mk3(FLT_MIN, FLT_MIN, FLT_MIN);
+    for (int c = 0; c < 8; c++) {
+        v3 in = mk3(lim.x * (float)(c >> 2 & 1), lim.y * (float)(c >> 1 & 1), lim.z * (float)(c & 1));
+        v3 o = rot2(kf, in);
+        mn.x = std::min(mn.x, o.x); mn.y = std::min(mn.y, o.y); mn.z = std::min(mn.z, o.z);
+        mx.x = std::max(mx.x, o.x); mx.y = std::max(mx.y, o.y); mx.z = std::max(mx.z, o.z);
+    }
+    out.size[0] = (uint32_t)std::ceil(std::fabs(mx.y - mn.y) / p.rev_longspacing);
+    out.size[1] = (uint32_t)std::ceil(std::fabs(mx.z - mn.z) / p.rev_latspacing);
+    out.size[2] = (uint32_t)std::ceil(std::fabs(mx.x - mn.x) / p.rev_latspacing);
+    out.minc = mn;
+    return !(out.size[0] > max_rev[0] || out.size[1] > max_rev[1] || out.size[2] > max_rev[2]);
+}
+
+float comp(v3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// One active beamlet of a beam.
+struct BeamletPlan {
+    int fidx;
+    v3 start, end;
+    float theta, phi;  // divergence angles of the beamlet axis
+    float len;
+    v3 pst;
+};
+
+struct BeamPlan {
+    int beam;                     // position in the request
+    const cccs_beam* b;
+    std::vector<BeamletPlan> bl;
+    int pd[3];
+    std::vector<int> bsizes;      // beamlets per batch
+};
+
+// calcBeamletAnchors (dosecalc-beamlet/nvbbRayConvolve_host.cu:180-280): beamlet isocentre,
+// divergence angles and the entry/exit points of the beamlet axis through the calc_bbox.
+bool beamlet_anchor(const cccs_problem& p, const cccs_beam& b, int idx, BeamletPlan& o, v3& biso) {
+    const v3 src = ld3(b.source), iso = ld3(b.isocenter);
+    const int fx = (int)b.fmap_size[0], fy = (int)b.fmap_size[1];
+    const int bx = idx % fx, by = idx / fx;
+    const v3 bdiff = mk3(b.beamlet_size[0] * (-0.5f * (float)fx + (float)bx + 0.5f), 0.f,
+                         b.beamlet_size[1] * (-0.5f * (float)fy + (float)by + 0.5f));
+    biso = iso + rot2(rot2_beam_inv(b.azimuth, b.zenith, b.coll), bdiff);
+    o.theta = std::acos(len3(iso - src) / len3(biso - src));
+    o.phi = std::atan2(bdiff.z, bdiff.x);
+    const v3 voxel = ld3(p.voxel), dstart = ld3(p.start);
+    const v3 bb_begin = dstart + voxel * ldu3(p.bbox_start);
+    const v3 bb_end = bb_begin + voxel * ldu3(p.bbox_size);
+    const double buffer = 0.05;
+    const v3 d = biso - src;
+    v3 hits[6];
+    for (int ax = 0; ax < 3; ax++) {
+        const double den = (double)comp(d, ax) + 1e-12;
+        const double a1 = (double)(comp(bb_begin, ax) - comp(src, ax)) / den;
+        const double a2 = (double)(comp(bb_end, ax) - comp(src, ax)) / den;
+        hits[2 * ax] = src + (float)a1 * d;
+        hits[2 * ax + 1] = src + (float)a2 * d;
+    }
+    double dmin = std::numeric_limits<double>::max(), dmax = std::numeric_limits<double>::min();
+    o.start = mk3(FLT_MAX, FLT_MAX, FLT_MAX);
+    o.end = mk3(FLT_MIN, FLT_MIN, FLT_MIN);
+    for (int ax = 0; ax < 3; ax++)
+        for (int s = 0; s < 2; s++) {
+            const v3 h = hits[2 * ax + s];
+            const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+            if (comp(h, a1) + buffer >= comp(bb_begin, a1) && comp(h, a1) - buffer <= comp(bb_end, a1) &&
+                comp(h, a2) + buffer >= comp(bb_begin, a2) && comp(h, a2) - buffer <= comp(bb_end, a2)) {
+                const float dist = len3(h - src);
+                if (dist < dmin) { o.start = h; dmin = len3(o.start - src); }
+                if (dist > dmax) { o.end = h; dmax = len3(o.end - src); }
+            }
+        }
+    return dmin < (double)std::numeric_limits<float>::max() && dmax > (double)std::numeric_limits<float>::min();
+}
+
+// Pillar sizes, origins and batch sizes of one beam (host.cu:575-834).
+int plan_beam(const cccs_ctx* ctx, const cccs_beam& b, const float* fmap, int beam_index, BeamPlan& P) {
+    const cccs_problem& p = ctx->prob;
+    P.beam = beam_index;
+    P.b = &b;
+    P.bl.clear();
+    P.bsizes.clear();
+    const int nf = (int)(b.fmap_size[0] * b.fmap_size[1]);
+    const v3 rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    const v3 src = ld3(b.source);
+    float max_len = 0.f, mbx = 0.f, mby = 0.f;
+    for (int i = 0; i < nf; i++) {
+        if (!(fmap[i] > 0.f)) continue;
+        BeamletPlan o;
+        o.fidx = i;
+        v3 biso;
+        if (!beamlet_anchor(p, b, i, o, biso)) return set_error(CCCS_EGEOM, "beamlet axis misses the calc_bbox");
+        o.len = len3(o.end - o.start);
+        max_len = std::max(max_len, o.len);
+        const float num = len3(o.end - src), den = len3(biso - src);
+        mbx = std::max(mbx, b.beamlet_size[0] * num / den);
+        mby = std::max(mby, b.beamlet_size[1] * num / den);
+        P.bl.push_back(o);
+    }
+    const int n = (int)P.bl.size();
+    if (n == 0) return CCCS_OK;
+    const float ke = p.kernel_extent;
+    const float plong = max_len + 2.f * ke + (float)kWall * rvs.y;
+    const float ptx = mbx + 2.f * ke + (float)kWall * rvs.x;
+    const float pty = mby + 2.f * ke + (float)kWall * rvs.x;
+    const v3 expand = rvs - mk3(std::fmod(ptx, rvs.x), std::fmod(plong, rvs.y), std::fmod(pty, rvs.z));
+    const v3 psize = mk3(ptx, plong, pty) + expand + 2.f * (float)kBuffer * rvs;
+    const v3 pdf = psize / rvs;
+    P.pd[0] = (int)pdf.x; P.pd[1] = (int)pdf.y; P.pd[2] = (int)pdf.z;
+    const Rot2 binv = rot2_beam_inv(b.azimuth, b.zenith, b.coll);
+    for (BeamletPlan& o : P.bl) {
+        v3 go = mk3(-0.5f * (psize.x + (float)kWall * rvs.x), -0.5f * (psize.y - o.len + (float)kWall * rvs.y),
+                    -0.5f * (psize.z + (float)kWall * rvs.z));
+        go = rot2(binv, rot1(beamlet_rot(o.theta, o.phi, true), go));
+        o.pst = o.start + go;
+    }
+    uint32_t mr[3] = {p.max_rev_size[0], p.max_rev_size[1], p.max_rev_size[2]};
+    if (mr[0] == 0) mr[0] = mr[1] = mr[2] = 500;  // dosecalc-preprocess/argparser.h:29
+    for (int nb = 1;; nb++) {
+        if (nb > kMaxBatches) return set_error(CCCS_ENOMEM, "beamlets do not fit max_rev_size in 100 batches");
+        P.bsizes.assign(nb, n / nb);
+        for (int r = 0, rem = n - (n / nb) * nb; rem > 0; r++, rem--) P.bsizes[r % nb]++;
+        bool ok = true;
+        for (int bi = 0; bi < nb && ok; bi++) {
+            const int m = P.bsizes[bi];
+            const int npx = (int)std::ceil(std::sqrt((float)m));
+            const int npy = (int)std::ceil((float)m / (float)npx);
+            const v3 gsize = mk3((float)(npx * P.pd[0] + kWall), (float)(P.pd[1] + kWall), (float)(npy * P.pd[2] + kWall)) * rvs;
+            for (int r = 0; r < ctx->nrays && ok; r++) {
+                Rev rv;
+                const float th = ctx->theta[r] * 3.14159265358979323846f / 180.f;
+                const float ph = ctx->phi[r] * 3.14159265358979323846f / 180.f;
+                if (!find_rev(p, mr, gsize, th, ph, rv)) ok = false;
+                else if (rv.size[0] > (uint32_t)kMaxLine) return set_error(CCCS_EGEOM, "REV line longer than 1024 samples");
+            }
+        }
+        if (ok) break;
+    }
+    return CCCS_OK;
+}
+
+struct BatchRef {
+    int plan;   // index into the beam plans
+    int first;  // first beamlet of the plan in this batch
+    int n;
+};
+
+// Device tables and sizes of one job.
+struct Job {
+    std::vector<BatchDev> batches;
+    std::vector<BeamletDev> beamlets;
+    std::vector<BlConeDev> cones;          // [ray][batch]
+    std::vector<uint64_t> ray_lines;       // lines per ray
+    uint64_t max_lines = 0, max_rev = 0, pdose = 0, arena = 0;
+    uint32_t tiles = 0;
+};
+
+// Calc_bbox-relative voxel box of one pillar's unpacked region (rim excluded), widened by 2.
+void pillar_box(const cccs_problem& p, const cccs_beam& b, const BeamletPlan& o, const int* pd, int* box0, int* boxn) {
+    const v3 rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    const Rot2 binv = rot2_beam_inv(b.azimuth, b.zenith, b.coll);
+    const Rot1 r = beamlet_rot(o.theta, o.phi, true);
+    const float lo = (float)(kWall + kBuffer) - 0.5f;
+    double mn[3] = {1e30, 1e30, 1e30}, mx[3] = {-1e30, -1e30, -1e30};
+    for (int c = 0; c < 8; c++) {
+        v3 q = mk3((c & 1) ? (float)(pd[0] - kBuffer) - 0.5f : lo, (c & 2) ? (float)(pd[1] - kBuffer) - 0.5f : lo,
+                   (c & 4) ? (float)(pd[2] - kBuffer) - 0.5f : lo);
+        v3 g = rot2(binv, rot1(r, q * rvs)) + o.pst;
+        for (int k = 0; k < 3; k++) {
+            const double idx = ((double)comp(g, k) - p.start[k]) / p.voxel[k] - (double)p.bbox_start[k];
+            mn[k] = std::min(mn[k], idx);
+            mx[k] = std::max(mx[k], idx);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        const long a = std::max(0L, (long)std::floor(mn[k]) - 2);
+        const long e = std::min((long)p.bbox_size[k], (long)std::ceil(mx[k]) + 3);
+        box0[k] = (int)a;
+        boxn[k] = (int)std::max(0L, e - a);
+    }
+}
+
+int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vector<BatchRef>& refs,
+              const std::vector<const float*>& d_fmaps, const std::vector<const float*>& d_tgrids, Job& J) {
+    const cccs_problem& p = ctx->prob;
+    const v3 rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    const int nb = (int)refs.size(), nrays = ctx->nrays;
+    J = Job();
+    J.batches.resize(nb);
+    J.cones.resize((size_t)nrays * nb);
+    J.ray_lines.assign(nrays, 0);
+    std::vector<uint64_t> ray_rev(nrays, 0);
+    uint32_t mr[3] = {p.max_rev_size[0], p.max_rev_size[1], p.max_rev_size[2]};
+    if (mr[0] == 0) mr[0] = mr[1] = mr[2] = 500;
+    for (int bi = 0; bi < nb; bi++) {
+        const BatchRef& R = refs[bi];
+        const BeamPlan& P = plans[R.plan];
+        const cccs_beam& b = *P.b;
+        BatchDev& B = J.batches[bi];
+        B.beam = make_beam_dev(b);
+        B.beam_inv = rot2_beam_inv(b.azimuth, b.zenith, b.coll);
+        B.fmap = d_fmaps[R.plan];
+        B.tgrid = d_tgrids.empty() ? nullptr : d_tgrids[R.plan];
+        B.n = R.n;
+        B.npx = (int)std::ceil(std::sqrt((float)R.n));
+        B.npy = (int)std::ceil((float)R.n / (float)B.npx);
+        for (int k = 0; k < 3; k++) B.pd[k] = P.pd[k];
+        B.gd[0] = B.npx * P.pd[0] + kWall;
+        B.gd[1] = P.pd[1] + kWall;
+        B.gd[2] = B.npy * P.pd[2] + kWall;
+        B.beamlet0 = (int)J.beamlets.size();
+        B.pdose = J.pdose;
+        J.pdose += (uint64_t)B.gd[0] * B.gd[1] * B.gd[2];
+        B.tiles[0] = (B.gd[1] + 15) / 16;
+        B.tiles[1] = (B.gd[2] + 3) / 4;
+        B.tiles[2] = (B.gd[0] + 3) / 4;
+        B.tile0 = J.tiles;
+        const uint64_t nt = (uint64_t)J.tiles + (uint64_t)B.tiles[0] * B.tiles[1] * B.tiles[2];
+        if (nt > 0xffffffffULL) return set_error(CCCS_EGEOM, "packed grids too large for one job");
+        J.tiles = (uint32_t)nt;
+        for (int q = 0; q < R.n; q++) {
+            const BeamletPlan& o = P.bl[R.first + q];
+            BeamletDev d;
+            d.pst = o.pst;
+            d.inv = beamlet_rot(o.theta, o.phi, true);
+            d.fwd = beamlet_rot(o.theta, o.phi, false);
+            d.fidx = o.fidx;
+            d.batch = bi;
+            d.pix = q % B.npx;
+            d.piy = q / B.npx;
+            pillar_box(p, b, o, P.pd, d.box0, d.boxn);
+            d.arena = J.arena;
+            J.arena += (uint64_t)d.boxn[0] * d.boxn[1] * d.boxn[2];
+            J.beamlets.push_back(d);
+        }
+        const v3 gsize = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]) * rvs;
+        for (int r = 0; r < nrays; r++) {
+            const float th = ctx->theta[r] * 3.14159265358979323846f / 180.f;
+            const float ph = ctx->phi[r] * 3.14159265358979323846f / 180.f;
+            Rev rv;
+            if (!find_rev(p, mr, gsize, th, ph, rv)) return set_error(CCCS_EGEOM, "REV grid exceeds max_rev_size");
+            if (rv.size[0] > (uint32_t)kMaxLine) return set_error(CCCS_EGEOM, "REV line longer than 1024 samples");
+            BlConeDev& c = J.cones[(size_t)r * nb + bi];
+            c.kinv = rot2_kernel_inv(th, ph);
+            c.kfwd = rot2_kernel_fwd(th, ph);
+            c.kinv_m = to_f(dm_kernel_inv(th, ph));
+            c.minc = rv.minc;
+            c.nx = (int)rv.size[0];
+            c.ny = (int)rv.size[1];
+            c.nz = (int)rv.size[2];
+            c.batch = bi;
+            c.line0 = J.ray_lines[r];
+            J.ray_lines[r] += (uint64_t)c.ny * c.nz;
+            ray_rev[r] += (uint64_t)c.nx * c.ny * c.nz;
+        }
+    }
+    for (int r = 0; r < nrays; r++) {
+        J.max_lines = std::max(J.max_lines, J.ray_lines[r]);
+        J.max_rev = std::max(J.max_rev, ray_rev[r]);
+    }
+    if (J.max_lines > 0x7fffffffULL || J.max_rev > 0xffffffffULL)
+        return set_error(CCCS_EGEOM, "REV grids of one cone exceed 2^32 samples in one job");
+    return CCCS_OK;
+}
+
+BlArgs base_args(cccs_ctx* ctx, int fast) {
+    const cccs_problem& p = ctx->prob;
+    BlArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.dens = ctx->d_dens;
+    a.tables = ctx->d_tables;
+    a.ntheta = (int)p.ntheta;
+    a.nradii = (int)p.nradii;
+    a.nphi = (int)p.nphi;
+    a.delr = p.rev_longspacing;
+    a.rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    a.rrv = mk3(1.f / a.rvs.x, 1.f / a.rvs.y, 1.f / a.rvs.z);
+    a.vd = ctx->vol;
+    a.spec = ctx->spec;
+    a.bhc = p.beamhard_correct;
+    a.fast = fast;
+    a.tg[0] = (int)p.size[0] + 2;
+    a.tg[1] = (int)p.size[1] + 2;
+    a.tg[2] = (int)p.size[2] + 2;
+    return a;
+}
+
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)std::min<uint64_t>((n + block - 1) / block, 1u << 20); }
+
+// Unit-TERMA grids of the job's beams (fast mode).
+int run_tgrids(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, std::vector<const float*>& d_tgrids) {
+    const cccs_problem& p = ctx->prob;
+    const uint64_t ng = (uint64_t)(p.size[0] + 2) * (p.size[1] + 2) * (p.size[2] + 2);
+    CCCS_TRY(ctx->bl_tgrid.ensure(sizeof(float) * ng * std::max<size_t>(1, plans.size())));
+    BlArgs a = base_args(ctx, 1);
+    d_tgrids.resize(plans.size());
+    for (size_t i = 0; i < plans.size(); i++) {
+        float* g = ctx->bl_tgrid.as<float>() + ng * i;
+        d_tgrids[i] = g;
+        const unsigned blocks = (unsigned)((ng + 255) / 256);
+        uint32_t* cnt = prof_counter(ctx, blocks);
+        int tok = prof_begin(ctx, K_BL_TERMA);
+        hipLaunchKernelGGL(bl_tgrid_kernel, dim3(blocks), dim3(256), 0, ctx->stream, a, make_beam_dev(*plans[i].b), g, cnt);
+        prof_end(ctx, tok, 4.0 * (double)ctx->vol.nx * ctx->vol.ny * ctx->vol.nz + 4.0 * (double)ng, 0.0, ng);
+        CCCS_HIP(hipGetLastError());
+    }
+    return CCCS_OK;
+}
+
+// Cone passes of one job: every cone direction over every batch, dose into the packed grids.
+int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
+    const cccs_problem& p = ctx->prob;
+    const int nb = (int)J.batches.size(), nrays = ctx->nrays;
+    // device tables: batches | beamlets | cones
+    const size_t sz_b = sizeof(BatchDev) * nb, sz_l = sizeof(BeamletDev) * J.beamlets.size(),
+                 sz_c = sizeof(BlConeDev) * J.cones.size();
+    const size_t o_l = (sz_b + 255) & ~size_t(255), o_c = (o_l + sz_l + 255) & ~size_t(255);
+    const size_t total = o_c + sz_c;
+    CCCS_TRY(ctx->bl_tab.ensure(total));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));  // the pinned staging buffer may still be in use
+    CCCS_TRY(ctx->h_bl.ensure(total));
+    uint8_t* h = ctx->h_bl.as<uint8_t>();
+    std::memcpy(h, J.batches.data(), sz_b);
+    std::memcpy(h + o_l, J.beamlets.data(), sz_l);
+    std::memcpy(h + o_c, J.cones.data(), sz_c);
+    uint8_t* d = ctx->bl_tab.as<uint8_t>();
+    CCCS_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, ctx->stream));
+    CCCS_TRY(ctx->bl_pdose.ensure(sizeof(float) * std::max<uint64_t>(1, J.pdose)));
+    CCCS_TRY(ctx->lines.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, J.max_lines)));
+    CCCS_TRY(ctx->rev.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+    BlArgs a = base_args(ctx, fast);
+    a.batches = reinterpret_cast<const BatchDev*>(d);
+    a.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l);
+    a.lines = ctx->lines.as<LineMeta>();
+    a.rev = ctx->rev.as<float>();
+    a.pdose = ctx->bl_pdose.as<float>();
+    a.nbatch = nb;
+    hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for(J.pdose, 256)), dim3(256), 0, ctx->stream, a.pdose, J.pdose, 0.f);
+    CCCS_HIP(hipGetLastError());
+    const int half = (int)p.ntheta / 2;
+    for (int r = 0; r < nrays; r++) {
+        a.cones = reinterpret_cast<const BlConeDev*>(d + o_c) + (size_t)r * nb;
+        a.nlines = J.ray_lines[r];
+        a.rowf = r % half;
+        a.rowb = (int)p.ntheta - 1 - (r % half);
+        a.counts = nullptr;
+        if (a.nlines == 0) continue;
+        hipLaunchKernelGGL(bl_interval_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a);
+        CCCS_HIP(hipGetLastError());
+        CCCS_TRY(launch_line_scan(ctx, a.lines, a.nlines));
+        uint32_t maxn = 1;
+        for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
+        a.counts = prof_counter(ctx, a.nlines);
+        int tok = prof_begin(ctx, K_BL_CONE);
+        hipLaunchKernelGGL(bl_cone_kernel, dim3((unsigned)a.nlines), dim3(64), sizeof(float2) * maxn, ctx->stream, a);
+        prof_end(ctx, tok, 0.0, 0.0, a.nlines);
+        CCCS_HIP(hipGetLastError());
+        a.counts = nullptr;
+        int tk = prof_begin(ctx, K_BL_BEV);
+        hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
+        prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
+        CCCS_HIP(hipGetLastError());
+    }
+    return CCCS_OK;
+}
+
+int run_job_unpack(cccs_ctx* ctx, const Job& J) {
+    const cccs_problem& p = ctx->prob;
+    CCCS_TRY(ctx->bl_arena.ensure(sizeof(float) * std::max<uint64_t>(1, J.arena)));
+    const size_t sz_b = sizeof(BatchDev) * J.batches.size();
+    const size_t o_l = (sz_b + 255) & ~size_t(255);
+    UnpackArgs u;
+    u.batches = ctx->bl_tab.as<BatchDev>();
+    u.beamlets = reinterpret_cast<const BeamletDev*>(ctx->bl_tab.as<uint8_t>() + o_l);
+    u.pdose = ctx->bl_pdose.as<float>();
+    u.arena = ctx->bl_arena.as<float>();
+    u.nbeamlets = (int)J.beamlets.size();
+    u.narena = J.arena;
+    u.start = ld3(p.start);
+    u.voxel = ld3(p.voxel);
+    u.rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    for (int k = 0; k < 3; k++) u.bbs[k] = (int)p.bbox_start[k];
+    if (J.arena == 0) return CCCS_OK;
+    const uint64_t blocks = (J.arena + 255) / 256;
+    if (blocks > 0x7fffffffULL) return set_error(CCCS_EGEOM, "column arena too large for one job");
+    int tok = prof_begin(ctx, K_BL_UNPACK);
+    hipLaunchKernelGGL(bl_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, u);
+    prof_end(ctx, tok, 8.0 * (double)J.arena, 0.0, J.arena);
+    CCCS_HIP(hipGetLastError());
+    return CCCS_OK;
+}
+
+// Splits the beams' batches into jobs whose device footprint stays within the budget.
+size_t job_bytes(const Job& J) {
+    return sizeof(float) * (J.pdose + J.max_rev + 4 * J.arena) + sizeof(LineMeta) * J.max_lines;
+}
+
+}  // namespace
+
+}  // namespace cccs
 
 using namespace cccs;
 
-extern "C" int cccs_beamlet_dose(cccs_ctx*, const cccs_beam*, int, const float* const*, float, int,
-                                 cccs_sparse_sink, void*) {
-    return set_error(CCCS_EINVAL, "beamlet path not built yet");
+// =================================================================================================
+// C-ABI
+
+namespace {
+
+// Uploads the fluence maps of the plans (one device block, pinned staging).
+int upload_fmaps(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const float* const* fluence,
+                 std::vector<const float*>& d_fmaps) {
+    size_t total = 0;
+    std::vector<size_t> off(plans.size());
+    for (size_t i = 0; i < plans.size(); i++) {
+        off[i] = total;
+        total += (size_t)plans[i].b->fmap_size[0] * plans[i].b->fmap_size[1];
+    }
+    CCCS_TRY(ctx->bl_fmap.ensure(sizeof(float) * std::max<size_t>(1, total)));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_TRY(ctx->h_bl_out.ensure(sizeof(float) * std::max<size_t>(1, total)));
+    float* h = ctx->h_bl_out.as<float>();
+    for (size_t i = 0; i < plans.size(); i++)
+        std::memcpy(h + off[i], fluence[plans[i].beam],
+                    sizeof(float) * plans[i].b->fmap_size[0] * plans[i].b->fmap_size[1]);
+    CCCS_HIP(hipMemcpyAsync(ctx->bl_fmap.p, h, sizeof(float) * total, hipMemcpyHostToDevice, ctx->stream));
+    d_fmaps.resize(plans.size());
+    for (size_t i = 0; i < plans.size(); i++) d_fmaps[i] = ctx->bl_fmap.as<float>() + off[i];
+    return CCCS_OK;
 }
 
-extern "C" int cccs_beamlet_dense(cccs_ctx*, const cccs_beam*, const float*, int, float*, int, int*) {
-    return set_error(CCCS_EINVAL, "beamlet path not built yet");
+// Runs the cone passes and the unpack of one job (the arena then holds every beamlet's box).
+int run_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vector<BatchRef>& refs,
+            const std::vector<const float*>& d_fmaps, const std::vector<const float*>& d_tgrids, int fast, Job& J) {
+    CCCS_TRY(build_job(ctx, plans, refs, d_fmaps, d_tgrids, J));
+    CCCS_TRY(run_job_cones(ctx, J, fast));
+    return run_job_unpack(ctx, J);
+}
+
+std::vector<BatchRef> all_batches(const std::vector<BeamPlan>& plans) {
+    std::vector<BatchRef> refs;
+    for (size_t i = 0; i < plans.size(); i++) {
+        int first = 0;
+        for (int n : plans[i].bsizes) {
+            refs.push_back(BatchRef{(int)i, first, n});
+            first += n;
+        }
+    }
+    return refs;
+}
+
+}  // namespace
+
+extern "C" int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int mode, float* out,
+                                  int max_beamlets, int* nactive) {
+    clear_error();
+    if (!ctx || !beam || !fluence || !out) return set_error(CCCS_EINVAL, "null argument");
+    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST) return set_error(CCCS_EINVAL, "unknown beamlet mode");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    CCCS_TRY(check_beam(ctx, *beam));
+    std::vector<BeamPlan> plans(1);
+    CCCS_TRY(plan_beam(ctx, *beam, fluence, 0, plans[0]));
+    const int n = (int)plans[0].bl.size();
+    if (nactive) *nactive = n;
+    if (n == 0) return CCCS_OK;
+    if (n > max_beamlets) return set_error(CCCS_EINVAL, "more active beamlets than max_beamlets");
+    const cccs_problem& p = ctx->prob;
+    const size_t nbb = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
+    std::memset(out, 0, sizeof(float) * nbb * n);
+    std::vector<const float*> d_fmaps, d_tgrids;
+    const float* fl[1] = {fluence};
+    CCCS_TRY(upload_fmaps(ctx, plans, fl, d_fmaps));
+    if (mode == CCCS_BEAMLET_FAST) CCCS_TRY(run_tgrids(ctx, plans, d_tgrids));
+    // one job per batch, as the reference processes its batches
+    int done = 0;
+    for (const BatchRef& R : all_batches(plans)) {
+        Job J;
+        CCCS_TRY(run_job(ctx, plans, std::vector<BatchRef>{R}, d_fmaps, d_tgrids, mode == CCCS_BEAMLET_FAST, J));
+        std::vector<float> arena(J.arena);
+        CCCS_HIP(hipStreamSynchronize(ctx->stream));
+        if (J.arena) CCCS_HIP(hipMemcpy(arena.data(), ctx->bl_arena.p, sizeof(float) * J.arena, hipMemcpyDeviceToHost));
+        for (int q = 0; q < R.n; q++) {
+            const BeamletDev& d = J.beamlets[q];
+            float* col = out + (size_t)(done + q) * nbb;
+            const float* src = arena.data() + d.arena;
+            for (int i = 0; i < d.boxn[0]; i++)
+                for (int j = 0; j < d.boxn[1]; j++)
+                    for (int k = 0; k < d.boxn[2]; k++) {
+                        const size_t X = (size_t)(d.box0[0] + i), Y = (size_t)(d.box0[1] + j), Z = (size_t)(d.box0[2] + k);
+                        col[X + p.bbox_size[0] * (Y + p.bbox_size[1] * Z)] =
+                            src[((size_t)i * d.boxn[1] + j) * d.boxn[2] + k];
+                    }
+        }
+        done += R.n;
+    }
+    return CCCS_OK;
+}
+
+extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, const float* const* fluence,
+                                 float threshold, int mode, cccs_sparse_sink sink, void* user) {
+    clear_error();
+    if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || (!fluence && nbeams > 0) || !sink)
+        return set_error(CCCS_EINVAL, "null argument");
+    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST) return set_error(CCCS_EINVAL, "unknown beamlet mode");
+    if (!(threshold >= 0.f)) return set_error(CCCS_EINVAL, "threshold must be >= 0");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    const cccs_problem& p = ctx->prob;
+    std::vector<BeamPlan> plans;
+    plans.reserve(nbeams);
+    for (int i = 0; i < nbeams; i++) {
+        if (!fluence[i]) return set_error(CCCS_EINVAL, "null fluence map");
+        CCCS_TRY(check_beam(ctx, beams[i]));
+        BeamPlan P;
+        CCCS_TRY(plan_beam(ctx, beams[i], fluence[i], i, P));
+        if (!P.bl.empty()) plans.push_back(std::move(P));
+    }
+    if (plans.empty()) return CCCS_OK;
+    CCCS_TRY(ctx->h_small.ensure(64));
+    std::vector<const float*> d_fmaps, d_tgrids;
+    CCCS_TRY(upload_fmaps(ctx, plans, fluence, d_fmaps));
+    if (mode == CCCS_BEAMLET_FAST) CCCS_TRY(run_tgrids(ctx, plans, d_tgrids));
+    const std::vector<BatchRef> refs = all_batches(plans);
+    const size_t nvox = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
+    const size_t budget = size_t(24) << 30;  // device bytes per job
+    size_t next = 0;
+    std::vector<uint64_t> keys;
+    std::vector<float> vals;
+    while (next < refs.size()) {
+        // grow the job batch by batch while it fits the budget
+        std::vector<BatchRef> sel{refs[next]};
+        Job J;
+        CCCS_TRY(build_job(ctx, plans, sel, d_fmaps, d_tgrids, J));
+        size_t k = next + 1;
+        while (k < refs.size()) {
+            std::vector<BatchRef> trial = sel;
+            trial.push_back(refs[k]);
+            Job T;
+            if (build_job(ctx, plans, trial, d_fmaps, d_tgrids, T) != CCCS_OK || job_bytes(T) > budget) {
+                clear_error();
+                break;
+            }
+            sel.swap(trial);
+            J = std::move(T);
+            k++;
+        }
+        next = k;
+        CCCS_TRY(run_job_cones(ctx, J, mode == CCCS_BEAMLET_FAST));
+        CCCS_TRY(run_job_unpack(ctx, J));
+        const int nbl = (int)J.beamlets.size();
+        CCCS_TRY(ctx->bl_cnt.ensure(sizeof(uint32_t) * 4 * nbl + sizeof(uint64_t) * (nbl + 1) + 256));
+        uint32_t* stat = ctx->bl_cnt.as<uint32_t>();
+        uint64_t* offs = reinterpret_cast<uint64_t*>(ctx->bl_cnt.as<uint8_t>() + ((sizeof(uint32_t) * 4 * nbl + 255) & ~size_t(255)));
+        const size_t sz_b = sizeof(BatchDev) * J.batches.size();
+        const BeamletDev* dbl = reinterpret_cast<const BeamletDev*>(ctx->bl_tab.as<uint8_t>() + ((sz_b + 255) & ~size_t(255)));
+        int tok = prof_begin(ctx, K_BL_COMPACT);
+        hipLaunchKernelGGL(bl_colstat_kernel, dim3(nbl), dim3(256), 0, ctx->stream, dbl, ctx->bl_arena.as<float>(), threshold, stat);
+        hipLaunchKernelGGL(bl_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, stat, nbl, offs);
+        CCCS_HIP(hipGetLastError());
+        // totals decide the output size
+        std::vector<uint32_t> hstat(4 * (size_t)nbl);
+        uint64_t nnz_total = 0;
+        CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, offs + nbl, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        CCCS_HIP(hipStreamSynchronize(ctx->stream));
+        std::memcpy(&nnz_total, ctx->h_small.p, sizeof(uint64_t));
+        CCCS_TRY(ctx->bl_keys.ensure(sizeof(uint64_t) * std::max<uint64_t>(1, nnz_total)));
+        CCCS_TRY(ctx->bl_vals.ensure(sizeof(float) * std::max<uint64_t>(1, nnz_total)));
+        hipLaunchKernelGGL(bl_compact_kernel, dim3(nbl), dim3(256), 0, ctx->stream, dbl, ctx->bl_arena.as<float>(),
+                           threshold, stat, offs, (int)p.size[0], (int)p.size[1], (int)p.bbox_start[0],
+                           (int)p.bbox_start[1], (int)p.bbox_start[2], ctx->bl_keys.as<uint64_t>(), ctx->bl_vals.as<float>());
+        prof_end(ctx, tok, 8.0 * (double)J.arena + 12.0 * (double)nnz_total, 0.0, J.arena);
+        CCCS_HIP(hipGetLastError());
+        keys.resize(nnz_total);
+        vals.resize(nnz_total);
+        CCCS_HIP(hipStreamSynchronize(ctx->stream));
+        CCCS_HIP(hipMemcpy(hstat.data(), stat, sizeof(uint32_t) * 4 * nbl, hipMemcpyDeviceToHost));
+        if (nnz_total) {
+            CCCS_HIP(hipMemcpy(keys.data(), ctx->bl_keys.p, sizeof(uint64_t) * nnz_total, hipMemcpyDeviceToHost));
+            CCCS_HIP(hipMemcpy(vals.data(), ctx->bl_vals.p, sizeof(float) * nnz_total, hipMemcpyDeviceToHost));
+        }
+        uint64_t pos = 0;
+        for (int q = 0; q < nbl; q++) {
+            const BeamletDev& d = J.beamlets[q];
+            const BatchRef& R = sel[d.batch];
+            const BeamPlan& P = plans[R.plan];
+            cccs_sparse_column col;
+            col.beam_index = P.beam;
+            col.beamlet_index = d.fidx;
+            col.nnz = hstat[4 * q];
+            col.keys = keys.data() + pos;
+            col.values = vals.data() + pos;
+            col.perc_nonzero = (float)(100.0 * (double)hstat[4 * q] / (double)nvox);
+            col.perc_dropped = (float)(100.0 * (double)hstat[4 * q + 1] / (double)nvox);
+            pos += hstat[4 * q];
+            if (sink(user, &col) != 0) return set_error(CCCS_EIO, "sparse sink aborted the call");
+        }
+    }
+    return CCCS_OK;
 }

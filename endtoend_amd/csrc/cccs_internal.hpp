@@ -198,12 +198,10 @@ struct SpecDev {
     float flu[kMaxSpec], en[kMaxSpec], mu[kMaxSpec];
 };
 
-// TERMA of one point (cudaSiddon_device, cudaSiddon.cu:117-208).  Returns 0 outside the field
-// or for a zero-weight beamlet; beamletnum >= 0 keeps only that beamlet's footprint.
-CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b, const SpecDev& sp,
-                          const VolDesc& vd, const float* __restrict__ dens, float bhc, int beamletnum,
-                          int* iters) {
-    if (iters) *iters = 0;
+// Fluence weight seen by point p: the beamlet-cell lookup of cudaSiddon_device
+// (cudaSiddon.cu:146-165).  0 outside the map, for a zero-weight cell, or (beamletnum >= 0) for
+// any cell other than beamletnum.
+CCCS_HD float point_fluence(v3 p, const float* __restrict__ fmap, const BeamDev& b, int beamletnum) {
     v3 diff = p - b.src;
     v3 projection = dot3(diff, b.dir_n) * b.dir_n;
     v3 rejection = diff - projection;
@@ -216,6 +214,30 @@ CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b
     int fidx = f2i_rd(fx) + b.fmx * f2i_rd(fz);
     if (fidx < 0 || fidx >= b.fmx * b.fmy || (beamletnum >= 0 && fidx != beamletnum)) return 0.f;
     float fluence = fmap[fidx];
+    return fluence <= 0.f ? 0.f : fluence;
+}
+
+// Spectrum-weighted attenuation sum_e phi_e E_e mu_e exp(-mu_e rpl) for unit fluence
+// (cudaSiddon.cu:190-200 with the fluence factored out).
+CCCS_HD float unit_terma(float rpl, const SpecDev& sp) {
+    float tsum = 0.f;
+    for (int e = 0; e < sp.n; e++) tsum += sp.flu[e] * sp.en[e] * sp.mu[e] * expf(-1.f * sp.mu[e] * rpl);
+    return tsum;
+}
+
+// Inverse-square factor SAD^2 / |p - src|^2 (cudaSiddon.cu:201-204).
+CCCS_HD float inv_square(v3 p, const BeamDev& b) {
+    float d = len3(p - b.src);
+    return (b.sad * b.sad) / (d * d);
+}
+
+// TERMA of one point (cudaSiddon_device, cudaSiddon.cu:117-208).  Returns 0 outside the field
+// or for a zero-weight beamlet; beamletnum >= 0 keeps only that beamlet's footprint.
+CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b, const SpecDev& sp,
+                          const VolDesc& vd, const float* __restrict__ dens, float bhc, int beamletnum,
+                          int* iters) {
+    if (iters) *iters = 0;
+    const float fluence = point_fluence(p, fmap, b, beamletnum);
     if (fluence <= 0.f) return 0.f;
     float rpl = siddon_march(p, vd.start, vd.end, vd.voxel, b.src, dens, vd.nx, vd.ny, vd.nz,
                              [](float, float, float, float) {}, iters);
@@ -224,9 +246,7 @@ CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b
         float this_fluence = fluence * sp.flu[e];
         tsum += this_fluence * sp.en[e] * sp.mu[e] * expf(-1.f * sp.mu[e] * rpl);
     }
-    float d = len3(p - b.src);
-    float invsq = (b.sad * b.sad) / (d * d);
-    return tsum * bhc * invsq;
+    return tsum * bhc * inv_square(p, b);
 }
 
 }  // namespace cccs

@@ -21,13 +21,13 @@ namespace cccs {
 // buffers
 int DevBuf::ensure(size_t need) {
     if (need <= bytes) return CCCS_OK;
+    const size_t want = std::max(need, bytes + bytes / 4);
     if (p) {
         (void)hipDeviceSynchronize();  // queued kernels may still read the old buffer
         (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
-    size_t want = std::max(need, bytes + bytes / 4);
     if (hipMalloc(&p, want) != hipSuccess) {
         p = nullptr;
         (void)hipGetLastError();
@@ -43,8 +43,13 @@ void DevBuf::release() {
 }
 int HostBuf::ensure(size_t need) {
     if (need <= bytes) return CCCS_OK;
-    if (p) { (void)hipHostFree(p); p = nullptr; bytes = 0; }
-    size_t want = std::max(need, bytes + bytes / 4);
+    const size_t want = std::max(need, bytes + bytes / 4);
+    if (p) {
+        (void)hipDeviceSynchronize();  // queued copies may still use the old buffer
+        (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
         p = nullptr;
         (void)hipGetLastError();
@@ -299,6 +304,9 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
     c->dose.release(); c->col.release(); c->lines.release(); c->scan.release();
     c->h_small.release(); c->h_cones.release();
+    c->bl_tab.release(); c->bl_pdose.release(); c->bl_tgrid.release(); c->bl_arena.release();
+    c->bl_cnt.release(); c->bl_keys.release(); c->bl_vals.release(); c->bl_fmap.release();
+    c->h_bl.release(); c->h_bl_out.release();
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (uint32_t* p : c->prof.chunks) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -104,6 +104,9 @@ struct cccs_ctx {
     size_t resident = 0;
     cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan;
     cccs::HostBuf h_small, h_cones;
+    // beamlet path: job tables, packed grids, unit-TERMA grids, column arena, sparse output
+    cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap;
+    cccs::HostBuf h_bl, h_bl_out;
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
 };

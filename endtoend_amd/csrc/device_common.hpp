@@ -47,8 +47,9 @@ struct LineMeta {
 // coordinates (u, v, w) = (sample, line y, line z), texel centres at +0.5, border 0: the same
 // operation sequence as tex3_border on the dense grid, so both give identical results.
 __device__ __forceinline__ float rev_fetch(const LineMeta* __restrict__ meta, const float* __restrict__ rev,
-                                           uint64_t line0, int ny, int nz, float u, float v, float w) {
+                                           uint64_t line0, int nx, int ny, int nz, float u, float v, float w) {
     const float ub = u - 0.5f, vb = v - 0.5f, wb = w - 0.5f;
+    if (!(ub > -1.f && vb > -1.f && wb > -1.f && ub < (float)nx && vb < (float)ny && wb < (float)nz)) return 0.f;
     const float fu = floorf(ub), fv = floorf(vb), fw = floorf(wb);
     const int i = (int)fu, j = (int)fv, k = (int)fw;
     const float a = ub - fu, b = vb - fv, c = wb - fw;
