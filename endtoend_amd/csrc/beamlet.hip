@@ -27,6 +27,8 @@ namespace cccs {
 constexpr int kWall = 2;        // pillar wall, REV samples (PILLAR_GRID, io_data_structs.h:139-160)
 constexpr int kBuffer = 2;      // pillar rim excluded from unpacking (same struct)
 constexpr int kMaxBatches = 100;  // batch-count retry limit of the reference (host.cu:692-834)
+constexpr uint32_t kSubQueues = 64;  // TERMA sub-queues per cone direction
+constexpr uint32_t kLineWaves = 8192;  // waves of the per-line kernels (each takes a stride of lines)
 
 // One Rodrigues rotation with host-evaluated sine/cosine.
 struct Rot1 {
@@ -70,6 +72,15 @@ struct BatchDev {
     uint64_t pdose;        // float offset of its packed dose grid ([pX][pZ][pY], pY fastest)
     uint32_t tile0;        // first gather tile
     int tiles[3];          // gather tiles along pY (16), pZ (4), pX (4)
+    float fc[2];           // beamlet axis in pillar sample coordinates (x, z)
+    float fw;              // bound on the footprint half-width [samples]: farther samples get no TERMA
+};
+
+// A footprint sample whose TERMA needs a Siddon march.
+struct TermaReq {
+    uint32_t slot;     // position in the REV store of the cone direction
+    uint32_t beamlet;  // job beamlet index
+    float x, y, z;     // TERMA point [cm]
 };
 
 // One (cone, batch) REV grid.
@@ -169,6 +180,16 @@ __device__ __forceinline__ float tgrid_fetch(const float* __restrict__ g, const 
     return c0 + w * (c1 - c0);
 }
 
+// fmodf(x, y) for 0 <= x < 2^24 and a positive integer y <= 2^20: n*y is exact, x - n*y is exact,
+// and the fix-ups absorb a quotient rounded across an integer.  Equal to fmodf bit for bit there.
+__device__ __forceinline__ float fmod_pos(float x, float y) {
+    const float n = truncf(x / y);
+    float r = fmaf(-n, y, x);
+    if (r < 0.f) r += y;
+    if (r >= y) r -= y;
+    return r;
+}
+
 // Pillar mapping of one REV sample (cudaBeamletRaytrace, nvbbRayConvolve_device.cu:33-68):
 // false for a wall, a sample outside the packed grid or past the last pillar; otherwise the
 // beamlet (pidx, batch-local) and the pillar-local coordinate inp (already shifted by -0.5).
@@ -181,7 +202,7 @@ __device__ __forceinline__ bool bl_locate(const BlArgs& a, const BlConeDev& c, c
     const int pix = f2i_rd(pb.x / pdf.x), piy = f2i_rd(pb.z / pdf.z);
     pidx = pix + piy * B.npx;
     if (pidx >= B.n) return false;
-    inp = mk3(fmodf(pb.x, pdf.x), fmodf(pb.y, pdf.y), fmodf(pb.z, pdf.z));
+    inp = mk3(fmod_pos(pb.x, pdf.x), fmod_pos(pb.y, pdf.y), fmod_pos(pb.z, pdf.z));
     if (inp.x < (float)kWall || inp.y < (float)kWall || inp.z < (float)kWall) return false;
     inp = inp - 0.5f;
     return true;
@@ -200,66 +221,137 @@ __device__ __forceinline__ float bl_density(const BlArgs& a, const BatchDev& B, 
     return tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, gi.x + 0.5f, gi.y + 0.5f, gi.z + 0.5f);
 }
 
-// PackRowConvolve for destination j (nvbbRayConvolve_device.cu:284-417): walks toward lower and
-// then higher indices inside the pillar segment [slo, shi] that holds j, each walk starting at j
-// itself (the self sample adds only its first half step).  Both walks together take shi-slo+2
-// steps, the same count for every destination of a segment.  The cumulative kernel is linear
-// between two mean radii, so the current interval stays in registers.
-__device__ __forceinline__ float walk_pack(const float2* __restrict__ s, int j, int slo, int shi,
-                                           const float* __restrict__ R, const float* __restrict__ IW,
-                                           const float* __restrict__ rowF, const float* __restrict__ rowB, int nr,
-                                           bool active, uint32_t& iters) {
-    const float rmax = R[nr - 1];
-    const int total = shi - slo + 2;
-    const int nfwd = j - slo + 1;
-    const float* row = rowF;
-    float r = 0.f, last = 0.f, lo = 0.f, hi = R[0], klo = 0.f, slope = rowF[0] * IW[0], sum = 0.f;
-    int rs = 0, k = j + 1, dir = -1;
-    bool alive = active;
-    for (int it = 0; it < total; ++it) {
-        if (it == nfwd) {
-            dir = 1;
-            k = j - 1;
-            row = rowB;
-            r = 0.f; last = 0.f; lo = 0.f; hi = R[0]; klo = 0.f; slope = rowB[0] * IW[0];
-            rs = 0;
-            alive = active;
-        }
-        k += dir;
-        const float2 e = s[k];
-        r += e.x;
-        const bool ok = alive && r <= rmax;
-        if (ok && hi < r) {
-            do {
-                rs++;
-                lo = hi;
-                hi = R[rs];
-                klo = row[rs - 1];
-                slope = (row[rs] - klo) * IW[rs];
-            } while (hi < r && rs < nr - 1);
-        }
-        const float cum = klo + (r - lo) * slope;
-        sum += ok ? (cum - last) * e.y : 0.f;
-        last = cum;
-        if (k != j) r += e.x;
-        iters += ok ? 1u : 0u;
-        alive = ok && rs < nr - 1;
-    }
-    return sum;
+// Cumulative kernel of one row at radiological distance r, linear between radii (the
+// reference's clamped linear texture lookup, nvbbRayConvolve_device.cu:330-352).  s is the
+// current interval (first radius >= r); r must not decrease between calls.
+__device__ __forceinline__ float kc_at(float r, const float* __restrict__ R, const float* __restrict__ IW,
+                                       const float* __restrict__ row, int nr, int& s) {
+    while (s < nr - 1 && R[s] < r) s++;
+    const float lo = s == 0 ? 0.f : R[s - 1];
+    const float klo = s == 0 ? 0.f : row[s - 1];
+    return klo + (r - lo) * ((row[s] - klo) * IW[s]);
 }
 
-// One wave per REV line of one cone direction (all batches of the job), XCD-aware order.
-__global__ void __launch_bounds__(64) bl_cone_kernel(BlArgs a) {
-    extern __shared__ float2 s_line[];
-    __shared__ short s_lo[kMaxLine], s_hi[kMaxLine];
-    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
-    const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
+// Stage 1 of a cone direction, one wave per REV line: pillar mapping of every stored sample and
+// the beamlet-footprint test (fluence cell of the sample == its beamlet).  A line without footprint
+// samples carries no TERMA, so all its cone dose is zero: it is marked empty.  Otherwise the half
+// steps (0.5 * delr * density, -1 for walls) go to H, TERMA slots are zeroed, and the footprint
+// samples are queued for their Siddon TERMA (fast mode evaluates TERMA here instead).
+__global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restrict__ H, TermaReq* __restrict__ q,
+                                                       uint32_t subcap, uint32_t* __restrict__ qcnt) {
+    __shared__ uint8_t s_fp[kMaxLine];
     const int tid = threadIdx.x;
+    // each wave walks a stride of lines; waves append to one of kSubQueues queues (fewer
+    // collisions on the counters)
+    const uint32_t sq = blockIdx.x % kSubQueues;
+    TermaReq* qs = q + (size_t)sq * subcap;
+    for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
     const LineMeta meta = a.lines[line];
+    if (meta.len <= 0) continue;
+    const BlConeDev& c = a.cones[find_cone(a.cones, a.nbatch, line)];
+    const BatchDev& B = a.batches[c.batch];
+    const uint64_t l = line - c.line0;
+    const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
+    const int i0 = meta.i0, m = meta.len;
+    bool any = false;
+    for (int base = 0; base < m; base += 64) {
+        const int j = base + tid;
+        bool fp = false;
+        int pidx;
+        v3 inp;
+        if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
+            fabsf(inp.z - B.fc[1]) <= B.fw) {
+            const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+            fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
+        }
+        if (j < m) s_fp[j] = fp ? 1 : 0;
+        any = any || __ballot(fp) != 0ull;
+    }
+    if (!any) {
+        if (tid == 0) a.lines[line].len = 0;
+        continue;
+    }
+    float* T = a.rev + meta.off;
+    float* Hl = H + meta.off;
+    for (int base = 0; base < m; base += 64) {
+        const int j = base + tid;
+        float h = -1.f, t = 0.f;
+        bool need = false;
+        TermaReq r;
+        int pidx;
+        v3 inp;
+        if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp)) {
+            const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+            h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
+            if (s_fp[j]) {
+                const v3 gcs = bl_terma_point(a, B, bl, inp);
+                if (a.fast) {
+                    const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
+                    const v3 gi = (gcs - a.vd.start) / a.vd.voxel;
+                    t = fl * tgrid_fetch(B.tgrid, a.tg, gi) * a.bhc * inv_square(gcs, B.beam);
+                } else {
+                    need = true;
+                    r.slot = (uint32_t)(meta.off + j);
+                    r.beamlet = (uint32_t)(B.beamlet0 + pidx);
+                    r.x = gcs.x; r.y = gcs.y; r.z = gcs.z;
+                }
+            }
+        }
+        if (j < m) {
+            Hl[j] = h;
+            T[j] = t;
+        }
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            uint32_t basepos = 0;
+            if (tid == 0) basepos = atomicAdd(qcnt + sq, (uint32_t)__popcll(mask));
+            basepos = __shfl(basepos, 0, 64);
+            const uint32_t pos = basepos + (uint32_t)__popcll(mask & ((1ull << tid) - 1ull));
+            if (need && pos < subcap) qs[pos] = r;
+        }
+    }
+    }
+}
+
+// Stage 2: the queued samples' TERMA (cudaSiddon_device with the beamlet filter), one sample per
+// lane, grid-stride over the queue.
+__global__ void __launch_bounds__(256) bl_siddon_kernel(BlArgs a, const TermaReq* __restrict__ q, uint32_t subcap,
+                                                        const uint32_t* __restrict__ qcnt, uint32_t* counts) {
+    // workgroup b serves sub-queue b % kSubQueues
+    const uint32_t sq = blockIdx.x % kSubQueues, per = gridDim.x / kSubQueues;
+    const uint32_t n = min(qcnt[sq], subcap);
+    const TermaReq* qs = q + (size_t)sq * subcap;
+    uint32_t steps = 0;
+    for (uint32_t i = (blockIdx.x / kSubQueues) * 256 + threadIdx.x; i < n; i += per * 256) {
+        const TermaReq r = qs[i];
+        const BeamletDev& bl = a.beamlets[r.beamlet];
+        const BatchDev& B = a.batches[bl.batch];
+        int it = 0;
+        a.rev[r.slot] = point_terma(mk3(r.x, r.y, r.z), B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, &it);
+        steps += (uint32_t)it;
+    }
+    if (counts) block_count(steps, counts, blockIdx.x);
+}
+
+// Stage 3, one wave per non-empty REV line: PackRowConvolve (nvbbRayConvolve_device.cu:284-417)
+// for every sample.  Both walks of destination j run inside the pillar segment [slo, shi] holding
+// j, start at j itself (the self sample adds only its first half step) and stop when the
+// radiological distance passes the last radius or enters the last radial interval.  Only samples
+// with TERMA contribute, and the distance from j to sample k is the difference of the line's
+// radiological prefix sums (FP64), so each destination visits just the footprint samples of its
+// segment, in the reference's walk order, instead of every sample.
+__global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __restrict__ H) {
+    extern __shared__ double s_dyn[];
+    __shared__ short s_lo[kMaxLine], s_hi[kMaxLine], s_fp[kMaxLine];
+    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    const int tid = threadIdx.x;
     uint32_t iters = 0;
-    if (meta.len > 0) {
-        const BlConeDev& c = a.cones[find_cone(a.cones, a.nbatch, line)];
-        const BatchDev& B = a.batches[c.batch];
+    for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
+        const LineMeta meta = a.lines[line];
+        if (meta.len <= 0) continue;
+        const int m = meta.len;
+        double* sC = s_dyn;                                        // radiological centre of each sample
+        float2* sL = reinterpret_cast<float2*>(s_dyn + ((m + 1) & ~1));  // {half step, TERMA}
         const int nr = a.nradii;
         if (tid < nr) {
             const float* K = a.tables;
@@ -269,89 +361,113 @@ __global__ void __launch_bounds__(64) bl_cone_kernel(BlArgs a) {
             sF[tid] = K[(size_t)a.rowf * nr + tid];
             sB[tid] = K[(size_t)a.rowb * nr + tid];
         }
-        const uint64_t l = line - c.line0;
-        const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
-        const int i0 = meta.i0, m = meta.len;
-        // pass 1: pillar mapping and density of every stored sample; samples inside their
-        // beamlet's footprint are queued for TERMA (fast mode evaluates it here)
-        short* s_req = s_lo;  // reused: the segment bounds are computed after the TERMA pass
-        int nreq = 0;
-        for (int base = 0; base < m; base += 64) {
-            const int j = base + tid;
-            float2 e = make_float2(-1.f, 0.f);
-            bool need = false;
-            int pidx;
-            v3 inp;
-            if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp)) {
-                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
-                e.x = 0.5f * (a.delr * bl_density(a, B, bl, inp));
-                const v3 gcs = bl_terma_point(a, B, bl, inp);
-                const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
-                if (fl > 0.f) {
-                    if (a.fast) {
-                        const v3 gi = (gcs - a.vd.start) / a.vd.voxel;
-                        e.y = fl * tgrid_fetch(B.tgrid, a.tg, gi) * a.bhc * inv_square(gcs, B.beam);
-                    } else {
-                        need = true;
-                    }
-                }
-            }
-            if (j < m) s_line[j] = e;
-            const uint64_t mask = __ballot(need);
-            if (need) s_req[nreq + __popcll(mask & ((1ull << tid) - 1ull))] = (short)j;
-            nreq += __popcll(mask);
-        }
+        float* T = a.rev + meta.off;
+        const float* Hl = H + meta.off;
+        for (int j = tid; j < m; j += 64) sL[j] = make_float2(Hl[j], T[j]);
         __syncthreads();
-        // pass 2: the queued samples' Siddon TERMA, one sample per lane
-        for (int r = 0; r < nreq; r += 64) {
-            if (r + tid < nreq) {
-                const int j = s_req[r + tid];
-                int pidx;
-                v3 inp;
-                bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp);
-                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
-                const v3 gcs = bl_terma_point(a, B, bl, inp);
-                s_line[j].y = point_terma(gcs, B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, nullptr);
-            }
-        }
-        __syncthreads();
-        // pillar segments: for each sample the nearest wall on either side (chunked wave scans)
+        // chunked scans: radiological prefix, nearest walls, footprint list
         const int per = (m + 63) / 64;
         const int ca = min(m, tid * per), cb = min(m, ca + per);
-        int last_wall = -1, first_wall = m;
-        for (int k = ca; k < cb; k++)
-            if (s_line[k].x < 0.f) last_wall = k;
+        double csum = 0.0;
+        int last_wall = -1, first_wall = m, nfp = 0;
+        for (int k = ca; k < cb; k++) {
+            const float h = sL[k].x;
+            if (h < 0.f) last_wall = k;
+            else csum += 2.0 * (double)h;
+            if (sL[k].y != 0.f) nfp++;
+        }
         for (int k = cb - 1; k >= ca; k--)
-            if (s_line[k].x < 0.f) first_wall = k;
-        int pre = last_wall, suf = first_wall;  // inclusive prefix max / suffix min over chunks
+            if (sL[k].x < 0.f) first_wall = k;
+        double pc = csum;
+        int pre = last_wall, suf = first_wall, pf = nfp;
         for (int o = 1; o < 64; o <<= 1) {
+            const double uc = __shfl_up(pc, o, 64);
             const int up = __shfl_up(pre, o, 64);
+            const int uf = __shfl_up(pf, o, 64);
             const int dn = __shfl_down(suf, o, 64);
-            if (tid >= o) pre = max(pre, up);
+            if (tid >= o) { pc += uc; pre = max(pre, up); pf += uf; }
             if (tid + o < 64) suf = min(suf, dn);
         }
+        double cacc = pc - csum;
         int left = __shfl_up(pre, 1, 64);
         int right = __shfl_down(suf, 1, 64);
+        int fpos = pf - nfp;
+        const int nfoot = __shfl(pf, 63, 64);
         if (tid == 0) left = -1;
         if (tid == 63) right = m;
         for (int k = ca; k < cb; k++) {
-            if (s_line[k].x < 0.f) left = k;
+            const float h = sL[k].x;
+            const double hh = h < 0.f ? 0.0 : (double)h;
+            sC[k] = cacc + hh;
+            cacc += 2.0 * hh;
+            if (h < 0.f) left = k;
             s_lo[k] = (short)(left + 1);
+            if (sL[k].y != 0.f) s_fp[fpos++] = (short)k;
         }
         for (int k = cb - 1; k >= ca; k--) {
-            if (s_line[k].x < 0.f) right = k;
+            if (sL[k].x < 0.f) right = k;
             s_hi[k] = (short)(right - 1);
         }
         __syncthreads();
-        float* out = a.rev + meta.off;
         const float nphi_f = (float)a.nphi;
+        const float rmax = sR[nr - 1], rlast = sR[nr - 2];
         for (int base = 0; base < m; base += 64) {
-            const int j = min(base + tid, m - 1);
-            const bool act = base + tid < m && s_line[j].x >= 0.f;
-            const int slo = act ? (int)s_lo[j] : j, shi = act ? (int)s_hi[j] : j;
-            const float v = walk_pack(s_line, j, slo, shi, sR, sIW, sF, sB, nr, act, iters);
-            if (base + tid < m) out[j] = act ? v / nphi_f : 0.f;
+            const int j = base + tid;
+            if (j >= m) break;
+            const float2 ej = sL[j];
+            float sum = 0.f;
+            if (ej.x >= 0.f) {
+                const int slo = s_lo[j], shi = s_hi[j];
+                const double cj = sC[j];
+                // first footprint entry >= j
+                int lo = 0, hi = nfoot;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_fp[mid] < j) lo = mid + 1; else hi = mid;
+                }
+                const int p = lo;
+                for (int dir = 0; dir < 2; dir++) {
+                    const float* row = dir == 0 ? sF : sB;
+                    int s = 0;
+                    // the self sample
+                    const float r0 = ej.x;
+                    if (!(r0 <= rmax)) continue;
+                    const float k0 = kc_at(r0, sR, sIW, row, nr, s);
+                    sum += k0 * ej.y;
+                    iters++;
+                    if (r0 > rlast) continue;
+                    if (dir == 0) {
+                        for (int q = p - 1; q >= 0; q--) {
+                            const int k = s_fp[q];
+                            if (k < slo) break;
+                            const float rp = k + 1 == j ? r0 : (float)(cj - sC[k + 1]);
+                            if (rp > rlast) break;
+                            const float rk = (float)(cj - sC[k]);
+                            if (rk > rmax) break;
+                            const float kp = kc_at(rp, sR, sIW, row, nr, s);
+                            const float kk = kc_at(rk, sR, sIW, row, nr, s);
+                            sum += (kk - kp) * sL[k].y;
+                            iters++;
+                        }
+                    } else {
+                        for (int q = (p < nfoot && s_fp[p] == j) ? p + 1 : p; q < nfoot; q++) {
+                            const int k = s_fp[q];
+                            if (k > shi) break;
+                            const float rp = k - 1 == j ? r0 : (float)(sC[k - 1] - cj);
+                            if (rp > rlast) break;
+                            const float rk = (float)(sC[k] - cj);
+                            if (rk > rmax) break;
+                            const float kp = kc_at(rp, sR, sIW, row, nr, s);
+                            const float kk = kc_at(rk, sR, sIW, row, nr, s);
+                            sum += (kk - kp) * sL[k].y;
+                            iters++;
+                        }
+                    }
+                }
+            }
+            T[j] = ej.x >= 0.f ? sum / nphi_f : 0.f;
         }
+        __syncthreads();  // the next line reuses the LDS arrays
     }
     if (a.counts) block_count(iters, a.counts, blockIdx.x);
 }
@@ -602,6 +718,9 @@ struct BeamPlan {
     const cccs_beam* b;
     std::vector<BeamletPlan> bl;
     int pd[3];
+    v3 psize;                     // pillar size [cm]
+    float max_bs;                 // largest divergent beamlet width at a beamlet's exit [cm]
+    float min_exit;               // smallest source-to-exit distance [cm]
     std::vector<int> bsizes;      // beamlets per batch
 };
 
@@ -656,7 +775,7 @@ int plan_beam(const cccs_ctx* ctx, const cccs_beam& b, const float* fmap, int be
     const int nf = (int)(b.fmap_size[0] * b.fmap_size[1]);
     const v3 rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
     const v3 src = ld3(b.source);
-    float max_len = 0.f, mbx = 0.f, mby = 0.f;
+    float max_len = 0.f, mbx = 0.f, mby = 0.f, min_exit = FLT_MAX;
     for (int i = 0; i < nf; i++) {
         if (!(fmap[i] > 0.f)) continue;
         BeamletPlan o;
@@ -668,6 +787,7 @@ int plan_beam(const cccs_ctx* ctx, const cccs_beam& b, const float* fmap, int be
         const float num = len3(o.end - src), den = len3(biso - src);
         mbx = std::max(mbx, b.beamlet_size[0] * num / den);
         mby = std::max(mby, b.beamlet_size[1] * num / den);
+        min_exit = std::min(min_exit, num);
         P.bl.push_back(o);
     }
     const int n = (int)P.bl.size();
@@ -680,6 +800,9 @@ int plan_beam(const cccs_ctx* ctx, const cccs_beam& b, const float* fmap, int be
     const v3 psize = mk3(ptx, plong, pty) + expand + 2.f * (float)kBuffer * rvs;
     const v3 pdf = psize / rvs;
     P.pd[0] = (int)pdf.x; P.pd[1] = (int)pdf.y; P.pd[2] = (int)pdf.z;
+    P.psize = psize;
+    P.max_bs = std::max(mbx, mby);
+    P.min_exit = min_exit;
     const Rot2 binv = rot2_beam_inv(b.azimuth, b.zenith, b.coll);
     for (BeamletPlan& o : P.bl) {
         v3 go = mk3(-0.5f * (psize.x + (float)kWall * rvs.x), -0.5f * (psize.y - o.len + (float)kWall * rvs.y),
@@ -788,6 +911,13 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
         B.tiles[1] = (B.gd[2] + 3) / 4;
         B.tiles[2] = (B.gd[0] + 3) / 4;
         B.tile0 = J.tiles;
+        // footprint prefilter: the beamlet axis runs along the pillar's y axis through its x/z
+        // centre; a sample farther from it than the widest divergent beamlet (at the far end of
+        // the pillar, with a 1.5x allowance for the tilt of the cell boundaries) plus two samples
+        // cannot project into its beamlet's fluence cell
+        B.fc[0] = 0.5f * (P.psize.x / rvs.x + (float)kWall);
+        B.fc[1] = 0.5f * (P.psize.z / rvs.z + (float)kWall);
+        B.fw = 0.75f * P.max_bs * (1.f + P.psize.y / P.min_exit) / std::min(rvs.x, rvs.z) + 2.f;
         const uint64_t nt = (uint64_t)J.tiles + (uint64_t)B.tiles[0] * B.tiles[1] * B.tiles[2];
         if (nt > 0xffffffffULL) return set_error(CCCS_EGEOM, "packed grids too large for one job");
         J.tiles = (uint32_t)nt;
@@ -901,6 +1031,14 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     CCCS_TRY(ctx->bl_pdose.ensure(sizeof(float) * std::max<uint64_t>(1, J.pdose)));
     CCCS_TRY(ctx->lines.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, J.max_lines)));
     CCCS_TRY(ctx->rev.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+    CCCS_TRY(ctx->bl_h.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+    CCCS_TRY(ctx->bl_qcnt.ensure(sizeof(uint32_t) * (size_t)std::max(1, nrays)));
+    CCCS_TRY(ctx->h_small.ensure(sizeof(uint32_t) * kMaxRays + 64));
+    // TERMA queue: footprint samples are a few percent of the stored samples; the counts are
+    // checked after the pass and the pass reruns with larger queues if one overflowed
+    uint64_t subcap = std::max<uint64_t>(uint64_t(1) << 14, J.max_rev / 8 / kSubQueues);
+    CCCS_TRY(ctx->bl_qcnt.ensure(sizeof(uint32_t) * kSubQueues * (size_t)std::max(1, nrays)));
+    CCCS_TRY(ctx->h_small.ensure(sizeof(uint32_t) * kSubQueues * kMaxRays));
     BlArgs a = base_args(ctx, fast);
     a.batches = reinterpret_cast<const BatchDev*>(d);
     a.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l);
@@ -908,33 +1046,65 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     a.rev = ctx->rev.as<float>();
     a.pdose = ctx->bl_pdose.as<float>();
     a.nbatch = nb;
-    hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for(J.pdose, 256)), dim3(256), 0, ctx->stream, a.pdose, J.pdose, 0.f);
-    CCCS_HIP(hipGetLastError());
     const int half = (int)p.ntheta / 2;
-    for (int r = 0; r < nrays; r++) {
-        a.cones = reinterpret_cast<const BlConeDev*>(d + o_c) + (size_t)r * nb;
-        a.nlines = J.ray_lines[r];
-        a.rowf = r % half;
-        a.rowb = (int)p.ntheta - 1 - (r % half);
-        a.counts = nullptr;
-        if (a.nlines == 0) continue;
-        hipLaunchKernelGGL(bl_interval_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a);
+    for (int attempt = 0; attempt < 3; attempt++) {
+        subcap = std::min<uint64_t>(subcap, 0xffffffffULL / kSubQueues);
+        CCCS_TRY(ctx->bl_queue.ensure(sizeof(TermaReq) * subcap * kSubQueues));
+        TermaReq* q = ctx->bl_queue.as<TermaReq>();
+        uint32_t* qcnt = ctx->bl_qcnt.as<uint32_t>();
+        hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for(J.pdose, 256)), dim3(256), 0, ctx->stream, a.pdose, J.pdose, 0.f);
+        hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for((uint64_t)kSubQueues * nrays, 256)), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<float*>(qcnt), (uint64_t)kSubQueues * nrays, 0.f);
         CCCS_HIP(hipGetLastError());
-        CCCS_TRY(launch_line_scan(ctx, a.lines, a.nlines));
-        uint32_t maxn = 1;
-        for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
-        a.counts = prof_counter(ctx, a.nlines);
-        int tok = prof_begin(ctx, K_BL_CONE);
-        hipLaunchKernelGGL(bl_cone_kernel, dim3((unsigned)a.nlines), dim3(64), sizeof(float2) * maxn, ctx->stream, a);
-        prof_end(ctx, tok, 0.0, 0.0, a.nlines);
-        CCCS_HIP(hipGetLastError());
-        a.counts = nullptr;
-        int tk = prof_begin(ctx, K_BL_BEV);
-        hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
-        prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
-        CCCS_HIP(hipGetLastError());
+        for (int r = 0; r < nrays; r++) {
+            a.cones = reinterpret_cast<const BlConeDev*>(d + o_c) + (size_t)r * nb;
+            a.nlines = J.ray_lines[r];
+            a.rowf = r % half;
+            a.rowb = (int)p.ntheta - 1 - (r % half);
+            a.counts = nullptr;
+            if (a.nlines == 0) continue;
+            hipLaunchKernelGGL(bl_interval_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a);
+            CCCS_HIP(hipGetLastError());
+            CCCS_TRY(launch_line_scan(ctx, a.lines, a.nlines));
+            uint32_t maxn = 1;
+            for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
+            const unsigned lw = (unsigned)std::min<uint64_t>(a.nlines, kLineWaves);
+            int tok = prof_begin(ctx, K_BL_RAY);
+            hipLaunchKernelGGL(bl_sample_kernel, dim3(lw), dim3(64), 0, ctx->stream, a, ctx->bl_h.as<float>(), q,
+                               (uint32_t)subcap, qcnt + (size_t)kSubQueues * r);
+            prof_end(ctx, tok, 0.0, 0.0, a.nlines);
+            CCCS_HIP(hipGetLastError());
+            if (!fast) {
+                const unsigned sb = kSubQueues * 64;
+                uint32_t* cnt = prof_counter(ctx, sb);
+                int ts = prof_begin(ctx, K_BL_TERMA);
+                hipLaunchKernelGGL(bl_siddon_kernel, dim3(sb), dim3(256), 0, ctx->stream, a, q, (uint32_t)subcap,
+                                   (const uint32_t*)(qcnt + (size_t)kSubQueues * r), cnt);
+                prof_end(ctx, ts, 0.0, 0.0, 0);
+                CCCS_HIP(hipGetLastError());
+            }
+            a.counts = prof_counter(ctx, lw);
+            const size_t lds = sizeof(double) * (size_t)((maxn + 1) & ~1u) + sizeof(float2) * maxn;
+            int tw = prof_begin(ctx, K_BL_CONE);
+            hipLaunchKernelGGL(bl_walk_kernel, dim3(lw), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>());
+            prof_end(ctx, tw, 0.0, 0.0, a.nlines);
+            CCCS_HIP(hipGetLastError());
+            a.counts = nullptr;
+            int tk = prof_begin(ctx, K_BL_BEV);
+            hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
+            prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
+            CCCS_HIP(hipGetLastError());
+        }
+        if (fast) return CCCS_OK;
+        uint32_t* hc = ctx->h_small.as<uint32_t>();
+        CCCS_HIP(hipMemcpyAsync(hc, qcnt, sizeof(uint32_t) * kSubQueues * nrays, hipMemcpyDeviceToHost, ctx->stream));
+        CCCS_HIP(hipStreamSynchronize(ctx->stream));
+        uint64_t need = 0;
+        for (int r = 0; r < kSubQueues * nrays; r++) need = std::max<uint64_t>(need, hc[r]);
+        if (need <= subcap) return CCCS_OK;
+        subcap = need + need / 4;
     }
-    return CCCS_OK;
+    return set_error(CCCS_ENOMEM, "TERMA queue overflow");
 }
 
 int run_job_unpack(cccs_ctx* ctx, const Job& J) {

@@ -306,6 +306,7 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     c->h_small.release(); c->h_cones.release();
     c->bl_tab.release(); c->bl_pdose.release(); c->bl_tgrid.release(); c->bl_arena.release();
     c->bl_cnt.release(); c->bl_keys.release(); c->bl_vals.release(); c->bl_fmap.release();
+    c->bl_h.release(); c->bl_queue.release(); c->bl_qcnt.release();
     c->h_bl.release(); c->h_bl_out.release();
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (uint32_t* p : c->prof.chunks) (void)hipFree(p);

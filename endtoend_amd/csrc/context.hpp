@@ -105,7 +105,7 @@ struct cccs_ctx {
     cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan;
     cccs::HostBuf h_small, h_cones;
     // beamlet path: job tables, packed grids, unit-TERMA grids, column arena, sparse output
-    cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap;
+    cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt;
     cccs::HostBuf h_bl, h_bl_out;
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
