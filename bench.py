@@ -1,17 +1,19 @@
 #!/usr/bin/env python3
-"""bench.py -- beamlet-dose throughput of the HIP CCCS engine on one MI355X.
+"""bench.py -- beamlet-dose throughput of the HIP CCCS engine on one MI355X, batched vs plain.
 
-Default workload: the plain 20-beam plan (200x200x160 thorax phantom at 2 mm, 20 coplanar
-beams, 40x40 x 5 mm fluence maps of seeded U[0,1] weights masked to the PTV projection,
-64 cones, FP32).  One step = the full-beam dose of all 20 beams through the plain path
-(TERMA -> crop box -> per-cone REV resample + line walk -> PVCS gather), with density,
-kernel tables and fluence maps resident in HBM before the timed region.  beamlet-doses are
-the active beamlets of the beams computed.
+Headline workload (the batched beamlet configuration): 128^3 water phantom at 2.5 mm with an
+ellipsoidal PTV (7.0 x 4.8 x 1.0 cm semi-axes), 50 non-coplanar seed-drawn beams, 40x40 fluence
+maps of 5 mm beamlets open over the PTV projection (13689 beamlets, 90-458 per beam), 64 cones,
+kernel_extent 1 cm, sparse columns thresholded at 1e-4 of each column's max.  One step = every
+beamlet column of all 50 beams through the batched faithful pipeline (per-sample Siddon TERMA, as
+the reference), columns built on the device; density and kernel tables resident in HBM.
 
-Prints ONE JSON line (see the repository's measurement contract).  Besides value/ms_per_step
-it reports per-beam latency (p50/p99), the dominant kernel's roofline position from HIP-event
-timing of every launch plus algorithmic work counters, and the CPU oracle timed on a bounded
-sample of the same beams (also used as the accuracy check of the measured path).
+Beside the headline the JSON line carries: the plain path (the reference dataflow on HIP: one
+batch at a time, per-cone launches over whole REV grids, dense per-beamlet unpack copied to the
+host and sparsified there) on a bounded sample of beams and the batched/plain ratio; the batched
+rate with columns copied to the host (PCIe-inclusive); p50/p99 latency of single-beam beamlet
+requests; the plain full-beam path (200x200x160 thorax, 20 coplanar beams); the dominant
+kernel's roofline position; and the CPU oracle timed on a bounded sample (also the accuracy check).
 """
 from __future__ import annotations
 
@@ -28,18 +30,30 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+THRESH = 1e-4
+
+# kernels priced against the FP32 VALU peak (geometry/walk arithmetic); the rest against HBM
+VALU_KERNELS = ("cone", "terma", "bl_terma", "bl_cone", "bl_raytrace")
 
 
 def _percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
 
-def roofline(prof: dict, steps: int):
+def _pmc_traffic(kernel: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if one exists."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def roofline(prof: dict):
     """Roofline row for the kernel with the largest measured time."""
     name, row = max(prof.items(), key=lambda kv: kv[1]["ms"])
     per_launch_ms = row["ms"] / max(1, row["launches"])
-    # the line walk is FP32-VALU bound; the resample/gather kernels are HBM bound
-    if name in ("cone", "bl_cone", "terma", "bl_terma", "bl_raytrace"):
+    if name in VALU_KERNELS:
         achieved = row["flops"] / (row["ms"] * 1e-3) / 1e12
         out = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                "frac": achieved / FP32_PEAK_TFLOPS}
@@ -53,42 +67,83 @@ def roofline(prof: dict, steps: int):
     return out
 
 
-def _pmc_traffic(kernel: str):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if one exists."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+def _kernels(prof):
+    return {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                "alg_GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
+                "alg_TFLOPs": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)} for k, v in prof.items()}
 
 
-def cpu_baseline(w, beams_idx, d_gpu_of):
-    """Oracle (reference algorithm restated in C++/OpenMP) on a bounded sample of the workload."""
-    from oracle import oracle as orc  # test infrastructure: the checker, never the measured path
-    from tests.dose_metrics import max_err_pct
+def _cores():
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
         pass
-    omp = int(os.environ.get("OMP_NUM_THREADS", cores))
-    nb = 0
+    return min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
+
+
+def cpu_baseline(w, eng, nbeamlets: int):
+    """The oracle (the reference algorithm restated in C++/OpenMP) on a bounded sample: the first
+    `nbeamlets` active beamlets of the first beam, dense columns; compared with the GPU's."""
+    from endtoend_amd import abi
+    from oracle import oracle as orc  # test infrastructure: the checker, never the measured path
+    from tests.dose_metrics import max_err_pct
+    f = w.fmaps[0].copy()
+    act = np.flatnonzero(f.ravel() > 0)
+    keep = np.zeros(f.size, bool)
+    keep[act[:nbeamlets]] = True
+    f.ravel()[~keep] = 0.0
     t0 = time.perf_counter()
-    errs = []
-    for i in beams_idx:
-        d, _ = orc.beam_dose(w.spec, [w.beams[i]], [w.fmaps[i]])
-        nb += int((w.fmaps[i] > 0).sum())
-        errs.append(max_err_pct(d, d_gpu_of(i)))
+    ref, _ = orc.beamlet_dense(w.spec, w.beams[0], f)
     dt = time.perf_counter() - t0
-    return {"value": nb / dt, "unit": "beamlet-doses/s", "cores": min(omp, cores), "kind": "port",
-            "sample": f"{len(beams_idx)} of {len(w.beams)} beams ({nb} beamlet-doses), {dt:.1f} s"}, max(errs)
+    got = eng.beamlet_dense(w.beams[0], f, abi.CCCS_BEAMLET_FAITHFUL)
+    err = max(max_err_pct(r, g) for r, g in zip(ref, got))
+    n = ref.shape[0]
+    return {"value": n / dt, "unit": "beamlet-doses/s", "cores": _cores(), "kind": "port",
+            "sample": f"{n} beamlets of beam 0 of the batched workload, dense columns, {dt:.1f} s"}, err
 
 
-def copy_calibration(w, mib: int, reps: int = 5):
+def plain_beam_path(steps: int = 2):
+    """The plain full-beam path on the 20-beam thorax plan: beamlet-doses/s and per-beam latency."""
+    from endtoend_amd import workloads as W
+    from endtoend_amd.engine import Engine
+    w = W.cfg2()
+    nv = w.spec.density.size
+    fl = np.concatenate([f.ravel() for f in w.fmaps]).astype(np.float32)
+    offs = np.cumsum([0] + [f.size for f in w.fmaps])[:-1]
+    eng = Engine(w.spec)
+    d_fl = eng.dev_alloc(fl.nbytes)
+    eng.dev_upload(d_fl, fl)
+    d_dose = eng.dev_alloc(nv * 4)
+    eng.beam_dose_device(w.beams, d_fl, d_dose)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.dev_zero(d_dose, nv * 4)
+        eng.beam_dose_device(w.beams, d_fl, d_dose)
+    eng.sync()
+    dt = (time.perf_counter() - t0) / steps
+    lat = []
+    for i, b in enumerate(w.beams):
+        eng.sync()
+        ta = time.perf_counter()
+        eng.beam_dose_device([b], d_fl + 4 * int(offs[i]), d_dose)
+        eng.sync()
+        lat.append(1e3 * (time.perf_counter() - ta))
+    eng.dev_free(d_fl)
+    eng.dev_free(d_dose)
+    eng.close()
+    return {"workload": "200x200x160 thorax @ 2 mm, 20 coplanar beams, 40x40 x 5 mm U[0,1] fluence masked "
+                        "to the PTV, full-beam dose", "value": w.active_beamlets / dt, "unit": "beamlet-doses/s",
+            "ms_per_plan": 1e3 * dt, "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)}}
+
+
+def copy_calibration(mib: int, reps: int = 5):
     """Streams mib MiB through the library's copy kernel `reps` times (reads = writes = mib MiB per
     launch), so rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
+    from endtoend_amd import workloads as W
     from endtoend_amd.engine import Engine
-    eng = Engine(w.spec)
+    eng = Engine(W.small().spec)
     n = mib << 20
     a, b = eng.dev_alloc(n), eng.dev_alloc(n)
     eng.dev_zero(a, n)
@@ -104,37 +159,34 @@ def copy_calibration(w, mib: int, reps: int = 5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-beams", type=int, default=2, help="beams in the oracle sample (0 = skip)")
-    ap.add_argument("--latency-reps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-beams", type=int, default=2, help="beams in the plain-path sample")
+    ap.add_argument("--latency-beams", type=int, default=20, help="single-beam requests timed for p50/p99")
+    ap.add_argument("--cpu-beamlets", type=int, default=12, help="beamlets in the oracle sample (0 = skip)")
+    ap.add_argument("--no-extras", action="store_true", help="headline only (profiling runs)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (beams shard across GPUs "
-              f"with one dose reduction; see DESIGN.md)", file=sys.stderr)
+              f"with no exchange in beamlet mode; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_mib > 0:
+        copy_calibration(args.copy_mib)
+        return
 
+    from endtoend_amd import abi
     from endtoend_amd import workloads as W
     from endtoend_amd.engine import Engine
 
-    w = W.cfg2()
-    if args.copy_mib > 0:
-        copy_calibration(w, args.copy_mib)
-        return
-    beams = w.beams
-    nv = w.spec.density.size
-    fl = np.concatenate([f.ravel() for f in w.fmaps]).astype(np.float32)
-    offs = np.cumsum([0] + [f.size for f in w.fmaps])[:-1]
+    w = W.cfg3()
     eng = Engine(w.spec)
-    d_fl = eng.dev_alloc(fl.nbytes)
-    eng.dev_upload(d_fl, fl)
-    d_dose = eng.dev_alloc(nv * 4)
+    nbl = w.active_beamlets
+    FA = abi.CCCS_BEAMLET_FAITHFUL
 
     def step():
-        eng.dev_zero(d_dose, nv * 4)
-        eng.beam_dose_device(beams, d_fl, d_dose)
+        eng.beamlet_dose(w.beams, w.fmaps, threshold=THRESH, mode=FA, keep=None)
 
     for _ in range(args.warmup):
         step()
@@ -151,34 +203,8 @@ def main():
     eng.profile(False)
     dt = t1 - t0
     ms_per_step = 1e3 * dt / args.steps
-    value = args.steps * w.active_beamlets / dt
+    value = args.steps * nbl / dt
 
-    # per-beam latency of the plain path (one beam per call, inputs resident)
-    lat = []
-    for _ in range(args.latency_reps):
-        for i, b in enumerate(beams):
-            eng.sync()
-            ta = time.perf_counter()
-            eng.beam_dose_device([b], d_fl + 4 * int(offs[i]), d_dose)
-            eng.sync()
-            lat.append(1e3 * (time.perf_counter() - ta))
-
-    cpu, err = None, None
-    if args.cpu_beams > 0:
-        idx = list(range(0, len(beams), max(1, len(beams) // args.cpu_beams)))[:args.cpu_beams]
-
-        def gpu_dose(i):
-            eng.dev_zero(d_dose, nv * 4)
-            eng.beam_dose_device([beams[i]], d_fl + 4 * int(offs[i]), d_dose)
-            out = np.zeros(w.spec.density.shape, np.float32)
-            eng.dev_download(d_dose, out)
-            return out
-
-        cpu, err = cpu_baseline(w, idx, gpu_dose)
-
-    kernels = {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                   "alg_GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
-                   "alg_TFLOPs": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)} for k, v in prof.items()}
     out = {
         "metric": "beamlet-doses/sec at <=0.5% Dmax err, batched vs plain; p50/p99 beam latency",
         "value": value,
@@ -192,19 +218,64 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain 20-beam plan: 200x200x160 thorax @ 2 mm, 20 coplanar beams, 40x40 x 5 mm "
-                               "U[0,1] fluence masked to PTV, 64 cones, one beam per launch",
-                   "path": "plain", "beamlet_doses_per_step": w.active_beamlets, "beams_per_step": len(beams),
-                   "seed": w.seed},
-        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99), "per": "beam"},
-        "max_err_pct_dmax": err,
-        "roofline": roofline(prof, args.steps),
-        "kernels": kernels,
-        "cpu_baseline": cpu,
+        "config": {"workload": "batched beamlet: 128^3 water @ 2.5 mm + ellipsoidal PTV (7.0x4.8x1.0 cm), 50 "
+                               "non-coplanar beams, 40x40 x 5 mm fluence open over the PTV projection, 64 cones, "
+                               "kernel_extent 1 cm, sparse columns at 1e-4 of column max, faithful per-sample Siddon "
+                               "TERMA, columns built in HBM",
+                   "path": "batched", "beamlet_doses_per_step": nbl, "beams_per_step": len(w.beams), "seed": w.seed},
+        "roofline": roofline(prof),
+        "kernels": _kernels(prof),
     }
-    eng.dev_free(d_fl)
-    eng.dev_free(d_dose)
-    eng.close()
+    if not args.no_extras:
+        # columns copied to the host through the native counting sink (PCIe-inclusive)
+        t0 = time.perf_counter()
+        tot = eng.beamlet_dose(w.beams, w.fmaps, threshold=THRESH, mode=FA, keep=False)
+        eng.sync()
+        out["pcie_inclusive"] = {"value": nbl / (time.perf_counter() - t0), "unit": "beamlet-doses/s",
+                                 "columns": tot["columns"], "nnz": tot["nnz"]}
+        # plain path (reference dataflow) on a bounded sample of beams
+        pb = list(range(min(args.plain_beams, len(w.beams))))
+        pbl = sum(int((w.fmaps[i] > 0).sum()) for i in pb)
+        eng.beamlet_dose([w.beams[i] for i in pb], [w.fmaps[i] for i in pb], threshold=THRESH,
+                         mode=abi.CCCS_BEAMLET_PLAIN, keep=False)  # warm
+        t0 = time.perf_counter()
+        eng.beamlet_dose([w.beams[i] for i in pb], [w.fmaps[i] for i in pb], threshold=THRESH,
+                         mode=abi.CCCS_BEAMLET_PLAIN, keep=False)
+        eng.sync()
+        plain = pbl / (time.perf_counter() - t0)
+        # batched on the same sample, device-resident like the headline
+        t0 = time.perf_counter()
+        eng.beamlet_dose([w.beams[i] for i in pb], [w.fmaps[i] for i in pb], threshold=THRESH, mode=FA, keep=None)
+        eng.sync()
+        same = pbl / (time.perf_counter() - t0)
+        out["plain"] = {"value": plain, "unit": "beamlet-doses/s", "sample": f"beams {pb} ({pbl} beamlets)",
+                        "batched_same_sample": same, "batched_vs_plain": value / plain}
+        # fast mode (approximate TERMA), one step
+        t0 = time.perf_counter()
+        eng.beamlet_dose(w.beams, w.fmaps, threshold=THRESH, mode=abi.CCCS_BEAMLET_FAST, keep=None)
+        eng.sync()
+        out["fast_mode"] = {"value": nbl / (time.perf_counter() - t0), "unit": "beamlet-doses/s",
+                            "note": "approximate TERMA, outside the 0.5% bar"}
+        # single-beam request latency (one beam per call, columns to the host)
+        lat = []
+        for i in range(min(args.latency_beams, len(w.beams))):
+            eng.sync()
+            ta = time.perf_counter()
+            eng.beamlet_dose([w.beams[i]], [w.fmaps[i]], threshold=THRESH, mode=FA, keep=False)
+            eng.sync()
+            lat.append(1e3 * (time.perf_counter() - ta))
+        out["latency_ms"] = {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99),
+                             "per": "single-beam beamlet request"}
+        cpu, err = (None, None)
+        if args.cpu_beamlets > 0:
+            cpu, err = cpu_baseline(w, eng, args.cpu_beamlets)
+        out["max_err_pct_dmax"] = err
+        out["cpu_baseline"] = cpu
+        eng.close()
+        out["plain_beam_path"] = plain_beam_path()
+    else:
+        eng.close()
+        out["cpu_baseline"] = None
     print(json.dumps(out))
 
 

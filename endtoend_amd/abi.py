@@ -23,6 +23,7 @@ CCCS_EBUSY = -6
 
 CCCS_BEAMLET_FAITHFUL = 0
 CCCS_BEAMLET_FAST = 1
+CCCS_BEAMLET_PLAIN = 2
 
 _fp = C.POINTER(C.c_float)
 
@@ -113,6 +114,10 @@ class KernelStats(C.Structure):
 
 SPARSE_SINK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(SparseColumn))
 
+
+class SinkTotals(C.Structure):
+    _fields_ = [("columns", C.c_uint64), ("nnz", C.c_uint64), ("value_sum", C.c_double)]
+
 # (name, restype, argtypes) of every entry point declared in include/cccs.h
 _P = C.c_void_p
 EXPORTS = [
@@ -128,6 +133,7 @@ EXPORTS = [
     ("cccs_beamlet_dose", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.POINTER(_fp), C.c_float, C.c_int,
                                     SPARSE_SINK, _P]),
     ("cccs_beamlet_dense", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int)]),
+    ("cccs_sink_count", C.c_int, [_P, C.POINTER(SparseColumn)]),
     ("cccs_beam_reconfigure", C.c_int, [C.POINTER(Beam), C.c_int]),
     ("cccs_build_kernel_from_files", C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, _fp, _fp,
                                                _fp, _fp, _fp, C.POINTER(C.c_int)]),

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "context.hpp"
@@ -675,6 +676,101 @@ __global__ void __launch_bounds__(256) bl_compact_kernel(const BeamletDev* __res
 }
 
 // ------------------------------------------------------------------------------------------------
+// plain path: the reference dataflow on HIP, kept as the baseline the batched path is measured
+// against (dosecalc-beamlet/nvbbRayConvolve_host.cu:692-1409): one batch at a time; per cone a
+// raytrace over the whole REV grid, PackRowConvolve with one thread per sample, and the
+// packed-BEV gather; then per beamlet a full calc_bbox unpack, a device-to-host copy of the
+// dense column and a host sparsify.
+
+// cudaBeamletRaytrace (nvbbRayConvolve_device.cu:19-242), one thread per REV sample: density
+// (-1e-12 for walls and outside) and the Siddon TERMA of the sample's beamlet.
+__global__ void __launch_bounds__(256) plain_ray_kernel(BlArgs a, float* __restrict__ dens, float* __restrict__ terma) {
+    const BlConeDev& c = a.cones[0];
+    const BatchDev& B = a.batches[c.batch];
+    const uint64_t n = (uint64_t)c.nx * c.ny * c.nz;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int tX = (int)(i % (uint64_t)c.nx);
+    const uint64_t l = i / (uint64_t)c.nx;
+    const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
+    float d = -1e-12f, t = 0.f;
+    int pidx;
+    v3 inp;
+    if (bl_locate(a, c, B, tX, tY, tZ, pidx, inp)) {
+        const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+        t = point_terma(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, nullptr);
+        d = bl_density(a, B, bl, inp);
+    }
+    dens[i] = d;
+    terma[i] = t;
+}
+
+// PackRowConvolve body for one destination (nvbbRayConvolve_device.cu:284-417): two walks that
+// stop at a wall, past the last radius, or after entering the last radial interval.
+__device__ float plain_walk(const float* __restrict__ d, const float* __restrict__ T, int n, int tx,
+                            const float* __restrict__ R, const float* __restrict__ IW, const float* __restrict__ rowF,
+                            const float* __restrict__ rowB, int nr, float delr) {
+    if (d[tx] < 0.f) return 0.f;
+    float sum = 0.f;
+    for (int dir = -1; dir <= 1; dir += 2) {
+        const float* row = dir < 0 ? rowF : rowB;
+        int k = tx, s = 0;
+        float r = 0.f, last = 0.f;
+        do {
+            if (k < 0 || k >= n || d[k] < 0.f) break;
+            const float h = 0.5f * (delr * d[k]);
+            r += h;
+            if (r > R[nr - 1]) break;
+            while (R[s] < r) s++;
+            const float lo = s == 0 ? 0.f : R[s - 1];
+            const float klo = s == 0 ? 0.f : row[s - 1];
+            const float cum = klo + (r - lo) * ((row[s] - klo) * IW[s]);
+            sum += (cum - last) * T[k];
+            last = cum;
+            if (k != tx) r += h;
+            k += dir;
+        } while (s < nr - 1);
+    }
+    return sum;
+}
+
+// One workgroup per REV line, the line's density and TERMA in LDS.
+__global__ void __launch_bounds__(256) plain_conv_kernel(BlArgs a, const float* __restrict__ dens,
+                                                         const float* __restrict__ terma, float* __restrict__ out) {
+    extern __shared__ float s_pl[];
+    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    const BlConeDev& c = a.cones[0];
+    const int n = c.nx, tid = threadIdx.x, nr = a.nradii;
+    const uint64_t base = (uint64_t)blockIdx.x * n;
+    for (int j = tid; j < n; j += 256) {
+        s_pl[j] = dens[base + j];
+        s_pl[n + j] = terma[base + j];
+    }
+    if (tid < nr) {
+        const float* MR = a.tables + (size_t)a.ntheta * nr + 2 * nr;
+        sR[tid] = MR[tid];
+        sIW[tid] = MR[nr + tid];
+        sF[tid] = a.tables[(size_t)a.rowf * nr + tid];
+        sB[tid] = a.tables[(size_t)a.rowb * nr + tid];
+    }
+    __syncthreads();
+    for (int j = tid; j < n; j += 256)
+        out[base + j] = plain_walk(s_pl, s_pl + n, n, j, sR, sIW, sF, sB, nr, a.delr) / (float)a.nphi;
+}
+
+// Line table of a dense REV grid (every sample stored), so the packed-BEV gather reads it too.
+__global__ void __launch_bounds__(256) plain_meta_kernel(LineMeta* __restrict__ lines, uint64_t nlines, int nx) {
+    const uint64_t l = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (l >= nlines) return;
+    LineMeta m;
+    m.i0 = 0;
+    m.len = nx;
+    m.off = (uint32_t)(l * (uint64_t)nx);
+    m.pad = 0;
+    lines[l] = m;
+}
+
+// ------------------------------------------------------------------------------------------------
 // host planning
 
 namespace {
@@ -1133,6 +1229,171 @@ int run_job_unpack(cccs_ctx* ctx, const Job& J) {
     return CCCS_OK;
 }
 
+// Keeps v > t * max(column) in the reference's emission order (_sparsify, sparse_io.cpp:22-58) for a
+// column stored i outer, k inner over the whole calc_bbox.
+void sparsify_host(const float* col, const cccs_problem& p, float thresh, std::vector<uint64_t>& keys,
+                   std::vector<float>& vals, uint64_t& dropped) {
+    const uint64_t bx = p.bbox_size[0], by = p.bbox_size[1], bz = p.bbox_size[2], n = bx * by * bz;
+    float mx = 0.f;
+    for (uint64_t e = 0; e < n; e++) mx = std::max(mx, col[e]);
+    const float cut = thresh * mx;
+    keys.clear();
+    vals.clear();
+    dropped = 0;
+    uint64_t e = 0;
+    for (uint64_t i = 0; i < bx; i++)
+        for (uint64_t j = 0; j < by; j++)
+            for (uint64_t k = 0; k < bz; k++, e++) {
+                const float v = col[e];
+                if (v <= cut) {
+                    if (v > 0.f) dropped++;
+                    continue;
+                }
+                keys.push_back(((k + p.bbox_start[2]) * p.size[1] + (j + p.bbox_start[1])) * p.size[0] +
+                               (i + p.bbox_start[0]));
+                vals.push_back(v);
+            }
+}
+
+// The plain path for one batch (see plain_ray_kernel): per cone raytrace / convolve / gather
+// launches, then per beamlet a full calc_bbox unpack copied to the host and sparsified there by
+// worker threads while the device unpacks the next group.
+int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const BatchRef& R,
+                    const std::vector<const float*>& d_fmaps, float thresh, cccs_sparse_sink sink, void* user) {
+    const cccs_problem& p = ctx->prob;
+    Job J;
+    CCCS_TRY(build_job(ctx, plans, std::vector<BatchRef>{R}, d_fmaps, std::vector<const float*>(), J));
+    const uint64_t nbb = (uint64_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
+    for (BeamletDev& d : J.beamlets) {
+        for (int k = 0; k < 3; k++) { d.box0[k] = 0; d.boxn[k] = (int)p.bbox_size[k]; }
+        d.arena = 0;
+    }
+    J.arena = nbb;
+    const int nrays = ctx->nrays;
+    const size_t sz_b = sizeof(BatchDev), sz_l = sizeof(BeamletDev) * J.beamlets.size(),
+                 sz_c = sizeof(BlConeDev) * J.cones.size();
+    const size_t o_l = (sz_b + 255) & ~size_t(255), o_c = (o_l + sz_l + 255) & ~size_t(255);
+    CCCS_TRY(ctx->bl_tab.ensure(o_c + sz_c));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_TRY(ctx->h_bl.ensure(o_c + sz_c));
+    uint8_t* h = ctx->h_bl.as<uint8_t>();
+    std::memcpy(h, J.batches.data(), sz_b);
+    std::memcpy(h + o_l, J.beamlets.data(), sz_l);
+    std::memcpy(h + o_c, J.cones.data(), sz_c);
+    uint8_t* d = ctx->bl_tab.as<uint8_t>();
+    CCCS_HIP(hipMemcpyAsync(d, h, o_c + sz_c, hipMemcpyHostToDevice, ctx->stream));
+    uint64_t max_rev = 0, max_lines = 0;
+    for (const BlConeDev& c : J.cones) {
+        max_rev = std::max<uint64_t>(max_rev, (uint64_t)c.nx * c.ny * c.nz);
+        max_lines = std::max<uint64_t>(max_lines, (uint64_t)c.ny * c.nz);
+    }
+    if (max_rev > 0xffffffffULL) return set_error(CCCS_EGEOM, "REV grid exceeds 2^32 samples");
+    CCCS_TRY(ctx->bl_pdose.ensure(sizeof(float) * std::max<uint64_t>(1, J.pdose)));
+    CCCS_TRY(ctx->lines.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, max_lines)));
+    CCCS_TRY(ctx->rev.ensure(sizeof(float) * std::max<uint64_t>(1, max_rev)));
+    CCCS_TRY(ctx->bl_h.ensure(sizeof(float) * 2 * std::max<uint64_t>(1, max_rev)));
+    BlArgs a = base_args(ctx, 0);
+    a.batches = reinterpret_cast<const BatchDev*>(d);
+    a.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l);
+    a.lines = ctx->lines.as<LineMeta>();
+    a.rev = ctx->rev.as<float>();
+    a.pdose = ctx->bl_pdose.as<float>();
+    a.nbatch = 1;
+    float* dd = ctx->bl_h.as<float>();
+    float* dt = dd + max_rev;
+    hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for(J.pdose, 256)), dim3(256), 0, ctx->stream, a.pdose, J.pdose, 0.f);
+    const int half = (int)p.ntheta / 2;
+    for (int r = 0; r < nrays; r++) {
+        const BlConeDev& c = J.cones[r];
+        a.cones = reinterpret_cast<const BlConeDev*>(d + o_c) + r;
+        a.nlines = (uint64_t)c.ny * c.nz;
+        a.rowf = r % half;
+        a.rowb = (int)p.ntheta - 1 - (r % half);
+        const uint64_t nrev = (uint64_t)c.nx * c.ny * c.nz;
+        if (nrev == 0) continue;
+        int t0 = prof_begin(ctx, K_BL_RAY);
+        hipLaunchKernelGGL(plain_ray_kernel, dim3((unsigned)((nrev + 255) / 256)), dim3(256), 0, ctx->stream, a, dd, dt);
+        prof_end(ctx, t0, 8.0 * (double)nrev, 0.0, nrev);
+        int t1 = prof_begin(ctx, K_BL_CONE);
+        hipLaunchKernelGGL(plain_conv_kernel, dim3((unsigned)a.nlines), dim3(256), sizeof(float) * 2 * c.nx, ctx->stream,
+                           a, (const float*)dd, (const float*)dt, a.rev);
+        prof_end(ctx, t1, 12.0 * (double)nrev, 0.0, nrev);
+        hipLaunchKernelGGL(plain_meta_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a.lines,
+                           a.nlines, c.nx);
+        int t2 = prof_begin(ctx, K_BL_BEV);
+        hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
+        prof_end(ctx, t2, 8.0 * (double)J.pdose, 0.0, J.pdose);
+        CCCS_HIP(hipGetLastError());
+    }
+    // per beamlet: dense unpack -> host -> sparsify (groups of G, two pinned slot sets)
+    const int G = 8, nbl = (int)J.beamlets.size();
+    CCCS_TRY(ctx->bl_arena.ensure(sizeof(float) * nbb * G));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_TRY(ctx->h_bl_out.ensure(sizeof(float) * nbb * 2 * G));
+    float* pin = ctx->h_bl_out.as<float>();
+    std::vector<std::vector<uint64_t>> keys(2 * G);
+    std::vector<std::vector<float>> vals(2 * G);
+    std::vector<uint64_t> drop(2 * G);
+    std::vector<std::thread> workers;
+    struct Joiner {  // no early return leaves a worker running
+        std::vector<std::thread>& w;
+        ~Joiner() {
+            for (std::thread& t : w)
+                if (t.joinable()) t.join();
+        }
+    } joiner{workers};
+    int pending0 = -1;  // first beamlet of the group being sparsified
+    auto emit = [&](int g0) -> int {
+        for (std::thread& t : workers) t.join();
+        workers.clear();
+        const int set = (g0 / G) % 2;
+        for (int q = g0; q < std::min(nbl, g0 + G); q++) {
+            const int sl = set * G + (q - g0);
+            cccs_sparse_column col;
+            col.beam_index = plans[R.plan].beam;
+            col.beamlet_index = J.beamlets[q].fidx;
+            col.nnz = keys[sl].size();
+            col.keys = keys[sl].data();
+            col.values = vals[sl].data();
+            col.perc_nonzero = (float)(100.0 * (double)keys[sl].size() / (double)nbb);
+            col.perc_dropped = (float)(100.0 * (double)drop[sl] / (double)nbb);
+            if (sink(user, &col) != 0) return set_error(CCCS_EIO, "sparse sink aborted the call");
+        }
+        return CCCS_OK;
+    };
+    UnpackArgs u;
+    u.batches = reinterpret_cast<const BatchDev*>(d);
+    u.pdose = a.pdose;
+    u.nbeamlets = 1;
+    u.narena = nbb;
+    u.start = ld3(p.start);
+    u.voxel = ld3(p.voxel);
+    u.rvs = a.rvs;
+    for (int k = 0; k < 3; k++) u.bbs[k] = (int)p.bbox_start[k];
+    for (int g0 = 0; g0 < nbl; g0 += G) {
+        const int set = (g0 / G) % 2;
+        for (int q = g0; q < std::min(nbl, g0 + G); q++) {
+            u.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l) + q;
+            u.arena = ctx->bl_arena.as<float>() + nbb * (size_t)(q - g0);
+            int tk = prof_begin(ctx, K_BL_UNPACK);
+            hipLaunchKernelGGL(bl_unpack_kernel, dim3((unsigned)((nbb + 255) / 256)), dim3(256), 0, ctx->stream, u);
+            prof_end(ctx, tk, 8.0 * (double)nbb, 0.0, nbb);
+            CCCS_HIP(hipMemcpyAsync(pin + nbb * (size_t)(set * G + (q - g0)), u.arena, sizeof(float) * nbb,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        }
+        CCCS_HIP(hipStreamSynchronize(ctx->stream));
+        if (pending0 >= 0) CCCS_TRY(emit(pending0));
+        for (int q = g0; q < std::min(nbl, g0 + G); q++) {
+            const int sl = set * G + (q - g0);
+            const float* col = pin + nbb * (size_t)sl;
+            workers.emplace_back([&, col, sl] { sparsify_host(col, p, thresh, keys[sl], vals[sl], drop[sl]); });
+        }
+        pending0 = g0;
+    }
+    if (pending0 >= 0) CCCS_TRY(emit(pending0));
+    return CCCS_OK;
+}
+
 // Splits the beams' batches into jobs whose device footprint stays within the budget.
 size_t job_bytes(const Job& J) {
     return sizeof(float) * (J.pdose + J.max_rev + 4 * J.arena) + sizeof(LineMeta) * J.max_lines;
@@ -1193,6 +1454,17 @@ std::vector<BatchRef> all_batches(const std::vector<BeamPlan>& plans) {
 
 }  // namespace
 
+extern "C" int cccs_sink_count(void* user, const cccs_sparse_column* column) {
+    if (!user || !column) return 1;
+    cccs_sink_totals* t = static_cast<cccs_sink_totals*>(user);
+    t->columns++;
+    t->nnz += column->nnz;
+    double s = 0.0;
+    for (uint64_t i = 0; i < column->nnz; i++) s += column->values[i];
+    t->value_sum += s;
+    return 0;
+}
+
 extern "C" int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int mode, float* out,
                                   int max_beamlets, int* nactive) {
     clear_error();
@@ -1241,9 +1513,11 @@ extern "C" int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const fl
 extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, const float* const* fluence,
                                  float threshold, int mode, cccs_sparse_sink sink, void* user) {
     clear_error();
-    if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || (!fluence && nbeams > 0) || !sink)
+    if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || (!fluence && nbeams > 0))
         return set_error(CCCS_EINVAL, "null argument");
-    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST) return set_error(CCCS_EINVAL, "unknown beamlet mode");
+    if (!sink && mode == CCCS_BEAMLET_PLAIN) return set_error(CCCS_EINVAL, "the plain path needs a sink");
+    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST && mode != CCCS_BEAMLET_PLAIN)
+        return set_error(CCCS_EINVAL, "unknown beamlet mode");
     if (!(threshold >= 0.f)) return set_error(CCCS_EINVAL, "threshold must be >= 0");
     CCCS_HIP(hipSetDevice(ctx->device));
     const cccs_problem& p = ctx->prob;
@@ -1262,6 +1536,10 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     CCCS_TRY(upload_fmaps(ctx, plans, fluence, d_fmaps));
     if (mode == CCCS_BEAMLET_FAST) CCCS_TRY(run_tgrids(ctx, plans, d_tgrids));
     const std::vector<BatchRef> refs = all_batches(plans);
+    if (mode == CCCS_BEAMLET_PLAIN) {
+        for (const BatchRef& R : refs) CCCS_TRY(run_plain_batch(ctx, plans, R, d_fmaps, threshold, sink, user));
+        return CCCS_OK;
+    }
     const size_t nvox = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
     const size_t budget = size_t(24) << 30;  // device bytes per job
     size_t next = 0;
@@ -1311,6 +1589,7 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
                            (int)p.bbox_start[1], (int)p.bbox_start[2], ctx->bl_keys.as<uint64_t>(), ctx->bl_vals.as<float>());
         prof_end(ctx, tok, 8.0 * (double)J.arena + 12.0 * (double)nnz_total, 0.0, J.arena);
         CCCS_HIP(hipGetLastError());
+        if (!sink) continue;  // columns stay on the device (overwritten by the next job)
         keys.resize(nnz_total);
         vals.resize(nnz_total);
         CCCS_HIP(hipStreamSynchronize(ctx->stream));

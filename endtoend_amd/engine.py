@@ -281,25 +281,33 @@ class Engine:
         return out[:n.value]
 
     def beamlet_dose(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray], threshold: float = 1e-4,
-                     mode: int = abi.CCCS_BEAMLET_FAST, keep: bool = True):
-        """Sparse columns of every active beamlet: list of (beam, beamlet, keys, values) or counts."""
-        cols = []
-        stats = {"columns": 0, "nnz": 0}
-
-        def sink(user, colp):
-            c = colp.contents
-            stats["columns"] += 1
-            stats["nnz"] += c.nnz
-            if keep:
-                k = np.ctypeslib.as_array(c.keys, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.uint64)
-                v = np.ctypeslib.as_array(c.values, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.float32)
-                cols.append((c.beam_index, c.beamlet_index, k, v))
-            return 0
-
-        cb = abi.SPARSE_SINK(sink)
+                     mode: int = abi.CCCS_BEAMLET_FAITHFUL, keep: bool = True):
+        """Sparse columns of every active beamlet: a list of (beam, beamlet, keys, values); with
+        keep=False only the totals (the library's native counting sink, no Python per column); with
+        keep=None the columns are built on the device and not copied out."""
         fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]
         ptrs = (abi._fp * len(fm))(*[abi.fptr(f) for f in fm])
         arr = beams_array(beams)
+        if keep is None:  # device-resident columns, nothing copied to the host
+            abi.check(self.lib.cccs_beamlet_dose(self.ctx, arr, len(beams), ptrs, threshold, mode,
+                                                 abi.SPARSE_SINK(), None), "cccs_beamlet_dose")
+            return None
+        if not keep:
+            tot = abi.SinkTotals()
+            sink = abi.SPARSE_SINK(C.cast(self.lib.cccs_sink_count, C.c_void_p).value)
+            abi.check(self.lib.cccs_beamlet_dose(self.ctx, arr, len(beams), ptrs, threshold, mode, sink,
+                                                 C.cast(C.pointer(tot), C.c_void_p)), "cccs_beamlet_dose")
+            return {"columns": int(tot.columns), "nnz": int(tot.nnz), "value_sum": float(tot.value_sum)}
+        cols = []
+
+        def sink_fn(user, colp):
+            c = colp.contents
+            k = np.ctypeslib.as_array(c.keys, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.uint64)
+            v = np.ctypeslib.as_array(c.values, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.float32)
+            cols.append((c.beam_index, c.beamlet_index, k, v))
+            return 0
+
+        cb = abi.SPARSE_SINK(sink_fn)
         abi.check(self.lib.cccs_beamlet_dose(self.ctx, arr, len(beams), ptrs, threshold, mode, cb, None),
                   "cccs_beamlet_dose")
-        return cols if keep else stats
+        return cols

@@ -109,12 +109,18 @@ typedef int (*cccs_sparse_sink)(void* user, const cccs_sparse_column* column);
 
 typedef struct cccs_ctx cccs_ctx;
 
-/* Selects how a beamlet call computes per-beamlet TERMA at REV samples.
- *   CCCS_BEAMLET_FAITHFUL : full Siddon march per REV sample per cone, exactly the reference
- *                           dataflow (dosecalc-beamlet/nvbbRayConvolve_device.cu:170-211).
- *   CCCS_BEAMLET_FAST     : Siddon once per context voxel, shared by every cone.            */
+/* Selects the beamlet pipeline.
+ *   CCCS_BEAMLET_FAITHFUL : batched pipeline; every footprint REV sample of every cone gets its
+ *                           own Siddon TERMA, as the reference computes it
+ *                           (dosecalc-beamlet/nvbbRayConvolve_device.cu:170-211).
+ *   CCCS_BEAMLET_FAST     : batched pipeline; TERMA interpolated from a per-beam unit-TERMA field
+ *                           on the CT grid (approximate: not bound by the 0.5 % Dmax tolerance).
+ *   CCCS_BEAMLET_PLAIN    : the reference dataflow (one batch at a time, per-cone launches over
+ *                           whole REV grids, dense per-beamlet unpack copied to the host and
+ *                           sparsified there); the baseline of the batched pipeline.           */
 #define CCCS_BEAMLET_FAITHFUL 0
 #define CCCS_BEAMLET_FAST     1
+#define CCCS_BEAMLET_PLAIN    2
 
 /* ---- context: replaces initCudaConstandTex / freeCudaTexture --------------------------------
  * Uploads density, kernel table, radii, spectrum and cone angles to the device and keeps them
@@ -154,10 +160,21 @@ int cccs_siddon_paths(cccs_ctx* ctx, const float* source, const float* points, i
 /* ---- beamlet coefficients: replaces radconvolveTexture (dosecalc-beamlet) + SparsifyManager.
  * Every active beamlet (fluence > 0) of every beam yields one sparse column, passed to `sink`
  * in beam order then active-beamlet order.  threshold: keep v > threshold * max(column).
- * mode: CCCS_BEAMLET_FAITHFUL or CCCS_BEAMLET_FAST. */
+ * sink may be NULL (not in the plain mode): the columns are then built on the device and not
+ * copied out -- the device-resident throughput of the pipeline. 
+ * mode: CCCS_BEAMLET_FAITHFUL, CCCS_BEAMLET_FAST or CCCS_BEAMLET_PLAIN. */
 int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
                       const float* const* fluence, float threshold, int mode,
                       cccs_sparse_sink sink, void* user);
+
+/* Counting sink for callers that need totals only (throughput runs, dry runs): pass a
+ * cccs_sink_totals as `user`; no column data is kept.  No counterpart in the reference. */
+typedef struct cccs_sink_totals {
+    uint64_t columns;
+    uint64_t nnz;
+    double   value_sum;
+} cccs_sink_totals;
+int cccs_sink_count(void* user, const cccs_sparse_column* column);
 
 /* Dense variant for parity tests: writes the bbox-sized dense column of each active beamlet of
  * ONE beam into out (nactive x bbox voxels, active-beamlet order). */

@@ -124,6 +124,28 @@ def test_sparse_columns(case, mode):
         assert np.max(np.abs(v - rf[li])) <= tol * float(r.max()) + 1e-12
 
 
+@pytest.mark.parametrize("mode", [abi.CCCS_BEAMLET_FAITHFUL, abi.CCCS_BEAMLET_PLAIN])
+def test_sparse_columns_rebuild_oracle(case, mode):
+    """The sparse columns, scattered back into dense calc_bbox columns, match the oracle's dense
+    columns within the dose tolerance plus the threshold (dropped entries are <= thr * max)."""
+    name, w, f, ref, eng = case
+    thr = 1e-4
+    cols = eng.beamlet_dose([w.beams[0]], [f], threshold=thr, mode=mode)
+    assert [c[1] for c in cols] == np.flatnonzero(f.ravel() > 0).tolist()
+    nx, ny = w.spec.size[0], w.spec.size[1]
+    bx, by, bz = (int(c) for c in w.spec.bbox_start)
+    sx, sy, sz = (int(c) for c in w.spec.bbox_size)
+    errs = []
+    for (bi, bl, k, v), r in zip(cols, ref):
+        kk = k.astype(np.int64)
+        x, y, z = kk % nx, (kk // nx) % ny, kk // (nx * ny)
+        dense = np.zeros((sz, sy, sx), np.float32)
+        dense[z - bz, y - by, x - bx] = v
+        errs.append(max_err_pct(r, dense))
+    print(f"{name}: mode {mode} sparse-rebuilt max err {max(errs):.5f} % Dmax", flush=True)
+    assert max(errs) <= FAITHFUL_TOL_PCT + 100.0 * thr
+
+
 def test_column_linearity(case):
     """Dose is linear in fluence: scaling one beamlet's weight scales only its column."""
     name, w, f, ref, eng = case
