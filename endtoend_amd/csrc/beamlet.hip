@@ -222,15 +222,19 @@ __device__ __forceinline__ float bl_density(const BlArgs& a, const BatchDev& B, 
     return tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, gi.x + 0.5f, gi.y + 0.5f, gi.z + 0.5f);
 }
 
-// Cumulative kernel of one row at radiological distance r, linear between radii (the
-// reference's clamped linear texture lookup, nvbbRayConvolve_device.cu:330-352).  s is the
-// current interval (first radius >= r); r must not decrease between calls.
-__device__ __forceinline__ float kc_at(float r, const float* __restrict__ R, const float* __restrict__ IW,
-                                       const float* __restrict__ row, int nr, int& s) {
-    while (s < nr - 1 && R[s] < r) s++;
-    const float lo = s == 0 ? 0.f : R[s - 1];
-    const float klo = s == 0 ? 0.f : row[s - 1];
-    return klo + (r - lo) * ((row[s] - klo) * IW[s]);
+// Cumulative kernel of one row, linear between radii (the reference's clamped linear texture
+// lookup, nvbbRayConvolve_device.cu:330-352).  Interval s of a row is {lo, K(lo), slope, hi}: the
+// value at r in (lo, hi] is K(lo) + (r - lo) * slope.  s tracks the first radius >= r and r must
+// not decrease between calls.
+__device__ __forceinline__ float kc_at(float r, const float4* __restrict__ tab, int nr, int& s, float4& iv) {
+    while (s < nr - 1 && iv.w < r) iv = tab[++s];
+    return iv.y + (r - iv.x) * iv.z;
+}
+
+// Sums one uint32 per lane over a 64-lane workgroup into part[slot] (no LDS).
+__device__ __forceinline__ void wave_count(uint32_t v, uint32_t* __restrict__ part, size_t slot) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x == 0) part[slot] = v;
 }
 
 // Stage 1 of a cone direction, one wave per REV line: pillar mapping of every stored sample and
@@ -254,13 +258,27 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     const uint64_t l = line - c.line0;
     const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
     const int i0 = meta.i0, m = meta.len;
+    // affine form of the packed coordinate along the line (matrix rotation): used only to skip
+    // samples that are far from every footprint; the exact mapping decides all others
+    const v3 q0 = mk3(fmaf(c.minc.x, a.rrv.x, (float)tZ), c.minc.y * a.rrv.y, fmaf(c.minc.z, a.rrv.z, (float)tY));
+    const v3 p0 = c.kinv_m(q0) + 0.5f;
+    const v3 dl = mk3(c.kinv_m.m[1], c.kinv_m.m[4], c.kinv_m.m[7]);
+    const float pdx = (float)B.pd[0], pdz = (float)B.pd[2], ipdx = 1.f / pdx, ipdz = 1.f / pdz;
+    const float cull = B.fw + 0.05f;
     bool any = false;
     for (int base = 0; base < m; base += 64) {
         const int j = base + tid;
         bool fp = false;
         int pidx;
         v3 inp;
-        if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
+        bool cand = j < m;
+        if (cand) {
+            const float t = (float)(i0 + j);
+            const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z;
+            const float xl = px - floorf(px * ipdx) * pdx, zl = pz - floorf(pz * ipdz) * pdz;
+            cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull;
+        }
+        if (cand && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
             fabsf(inp.z - B.fc[1]) <= B.fw) {
             const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
             fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
@@ -335,33 +353,43 @@ __global__ void __launch_bounds__(256) bl_siddon_kernel(BlArgs a, const TermaReq
 }
 
 // Stage 3, one wave per non-empty REV line: PackRowConvolve (nvbbRayConvolve_device.cu:284-417)
-// for every sample.  Both walks of destination j run inside the pillar segment [slo, shi] holding
-// j, start at j itself (the self sample adds only its first half step) and stop when the
-// radiological distance passes the last radius or enters the last radial interval.  Only samples
-// with TERMA contribute, and the distance from j to sample k is the difference of the line's
-// radiological prefix sums (FP64), so each destination visits just the footprint samples of its
-// segment, in the reference's walk order, instead of every sample.
-__global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __restrict__ H) {
+// for every sample.  Both walks of destination j run inside the pillar segment holding j, start at
+// j itself (the self sample adds only its first half step) and stop when the radiological
+// distance passes the last radius or enters the last radial interval.  Only samples with TERMA
+// contribute, and the distance from j to sample k is the difference of the line's radiological
+// prefix sums (FP64), so each destination visits just the footprint samples of its segment, in the
+// reference's walk order, instead of every sample.
+//
+// Dynamic LDS: for up to maxn samples {half step, TERMA} (8 B), the radiological centre (FP64),
+// the segment bounds and the footprint list (3 x 2 B).
+__global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __restrict__ H, int maxn) {
     extern __shared__ double s_dyn[];
-    __shared__ short s_lo[kMaxLine], s_hi[kMaxLine], s_fp[kMaxLine];
-    __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    __shared__ float4 sTab[2][kMaxRadii];
     const int tid = threadIdx.x;
+    const int nr = a.nradii;
+    if (tid < nr) {
+        const float* K = a.tables;
+        const float* MR = a.tables + (size_t)a.ntheta * nr + 2 * nr;
+        for (int d = 0; d < 2; d++) {
+            const float* row = K + (size_t)(d == 0 ? a.rowf : a.rowb) * nr;
+            const float lo = tid == 0 ? 0.f : MR[tid - 1];
+            const float klo = tid == 0 ? 0.f : row[tid - 1];
+            sTab[d][tid] = make_float4(lo, klo, (row[tid] - klo) * MR[nr + tid], MR[tid]);
+        }
+    }
+    double* sC = s_dyn;
+    float2* sL = reinterpret_cast<float2*>(s_dyn + maxn);
+    short* s_lo = reinterpret_cast<short*>(sL + maxn);
+    short* s_hi = s_lo + maxn;
+    short* s_fp = s_hi + maxn;
+    __syncthreads();
+    const float rmax = sTab[0][nr - 1].w, rlast = sTab[0][nr - 2].w;
+    const float nphi_f = (float)a.nphi;
     uint32_t iters = 0;
     for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
         const LineMeta meta = a.lines[line];
         if (meta.len <= 0) continue;
         const int m = meta.len;
-        double* sC = s_dyn;                                        // radiological centre of each sample
-        float2* sL = reinterpret_cast<float2*>(s_dyn + ((m + 1) & ~1));  // {half step, TERMA}
-        const int nr = a.nradii;
-        if (tid < nr) {
-            const float* K = a.tables;
-            const float* MR = a.tables + (size_t)a.ntheta * nr + 2 * nr;
-            sR[tid] = MR[tid];
-            sIW[tid] = MR[nr + tid];
-            sF[tid] = K[(size_t)a.rowf * nr + tid];
-            sB[tid] = K[(size_t)a.rowb * nr + tid];
-        }
         float* T = a.rev + meta.off;
         const float* Hl = H + meta.off;
         for (int j = tid; j < m; j += 64) sL[j] = make_float2(Hl[j], T[j]);
@@ -372,10 +400,10 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
         double csum = 0.0;
         int last_wall = -1, first_wall = m, nfp = 0;
         for (int k = ca; k < cb; k++) {
-            const float h = sL[k].x;
-            if (h < 0.f) last_wall = k;
-            else csum += 2.0 * (double)h;
-            if (sL[k].y != 0.f) nfp++;
+            const float2 e = sL[k];
+            if (e.x < 0.f) last_wall = k;
+            else csum += 2.0 * (double)e.x;
+            if (e.y != 0.f) nfp++;
         }
         for (int k = cb - 1; k >= ca; k--)
             if (sL[k].x < 0.f) first_wall = k;
@@ -397,56 +425,53 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
         if (tid == 0) left = -1;
         if (tid == 63) right = m;
         for (int k = ca; k < cb; k++) {
-            const float h = sL[k].x;
-            const double hh = h < 0.f ? 0.0 : (double)h;
+            const float2 e = sL[k];
+            const double hh = e.x < 0.f ? 0.0 : (double)e.x;
             sC[k] = cacc + hh;
             cacc += 2.0 * hh;
-            if (h < 0.f) left = k;
+            if (e.x < 0.f) left = k;
             s_lo[k] = (short)(left + 1);
-            if (sL[k].y != 0.f) s_fp[fpos++] = (short)k;
+            if (e.y != 0.f) s_fp[fpos++] = (short)k;
         }
         for (int k = cb - 1; k >= ca; k--) {
             if (sL[k].x < 0.f) right = k;
             s_hi[k] = (short)(right - 1);
         }
         __syncthreads();
-        const float nphi_f = (float)a.nphi;
-        const float rmax = sR[nr - 1], rlast = sR[nr - 2];
         for (int base = 0; base < m; base += 64) {
             const int j = base + tid;
             if (j >= m) break;
             const float2 ej = sL[j];
             float sum = 0.f;
-            if (ej.x >= 0.f) {
+            if (ej.x >= 0.f && nfoot > 0) {
                 const int slo = s_lo[j], shi = s_hi[j];
                 const double cj = sC[j];
-                // first footprint entry >= j
-                int lo = 0, hi = nfoot;
+                int lo = 0, hi = nfoot;  // first footprint entry >= j
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
                     if (s_fp[mid] < j) lo = mid + 1; else hi = mid;
                 }
                 const int p = lo;
-                for (int dir = 0; dir < 2; dir++) {
-                    const float* row = dir == 0 ? sF : sB;
+                const float r0 = ej.x;
+                for (int d = 0; d < 2; d++) {
+                    const float4* tab = sTab[d];
                     int s = 0;
-                    // the self sample
-                    const float r0 = ej.x;
+                    float4 iv = tab[0];
                     if (!(r0 <= rmax)) continue;
-                    const float k0 = kc_at(r0, sR, sIW, row, nr, s);
-                    sum += k0 * ej.y;
+                    sum += kc_at(r0, tab, nr, s, iv) * ej.y;  // the self sample
                     iters++;
                     if (r0 > rlast) continue;
-                    if (dir == 0) {
+                    if (d == 0) {
                         for (int q = p - 1; q >= 0; q--) {
                             const int k = s_fp[q];
                             if (k < slo) break;
+                            const double ck = sC[k];
                             const float rp = k + 1 == j ? r0 : (float)(cj - sC[k + 1]);
                             if (rp > rlast) break;
-                            const float rk = (float)(cj - sC[k]);
+                            const float rk = (float)(cj - ck);
                             if (rk > rmax) break;
-                            const float kp = kc_at(rp, sR, sIW, row, nr, s);
-                            const float kk = kc_at(rk, sR, sIW, row, nr, s);
+                            const float kp = kc_at(rp, tab, nr, s, iv);
+                            const float kk = kc_at(rk, tab, nr, s, iv);
                             sum += (kk - kp) * sL[k].y;
                             iters++;
                         }
@@ -454,12 +479,13 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
                         for (int q = (p < nfoot && s_fp[p] == j) ? p + 1 : p; q < nfoot; q++) {
                             const int k = s_fp[q];
                             if (k > shi) break;
+                            const double ck = sC[k];
                             const float rp = k - 1 == j ? r0 : (float)(sC[k - 1] - cj);
                             if (rp > rlast) break;
-                            const float rk = (float)(sC[k] - cj);
+                            const float rk = (float)(ck - cj);
                             if (rk > rmax) break;
-                            const float kp = kc_at(rp, sR, sIW, row, nr, s);
-                            const float kk = kc_at(rk, sR, sIW, row, nr, s);
+                            const float kp = kc_at(rp, tab, nr, s, iv);
+                            const float kk = kc_at(rk, tab, nr, s, iv);
                             sum += (kk - kp) * sL[k].y;
                             iters++;
                         }
@@ -470,7 +496,7 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
         }
         __syncthreads();  // the next line reuses the LDS arrays
     }
-    if (a.counts) block_count(iters, a.counts, blockIdx.x);
+    if (a.counts) wave_count(iters, a.counts, blockIdx.x);
 }
 
 // PackedREVtoBEVdose (nvbbRayConvolve_device.cu:427-452) for one cone direction over every batch's
@@ -1008,12 +1034,13 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
         B.tiles[2] = (B.gd[0] + 3) / 4;
         B.tile0 = J.tiles;
         // footprint prefilter: the beamlet axis runs along the pillar's y axis through its x/z
-        // centre; a sample farther from it than the widest divergent beamlet (at the far end of
-        // the pillar, with a 1.5x allowance for the tilt of the cell boundaries) plus two samples
+        // centre; a sample farther from it than half the widest divergent beamlet (at the far end
+        // of the pillar, with a 1.2x allowance for the tilt of the cell boundaries) plus one sample
         // cannot project into its beamlet's fluence cell
         B.fc[0] = 0.5f * (P.psize.x / rvs.x + (float)kWall);
         B.fc[1] = 0.5f * (P.psize.z / rvs.z + (float)kWall);
-        B.fw = 0.75f * P.max_bs * (1.f + P.psize.y / P.min_exit) / std::min(rvs.x, rvs.z) + 2.f;
+        B.fw = 0.5f * P.max_bs * 1.2f * (1.f + P.psize.y / P.min_exit) / std::min(rvs.x, rvs.z) + 1.f;
+        if (std::fabs(rvs.x - rvs.y) > 1e-6f * rvs.x) B.fw = 1e30f;  // the bound assumes isotropic spacing
         const uint64_t nt = (uint64_t)J.tiles + (uint64_t)B.tiles[0] * B.tiles[1] * B.tiles[2];
         if (nt > 0xffffffffULL) return set_error(CCCS_EGEOM, "packed grids too large for one job");
         J.tiles = (uint32_t)nt;
@@ -1180,9 +1207,10 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                 CCCS_HIP(hipGetLastError());
             }
             a.counts = prof_counter(ctx, lw);
-            const size_t lds = sizeof(double) * (size_t)((maxn + 1) & ~1u) + sizeof(float2) * maxn;
+            const size_t lds = (sizeof(double) + sizeof(float2) + 3 * sizeof(short)) * (size_t)maxn;
             int tw = prof_begin(ctx, K_BL_CONE);
-            hipLaunchKernelGGL(bl_walk_kernel, dim3(lw), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>());
+            hipLaunchKernelGGL(bl_walk_kernel, dim3(lw), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>(),
+                               (int)maxn);
             prof_end(ctx, tw, 0.0, 0.0, a.nlines);
             CCCS_HIP(hipGetLastError());
             a.counts = nullptr;
