@@ -33,7 +33,7 @@ FP32_PEAK_TFLOPS = 157.3
 THRESH = 1e-4
 
 # kernels priced against the FP32 VALU peak (geometry/walk arithmetic); the rest against HBM
-VALU_KERNELS = ("cone", "terma", "bl_terma", "bl_cone", "bl_raytrace")
+VALU_KERNELS = ("cone", "terma", "bl_tgrid", "bl_sample", "bl_siddon", "bl_walk", "plain_ray", "plain_conv")
 
 
 def _percentile(v, q):
@@ -165,15 +165,20 @@ def main():
     ap.add_argument("--latency-beams", type=int, default=20, help="single-beam requests timed for p50/p99")
     ap.add_argument("--cpu-beamlets", type=int, default=12, help="beamlets in the oracle sample (0 = skip)")
     ap.add_argument("--no-extras", action="store_true", help="headline only (profiling runs)")
+    ap.add_argument("--plain-beam-only", action="store_true",
+                    help="only the plain full-beam path on the thorax plan (profiling runs)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (beams shard across GPUs "
-              f"with no exchange in beamlet mode; see DESIGN.md)", file=sys.stderr)
+              f"with no exchange in beamlet mode)", file=sys.stderr)
         sys.exit(2)
     if args.copy_mib > 0:
         copy_calibration(args.copy_mib)
+        return
+    if args.plain_beam_only:
+        print(json.dumps(plain_beam_path(steps=args.steps)))
         return
 
     from endtoend_amd import abi

@@ -237,6 +237,12 @@ __device__ __forceinline__ void wave_count(uint32_t v, uint32_t* __restrict__ pa
     if (threadIdx.x == 0) part[slot] = v;
 }
 
+// FP32 operations counted per sample by the sample stage: the affine pre-cull, the exact pillar
+// mapping (two Rodrigues rotations, two divisions, three fmods, bounds), the fluence-cell test of
+// a TERMA point (three rotations, projection, two divisions), and the trilinear density with its
+// point (three rotations, 8-tap interpolation).
+constexpr uint32_t kOpsCull = 12, kOpsLocate = 70, kOpsFluence = 150, kOpsDensity = 130;
+
 // Stage 1 of a cone direction, one wave per REV line: pillar mapping of every stored sample and
 // the beamlet-footprint test (fluence cell of the sample == its beamlet).  A line without footprint
 // samples carries no TERMA, so all its cone dose is zero: it is marked empty.  Otherwise the half
@@ -246,6 +252,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
                                                        uint32_t subcap, uint32_t* __restrict__ qcnt) {
     __shared__ uint8_t s_fp[kMaxLine];
     const int tid = threadIdx.x;
+    uint32_t ops = 0;
     // each wave walks a stride of lines; waves append to one of kSubQueues queues (fewer
     // collisions on the counters)
     const uint32_t sq = blockIdx.x % kSubQueues;
@@ -278,10 +285,12 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             const float xl = px - floorf(px * ipdx) * pdx, zl = pz - floorf(pz * ipdz) * pdz;
             cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull;
         }
+        ops += (j < m ? kOpsCull : 0u) + (cand ? kOpsLocate : 0u);
         if (cand && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
             fabsf(inp.z - B.fc[1]) <= B.fw) {
             const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
             fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
+            ops += kOpsFluence;
         }
         if (j < m) s_fp[j] = fp ? 1 : 0;
         any = any || __ballot(fp) != 0ull;
@@ -299,9 +308,11 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         TermaReq r;
         int pidx;
         v3 inp;
+        ops += j < m ? kOpsLocate : 0u;
         if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp)) {
             const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
             h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
+            ops += kOpsDensity;
             if (s_fp[j]) {
                 const v3 gcs = bl_terma_point(a, B, bl, inp);
                 if (a.fast) {
@@ -330,6 +341,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         }
     }
     }
+    if (a.counts) wave_count(ops, a.counts, blockIdx.x);
 }
 
 // Stage 2: the queued samples' TERMA (cudaSiddon_device with the beamlet filter), one sample per
@@ -1125,7 +1137,7 @@ int run_tgrids(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, std::vector<co
         d_tgrids[i] = g;
         const unsigned blocks = (unsigned)((ng + 255) / 256);
         uint32_t* cnt = prof_counter(ctx, blocks);
-        int tok = prof_begin(ctx, K_BL_TERMA);
+        int tok = prof_begin(ctx, K_BL_TGRID);
         hipLaunchKernelGGL(bl_tgrid_kernel, dim3(blocks), dim3(256), 0, ctx->stream, a, make_beam_dev(*plans[i].b), g, cnt);
         prof_end(ctx, tok, 4.0 * (double)ctx->vol.nx * ctx->vol.ny * ctx->vol.nz + 4.0 * (double)ng, 0.0, ng);
         CCCS_HIP(hipGetLastError());
@@ -1192,7 +1204,8 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             uint32_t maxn = 1;
             for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
             const unsigned lw = (unsigned)std::min<uint64_t>(a.nlines, kLineWaves);
-            int tok = prof_begin(ctx, K_BL_RAY);
+            a.counts = prof_counter(ctx, lw);
+            int tok = prof_begin(ctx, K_BL_SAMPLE);
             hipLaunchKernelGGL(bl_sample_kernel, dim3(lw), dim3(64), 0, ctx->stream, a, ctx->bl_h.as<float>(), q,
                                (uint32_t)subcap, qcnt + (size_t)kSubQueues * r);
             prof_end(ctx, tok, 0.0, 0.0, a.nlines);
@@ -1200,7 +1213,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             if (!fast) {
                 const unsigned sb = kSubQueues * 64;
                 uint32_t* cnt = prof_counter(ctx, sb);
-                int ts = prof_begin(ctx, K_BL_TERMA);
+                int ts = prof_begin(ctx, K_BL_SIDDON);
                 hipLaunchKernelGGL(bl_siddon_kernel, dim3(sb), dim3(256), 0, ctx->stream, a, q, (uint32_t)subcap,
                                    (const uint32_t*)(qcnt + (size_t)kSubQueues * r), cnt);
                 prof_end(ctx, ts, 0.0, 0.0, 0);
@@ -1208,13 +1221,13 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             }
             a.counts = prof_counter(ctx, lw);
             const size_t lds = (sizeof(double) + sizeof(float2) + 3 * sizeof(short)) * (size_t)maxn;
-            int tw = prof_begin(ctx, K_BL_CONE);
+            int tw = prof_begin(ctx, K_BL_WALK);
             hipLaunchKernelGGL(bl_walk_kernel, dim3(lw), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>(),
                                (int)maxn);
             prof_end(ctx, tw, 0.0, 0.0, a.nlines);
             CCCS_HIP(hipGetLastError());
             a.counts = nullptr;
-            int tk = prof_begin(ctx, K_BL_BEV);
+            int tk = prof_begin(ctx, K_BL_GATHER);
             hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
             prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
             CCCS_HIP(hipGetLastError());
@@ -1339,16 +1352,16 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
         a.rowb = (int)p.ntheta - 1 - (r % half);
         const uint64_t nrev = (uint64_t)c.nx * c.ny * c.nz;
         if (nrev == 0) continue;
-        int t0 = prof_begin(ctx, K_BL_RAY);
+        int t0 = prof_begin(ctx, K_PL_RAY);
         hipLaunchKernelGGL(plain_ray_kernel, dim3((unsigned)((nrev + 255) / 256)), dim3(256), 0, ctx->stream, a, dd, dt);
         prof_end(ctx, t0, 8.0 * (double)nrev, 0.0, nrev);
-        int t1 = prof_begin(ctx, K_BL_CONE);
+        int t1 = prof_begin(ctx, K_PL_CONV);
         hipLaunchKernelGGL(plain_conv_kernel, dim3((unsigned)a.nlines), dim3(256), sizeof(float) * 2 * c.nx, ctx->stream,
                            a, (const float*)dd, (const float*)dt, a.rev);
         prof_end(ctx, t1, 12.0 * (double)nrev, 0.0, nrev);
         hipLaunchKernelGGL(plain_meta_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a.lines,
                            a.nlines, c.nx);
-        int t2 = prof_begin(ctx, K_BL_BEV);
+        int t2 = prof_begin(ctx, K_BL_GATHER);
         hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
         prof_end(ctx, t2, 8.0 * (double)J.pdose, 0.0, J.pdose);
         CCCS_HIP(hipGetLastError());

@@ -64,8 +64,9 @@ void HostBuf::release() {
     bytes = 0;
 }
 
-const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_terma", "bl_raytrace",
-                                            "bl_cone", "bl_rev2bev", "bl_unpack", "bl_compact"};
+const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_tgrid", "bl_sample", "bl_siddon",
+                                            "bl_walk", "bl_gather", "bl_unpack", "bl_compact", "plain_ray",
+                                            "plain_conv"};
 
 static int prof_new_chunk(Profiler& P) {
     uint32_t* p = nullptr;
@@ -447,10 +448,11 @@ extern "C" int cccs_ctx_profile_read(cccs_ctx* ctx, cccs_kernel_stats* out, int 
         }
     }
     rows[K_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_CONE];
-    rows[K_BL_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_BL_CONE];
+    rows[K_BL_WALK].alg_flops += kFlopsPerWalkIter * (double)cnt[K_BL_WALK];
     rows[K_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_TERMA];
-    rows[K_BL_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_TERMA];
-    rows[K_BL_RAY].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_RAY];
+    rows[K_BL_TGRID].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_TGRID];
+    rows[K_BL_SIDDON].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_SIDDON];
+    rows[K_BL_SAMPLE].alg_flops += (double)cnt[K_BL_SAMPLE];  // counted in FLOPs by the kernel
     int n = 0;
     for (int k = 0; k < K_COUNT && n < max_rows; k++) {
         if (rows[k].launches == 0) continue;

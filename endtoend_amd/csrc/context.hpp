@@ -60,8 +60,8 @@ struct BeamGeom {
 };
 
 // ---- profiling ---------------------------------------------------------------------------------
-enum KernelKind { K_TERMA = 0, K_CROP, K_CONE, K_PVCS, K_BL_TERMA, K_BL_RAY, K_BL_CONE, K_BL_BEV, K_BL_UNPACK,
-                  K_BL_COMPACT, K_COUNT };
+enum KernelKind { K_TERMA = 0, K_CROP, K_CONE, K_PVCS, K_BL_TGRID, K_BL_SAMPLE, K_BL_SIDDON, K_BL_WALK, K_BL_GATHER,
+                  K_BL_UNPACK, K_BL_COMPACT, K_PL_RAY, K_PL_CONV, K_COUNT };
 extern const char* const kKernelNames[K_COUNT];
 
 struct ProfRec {

@@ -974,6 +974,23 @@ int64_t orc_sparsify(const float* array, const uint32_t* size, const uint32_t* c
     return (int64_t)nnz;
 }
 
+int orc_beamlet_anchor(const cccs_problem* p, const cccs_beam* beamp, int idx, float* start3, float* end3,
+                       float* biso3, float* angles2) {
+    if (!problem_ok(p) || !beamp || !start3 || !end3 || !biso3 || !angles2) return CCCS_EINVAL;
+    const cccs_beam& b = *beamp;
+    Prob P(p);
+    f3 s, e, bi;
+    f2 ang;
+    f2 bsz{b.beamlet_size[0], b.beamlet_size[1]};
+    bool ok = beamlet_anchors(s, e, ang, bi, from_f(b.source), from_f(b.isocenter), (unsigned)idx, bsz, b.fmap_size,
+                              P.voxel, P.start, p->bbox_start, p->bbox_size, b.azimuth, b.zenith, b.coll);
+    start3[0] = s.x; start3[1] = s.y; start3[2] = s.z;
+    end3[0] = e.x; end3[1] = e.y; end3[2] = e.z;
+    biso3[0] = bi.x; biso3[1] = bi.y; biso3[2] = bi.z;
+    angles2[0] = ang.x; angles2[1] = ang.y;
+    return ok ? CCCS_OK : CCCS_EGEOM;
+}
+
 int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beamp, const float* fmap, int mode, float* out,
                       int max_beamlets, int* nactive, orc_stats* st) {
     if (!problem_ok(p) || !beamp || !fmap || mode != CCCS_BEAMLET_FAITHFUL) return CCCS_EINVAL;

@@ -89,6 +89,12 @@ int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beam, const float*
 
 /* A15: _sparsify (sparse_io.cpp:22-58) of one dense calc_bbox column. Returns nnz, writes at
  * most cap entries. */
+/* A10 alone: calcBeamletAnchors (dosecalc-beamlet/nvbbRayConvolve_host.cu:180-280) for fluence
+ * index idx: entry/exit of the beamlet axis through the calc_bbox, its isocentre and its
+ * divergence angles {theta, phi}.  Returns CCCS_EGEOM when the axis misses the box. */
+int orc_beamlet_anchor(const cccs_problem* p, const cccs_beam* beam, int idx, float* start3, float* end3,
+                       float* biso3, float* angles2);
+
 int64_t orc_sparsify(const float* dense, const uint32_t* size, const uint32_t* crop_start,
                      const uint32_t* crop_size, float thresh, uint64_t* keys, float* vals,
                      int64_t cap, float* perc_nonzero, float* perc_dropped);

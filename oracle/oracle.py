@@ -60,6 +60,7 @@ def load() -> C.CDLL:
         "orc_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
         "orc_pack_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
         "orc_beamlet_dense": [P, B, _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int), C.POINTER(Stats)],
+        "orc_beamlet_anchor": [P, B, C.c_int, _fp, _fp, _fp, _fp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -195,6 +196,18 @@ def beamlet_dense(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, max_beaml
     _chk(load().orc_beamlet_dense(C.byref(p), C.byref(beam), fp(f), 0, fp(out), cap, C.byref(n), C.byref(st)),
          "beamlet_dense")
     return out[:n.value], st.as_dict()
+
+
+def beamlet_anchor(spec: ProblemSpec, beam: abi.Beam, idx: int):
+    """Entry, exit, isocentre and divergence angles of beamlet `idx`'s axis (None if it misses)."""
+    p = spec.struct()
+    s, e, bi, a = (np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32),
+                   np.zeros(2, np.float32))
+    rc = load().orc_beamlet_anchor(C.byref(p), C.byref(beam), idx, fp(s), fp(e), fp(bi), fp(a))
+    if rc == abi.CCCS_EGEOM:
+        return None
+    _chk(rc, "beamlet_anchor")
+    return s, e, bi, a
 
 
 def sparsify(dense: np.ndarray, size, crop_start, crop_size, thresh: float):
