@@ -134,6 +134,14 @@ EXPORTS = [
                                     SPARSE_SINK, _P]),
     ("cccs_beamlet_dense", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int)]),
     ("cccs_sink_count", C.c_int, [_P, C.POINTER(SparseColumn)]),
+    ("cccs_load_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_int, C.POINTER(C.c_int)]),
+    ("cccs_load_omni_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_char_p, C.c_int,
+                                           C.POINTER(C.c_int)]),
+    ("cccs_load_omni_header", C.c_int, [C.c_char_p, C.POINTER(Problem), C.POINTER(C.c_int), C.c_char_p, C.c_char_p,
+                                        C.POINTER(C.c_int)]),
+    ("cccs_write_omni_header", C.c_int, [C.c_char_p, C.POINTER(Problem), C.c_int, C.c_char_p, C.c_char_p, C.c_int]),
+    ("cccs_read_f32", C.c_int, [C.c_char_p, _fp, C.c_uint64]),
+    ("cccs_write_f32", C.c_int, [C.c_char_p, _fp, C.c_uint64]),
     ("cccs_beam_reconfigure", C.c_int, [C.POINTER(Beam), C.c_int]),
     ("cccs_build_kernel_from_files", C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, _fp, _fp,
                                                _fp, _fp, _fp, C.POINTER(C.c_int)]),

@@ -230,8 +230,9 @@ extern "C" int cccs_beam_reconfigure(cccs_beam* beam, int orientation) {
         beam->azimuth = std::fmod(beam->azimuth, 2 * kPIf);
         beam->zenith = std::fmod(beam->zenith, 2 * kPIf);
         if (!close(beam->zenith, 0.f) && (close(beam->azimuth, 0.f) || close(beam->azimuth, kPIf))) beam->zenith = 0.f;
-        // source = iso + inverse beam rotation of (0, -sad, 0)
-        v3 s = dm_apply_f(dm_beam_inv(beam->azimuth, beam->zenith, 0.f), mk3(0.f, -beam->sad, 0.f)) + iso;
+        // source = iso + inverse beam rotation of (0, -sad, 0), in the reference's float Rodrigues
+        // form (calc_source_from_angles, beam.cpp:92-97) so the source matches it bit for bit
+        v3 s = rot2(rot2_beam_inv(beam->azimuth, beam->zenith, 0.f), mk3(0.f, -beam->sad, 0.f)) + iso;
         v3 d = norm3(iso - s);
         beam->source[0] = s.x; beam->source[1] = s.y; beam->source[2] = s.z;
         beam->direction[0] = d.x; beam->direction[1] = d.y; beam->direction[2] = d.z;

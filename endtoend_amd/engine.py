@@ -311,3 +311,59 @@ class Engine:
         abi.check(self.lib.cccs_beamlet_dose(self.ctx, arr, len(beams), ptrs, threshold, mode, cb, None),
                   "cccs_beamlet_dose")
         return cols
+
+# ---- the reference's file formats (host functions of the library; no device needed) ----------
+
+def load_beam_list(path: str, requested: int = 0) -> list:
+    """Beams of a beam list file (load_beam_list semantics), reconfigured."""
+    lib = abi.load()
+    n = C.c_int(0)
+    cap = 4096
+    while True:
+        arr = (abi.Beam * cap)()
+        rc = lib.cccs_load_beam_list(path.encode(), requested, arr, cap, C.byref(n))
+        if rc == abi.CCCS_EINVAL and n.value > cap:
+            cap = n.value
+            continue
+        abi.check(rc, "cccs_load_beam_list")
+        return [arr[i] for i in range(n.value)]
+
+
+def load_omni_beam_list(path: str, beam_count: int):
+    """(beams, fluence-map file names) of an omni_beam_list.txt."""
+    lib = abi.load()
+    arr = (abi.Beam * max(1, beam_count))()
+    names = C.create_string_buffer(1024 * max(1, beam_count))
+    n = C.c_int(0)
+    abi.check(lib.cccs_load_omni_beam_list(path.encode(), beam_count, arr, names, 1024, C.byref(n)),
+              "cccs_load_omni_beam_list")
+    raw = names.raw
+    fn = [raw[i * 1024:(i + 1) * 1024].split(b"\0", 1)[0].decode() for i in range(n.value)]
+    return [arr[i] for i in range(n.value)], fn
+
+
+def load_omni_header(path: str) -> dict:
+    lib = abi.load()
+    p = abi.Problem()
+    bc, red = C.c_int(0), C.c_int(0)
+    spec, target = C.create_string_buffer(100), C.create_string_buffer(100)
+    abi.check(lib.cccs_load_omni_header(path.encode(), C.byref(p), C.byref(bc), spec, target, C.byref(red)),
+              "cccs_load_omni_header")
+    return {"problem": p, "beam_count": bc.value, "beam_spec": spec.value.decode(),
+            "target_structure": target.value.decode(), "reduce": bool(red.value)}
+
+
+def write_omni_header(path: str, p, beam_count: int, beam_spec: str, target: str, reduce: bool = False) -> None:
+    abi.check(abi.load().cccs_write_omni_header(path.encode(), C.byref(p), beam_count, beam_spec.encode(),
+                                                target.encode(), int(reduce)), "cccs_write_omni_header")
+
+
+def read_f32(path: str, shape) -> np.ndarray:
+    out = np.zeros(shape, np.float32)
+    abi.check(abi.load().cccs_read_f32(path.encode(), abi.fptr(out), out.size), "cccs_read_f32")
+    return out
+
+
+def write_f32(path: str, data: np.ndarray) -> None:
+    d = np.ascontiguousarray(data, np.float32)
+    abi.check(abi.load().cccs_write_f32(path.encode(), abi.fptr(d), d.size), "cccs_write_f32")

@@ -181,6 +181,29 @@ int cccs_sink_count(void* user, const cccs_sparse_column* column);
 int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int mode,
                        float* out, int max_beamlets, int* nactive);
 
+/* ---- the reference's file formats (host only) -----------------------------------------------
+ * Beam list "az zen [coll] [iso: x y z] [sad: s] [dir: x y z]" in degrees/cm, '#' comments,
+ * default SAD 100 cm; each beam is reconfigured (from angles, or from dir: when given).
+ * requested <= 0 reads every line.  fmap_size/beamlet_size are left 0 for the caller.
+ * Replaces load_beam_list (DoseCalcIO/beam.cpp:369-515). */
+int cccs_load_beam_list(const char* path, int requested, cccs_beam* beams, int cap, int* count);
+/* omni_beam_list.txt written by the preprocessor; fmap_names receives beam_count strings of
+ * name_len bytes (may be NULL).  Replaces load_omni_beam_list (DoseCalcIO/beam.cpp:516-576). */
+int cccs_load_omni_beam_list(const char* path, int beam_count, cccs_beam* beams, char* fmap_names,
+                             int name_len, int* count);
+/* omni-header.txt: fills the scalar fields of `p` (tables and pointers untouched) and clamps the
+ * calc_bbox into the volume.  beam_spec/target_structure need 100 bytes each (may be NULL).
+ * Replaces load_omni_header (DoseCalcIO/calc_header.cpp:81-161); the writer restates
+ * write_omni_header (calc_header.cpp:12-80). */
+int cccs_load_omni_header(const char* path, cccs_problem* p, int* beam_count, char* beam_spec,
+                          char* target_structure, int* reduce);
+int cccs_write_omni_header(const char* path, const cccs_problem* p, int beam_count,
+                           const char* beam_spec, const char* target_structure, int reduce);
+/* Raw little-endian float32 files: density.raw, fluence maps (fluence.cpp:19-44, exact size
+ * required), dose output (<out>.raw, DoseCalcIO/binary_io.h:47-86). */
+int cccs_read_f32(const char* path, float* out, uint64_t n);
+int cccs_write_f32(const char* path, const float* data, uint64_t n);
+
 /* ---- host-side helpers restating the reference's input preparation ------------------------
  * Beam angles -> source/direction (BEAM::reconfigure, beam.cpp:300-331). orientation 0 = from
  * angles, 1 = from direction. */
