@@ -59,6 +59,7 @@ struct BeamletDev {
     int box0[3];     // calc_bbox-relative voxel box that can receive its dose
     int boxn[3];
     uint64_t arena;  // first element of its box in the column arena (i outer, k inner)
+    m33 fwd_m;       // PVCS -> beamlet frame (beamlet forward * beam forward), matrix form
 };
 
 struct BatchDev {
@@ -606,45 +607,111 @@ __global__ void __launch_bounds__(256) bl_unpack_kernel(UnpackArgs a) {
     a.arena[e] = v;
 }
 
-// Per-beamlet column statistics (one workgroup per beamlet): max, kept (v > t * max) and dropped
-// (0 < v <= t * max) counts, as _sparsify (DoseCalcIO/sparse_io.cpp:22-58) defines them.
-__global__ void __launch_bounds__(256) bl_colstat_kernel(const BeamletDev* __restrict__ bls, const float* __restrict__ arena,
-                                                         float thresh, uint32_t* __restrict__ stat) {
-    __shared__ float s_max[256];
-    __shared__ uint32_t s_a[256], s_b[256];
-    const BeamletDev& bl = bls[blockIdx.x];
-    const uint64_t n = (uint64_t)bl.boxn[0] * bl.boxn[1] * bl.boxn[2];
-    const float* col = arena + bl.arena;
-    const int t = threadIdx.x;
-    float mx = -FLT_MAX;
-    for (uint64_t i = t; i < n; i += 256) mx = fmaxf(mx, col[i]);
-    s_max[t] = mx;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) s_max[t] = fmaxf(s_max[t], s_max[t + w]);
-        __syncthreads();
+// ---- sparse columns without an arena -------------------------------------------------------------
+// One workgroup per beamlet walks the calc_bbox rows (i, j) of its box in the emission order of
+// _sparsify (i outer, j, k inner).  The pillar's voxels along a row form one interval of k (the
+// pillar is a box in its own frame and the frame is affine in k), found by clipping the row
+// against the pillar's three slabs (matrix form, widened); inside it the exact unpack test and
+// fetch of UnpackBEVDosePillar decide each value.  Nothing is stored between passes: the values
+// are recomputed for the max, the counts and the emission.
+
+struct ColArgs {
+    const BatchDev* batches;
+    const BeamletDev* beamlets;
+    const float* pdose;
+    v3 start, voxel, rvs;
+    int bbs[3];        // calc_bbox start
+    int nx, ny;        // full volume dims (keys)
+    float thresh;
+    uint32_t* stat;    // per beamlet {kept, dropped, max bits, 0}
+    const uint64_t* off;
+    uint64_t* keys;
+    float* vals;
+};
+
+// UnpackBEVDosePillar (nvbbRayConvolve_device.cu:459-521) at one calc_bbox voxel.
+__device__ __forceinline__ float pillar_value(const ColArgs& a, const BeamletDev& bl, const BatchDev& B, int tX, int tY,
+                                              int tZ) {
+    const v3 gc = mk3(((float)tX + (float)a.bbs[0]) * a.voxel.x + a.start.x,
+                      ((float)tY + (float)a.bbs[1]) * a.voxel.y + a.start.y,
+                      ((float)tZ + (float)a.bbs[2]) * a.voxel.z + a.start.z);
+    v3 pidx = rot1(bl.fwd, rot2(B.beam.beam_fwd, gc - bl.pst)) / a.rvs + 0.5f;
+    const float lo_f = (float)(kWall + kBuffer);
+    if (pidx.x < lo_f || pidx.y < lo_f || pidx.z < lo_f || pidx.x >= (float)(B.pd[0] - kBuffer) ||
+        pidx.y >= (float)(B.pd[1] - kBuffer) || pidx.z >= (float)(B.pd[2] - kBuffer))
+        return 0.f;
+    pidx = mk3(pidx.x + (float)(bl.pix * B.pd[0]), pidx.y, pidx.z + (float)(bl.piy * B.pd[2]));
+    const float dc = tex3_border(a.pdose + B.pdose, B.gd[1], B.gd[2], B.gd[0], pidx.y, pidx.z, pidx.x);
+    return dc > 0.f ? dc : 0.f;
+}
+
+// k range [k0, k1) of row (tX, tY) that can lie inside the pillar's unpacked region.
+__device__ __forceinline__ void pillar_row(const ColArgs& a, const BeamletDev& bl, const BatchDev& B, int tX, int tY,
+                                           int& k0, int& k1) {
+    const v3 g0 = mk3(((float)tX + (float)a.bbs[0]) * a.voxel.x + a.start.x,
+                      ((float)tY + (float)a.bbs[1]) * a.voxel.y + a.start.y,
+                      ((float)bl.box0[2] + (float)a.bbs[2]) * a.voxel.z + a.start.z);
+    const v3 q0 = bl.fwd_m(g0 - bl.pst) / a.rvs + 0.5f;
+    const v3 dq = mk3(bl.fwd_m.m[2] * a.voxel.z / a.rvs.x, bl.fwd_m.m[5] * a.voxel.z / a.rvs.y,
+                      bl.fwd_m.m[8] * a.voxel.z / a.rvs.z);
+    float t0 = 0.f, t1 = (float)bl.boxn[2];
+    const float lo_f = (float)(kWall + kBuffer);
+    clip_slab(q0.x, dq.x, lo_f, (float)(B.pd[0] - kBuffer), t0, t1);
+    clip_slab(q0.y, dq.y, lo_f, (float)(B.pd[1] - kBuffer), t0, t1);
+    clip_slab(q0.z, dq.z, lo_f, (float)(B.pd[2] - kBuffer), t0, t1);
+    k0 = 0;
+    k1 = 0;
+    if (t0 <= t1) {
+        k0 = max(0, (int)floorf(t0) - 1);
+        k1 = min(bl.boxn[2], (int)ceilf(t1) + 2);
+        if (k1 < k0) k1 = k0;
     }
-    // the column outside the box is zero, so its max is at least 0
-    const float cmax = fmaxf(s_max[0], 0.f);
-    const float cut = thresh * cmax;
+}
+
+// Per-beamlet column statistics of _sparsify (DoseCalcIO/sparse_io.cpp:22-58): max over the
+// column (zeros outside the pillar count), kept (v > t * max) and dropped (0 < v <= t * max).
+__global__ void __launch_bounds__(256) bl_colscan_kernel(ColArgs a) {
+    __shared__ float s_max[4];
+    __shared__ uint32_t s_k[4], s_d[4];
+    const BeamletDev& bl = a.beamlets[blockIdx.x];
+    const BatchDev& B = a.batches[bl.batch];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int rows = bl.boxn[0] * bl.boxn[1];
+    float mx = 0.f;
+    for (int r = t; r < rows; r += 256) {
+        const int i = r / bl.boxn[1], j = r - i * bl.boxn[1];
+        int k0, k1;
+        pillar_row(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, k0, k1);
+        for (int k = k0; k < k1; k++)
+            mx = fmaxf(mx, pillar_value(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, bl.box0[2] + k));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) s_max[w] = mx;
+    __syncthreads();
+    const float cmax = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+    const float cut = a.thresh * cmax;
     uint32_t kept = 0, dropped = 0;
-    for (uint64_t i = t; i < n; i += 256) {
-        const float v = col[i];
-        if (v > cut) kept++;
-        else if (v > 0.f) dropped++;
+    for (int r = t; r < rows; r += 256) {
+        const int i = r / bl.boxn[1], j = r - i * bl.boxn[1];
+        int k0, k1;
+        pillar_row(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, k0, k1);
+        for (int k = k0; k < k1; k++) {
+            const float v = pillar_value(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, bl.box0[2] + k);
+            if (v > cut) kept++;
+            else if (v > 0.f) dropped++;
+        }
     }
-    s_a[t] = kept;
-    s_b[t] = dropped;
+    for (int o = 32; o > 0; o >>= 1) {
+        kept += __shfl_xor(kept, o, 64);
+        dropped += __shfl_xor(dropped, o, 64);
+    }
+    if (lane == 0) { s_k[w] = kept; s_d[w] = dropped; }
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) { s_a[t] += s_a[t + w]; s_b[t] += s_b[t + w]; }
-        __syncthreads();
-    }
     if (t == 0) {
-        stat[4 * blockIdx.x + 0] = s_a[0];
-        stat[4 * blockIdx.x + 1] = s_b[0];
-        stat[4 * blockIdx.x + 2] = __float_as_uint(cmax);
-        stat[4 * blockIdx.x + 3] = 0;
+        a.stat[4 * blockIdx.x + 0] = s_k[0] + s_k[1] + s_k[2] + s_k[3];
+        a.stat[4 * blockIdx.x + 1] = s_d[0] + s_d[1] + s_d[2] + s_d[3];
+        a.stat[4 * blockIdx.x + 2] = __float_as_uint(cmax);
+        a.stat[4 * blockIdx.x + 3] = 0;
     }
 }
 
@@ -670,46 +737,57 @@ __global__ void __launch_bounds__(1024) bl_offsets_kernel(uint32_t* __restrict__
     if (threadIdx.x == 0) off[n] = carry;
 }
 
-// Writes each beamlet's kept entries (one workgroup per beamlet, order-preserving block scans):
-// key = full-volume linear index, value = dose.
-__global__ void __launch_bounds__(256) bl_compact_kernel(const BeamletDev* __restrict__ bls, const float* __restrict__ arena,
-                                                         float thresh, const uint32_t* __restrict__ stat,
-                                                         const uint64_t* __restrict__ off, int nx, int ny, int bb0,
-                                                         int bb1, int bb2, uint64_t* __restrict__ keys,
-                                                         float* __restrict__ vals) {
-    __shared__ uint32_t s[256];
-    const BeamletDev& bl = bls[blockIdx.x];
-    const uint64_t n = (uint64_t)bl.boxn[0] * bl.boxn[1] * bl.boxn[2];
-    const float* col = arena + bl.arena;
-    const float cut = thresh * __uint_as_float(stat[4 * blockIdx.x + 2]);
-    uint64_t pos = off[blockIdx.x];
-    const int t = threadIdx.x;
-    for (uint64_t base = 0; base < n; base += 256) {
-        const uint64_t i = base + t;
-        const float v = i < n ? col[i] : 0.f;
-        const uint32_t keep = (i < n && v > cut) ? 1u : 0u;
-        s[t] = keep;
-        __syncthreads();
-        for (int o = 1; o < 256; o <<= 1) {
-            const uint32_t add = t >= o ? s[t - o] : 0u;
-            __syncthreads();
-            s[t] += add;
-            __syncthreads();
+// Writes each beamlet's kept entries at its offset (one workgroup per beamlet): chunks of 256 rows
+// in order; each thread counts its row, a workgroup scan places the rows, and each thread writes
+// its row's entries in k order.  key = full-volume linear index, value = dose.
+__global__ void __launch_bounds__(256) bl_emit_kernel(ColArgs a) {
+    __shared__ uint32_t s_w[4];
+    const BeamletDev& bl = a.beamlets[blockIdx.x];
+    const BatchDev& B = a.batches[bl.batch];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int rows = bl.boxn[0] * bl.boxn[1];
+    const float cut = a.thresh * __uint_as_float(a.stat[4 * blockIdx.x + 2]);
+    uint64_t pos = a.off[blockIdx.x];
+    for (int base = 0; base < rows; base += 256) {
+        const int r = base + t;
+        int i = 0, j = 0, k0 = 0, k1 = 0;
+        uint32_t n = 0;
+        if (r < rows) {
+            i = r / bl.boxn[1];
+            j = r - i * bl.boxn[1];
+            pillar_row(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, k0, k1);
+            for (int k = k0; k < k1; k++)
+                n += pillar_value(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, bl.box0[2] + k) > cut ? 1u : 0u;
         }
-        if (keep) {
-            const int k = (int)(i % (uint64_t)bl.boxn[2]);
-            const int j = (int)((i / (uint64_t)bl.boxn[2]) % (uint64_t)bl.boxn[1]);
-            const int ii = (int)(i / ((uint64_t)bl.boxn[2] * bl.boxn[1]));
-            const uint64_t key = ((uint64_t)(bl.box0[2] + k + bb2) * (uint64_t)ny + (uint64_t)(bl.box0[1] + j + bb1)) *
-                                     (uint64_t)nx +
-                                 (uint64_t)(bl.box0[0] + ii + bb0);
-            const uint64_t p = pos + s[t] - 1;
-            keys[p] = key;
-            vals[p] = v;
+        // exclusive scan of n over the workgroup (wave scans + 4 wave totals)
+        uint32_t inc = n;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += u;
         }
-        const uint32_t tot = s[255];
+        if (lane == 63) s_w[w] = inc;
         __syncthreads();
-        pos += tot;
+        uint32_t before = 0, total = 0;
+        for (int q = 0; q < 4; q++) {
+            before += q < w ? s_w[q] : 0u;
+            total += s_w[q];
+        }
+        __syncthreads();
+        uint64_t p = pos + before + inc - n;
+        if (n) {
+            const int X = bl.box0[0] + i, Y = bl.box0[1] + j;
+            for (int k = k0; k < k1; k++) {
+                const float v = pillar_value(a, bl, B, X, Y, bl.box0[2] + k);
+                if (v > cut) {
+                    a.keys[p] = ((uint64_t)(bl.box0[2] + k + a.bbs[2]) * (uint64_t)a.ny + (uint64_t)(Y + a.bbs[1])) *
+                                    (uint64_t)a.nx +
+                                (uint64_t)(X + a.bbs[0]);
+                    a.vals[p] = v;
+                    p++;
+                }
+            }
+        }
+        pos += total;
     }
 }
 
@@ -982,6 +1060,7 @@ struct Job {
     std::vector<BlConeDev> cones;          // [ray][batch]
     std::vector<uint64_t> ray_lines;       // lines per ray
     uint64_t max_lines = 0, max_rev = 0, pdose = 0, arena = 0;
+    uint64_t pvox = 0;                     // pillar voxels of all beamlets (bounds the sparse output)
     uint32_t tiles = 0;
 };
 
@@ -1062,6 +1141,7 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
             d.pst = o.pst;
             d.inv = beamlet_rot(o.theta, o.phi, true);
             d.fwd = beamlet_rot(o.theta, o.phi, false);
+            d.fwd_m = to_f(dm_mul(dm_beamlet_fwd(o.theta, o.phi), dm_beam_fwd(b.azimuth, b.zenith, b.coll)));
             d.fidx = o.fidx;
             d.batch = bi;
             d.pix = q % B.npx;
@@ -1069,6 +1149,7 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
             pillar_box(p, b, o, P.pd, d.box0, d.boxn);
             d.arena = J.arena;
             J.arena += (uint64_t)d.boxn[0] * d.boxn[1] * d.boxn[2];
+            J.pvox += (uint64_t)P.pd[0] * P.pd[1] * P.pd[2];
             J.beamlets.push_back(d);
         }
         const v3 gsize = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]) * rvs;
@@ -1437,7 +1518,9 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
 
 // Splits the beams' batches into jobs whose device footprint stays within the budget.
 size_t job_bytes(const Job& J) {
-    return sizeof(float) * (J.pdose + J.max_rev + 4 * J.arena) + sizeof(LineMeta) * J.max_lines;
+    // packed grids, one cone's REV store and half steps, the line table, and the sparse output
+    // (at most one entry per pillar voxel: 12 B each)
+    return sizeof(float) * (J.pdose + 2 * J.max_rev) + sizeof(LineMeta) * J.max_lines + 12 * J.pvox;
 }
 
 }  // namespace
@@ -1606,15 +1689,29 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         }
         next = k;
         CCCS_TRY(run_job_cones(ctx, J, mode == CCCS_BEAMLET_FAST));
-        CCCS_TRY(run_job_unpack(ctx, J));
         const int nbl = (int)J.beamlets.size();
         CCCS_TRY(ctx->bl_cnt.ensure(sizeof(uint32_t) * 4 * nbl + sizeof(uint64_t) * (nbl + 1) + 256));
         uint32_t* stat = ctx->bl_cnt.as<uint32_t>();
         uint64_t* offs = reinterpret_cast<uint64_t*>(ctx->bl_cnt.as<uint8_t>() + ((sizeof(uint32_t) * 4 * nbl + 255) & ~size_t(255)));
         const size_t sz_b = sizeof(BatchDev) * J.batches.size();
-        const BeamletDev* dbl = reinterpret_cast<const BeamletDev*>(ctx->bl_tab.as<uint8_t>() + ((sz_b + 255) & ~size_t(255)));
-        int tok = prof_begin(ctx, K_BL_COMPACT);
-        hipLaunchKernelGGL(bl_colstat_kernel, dim3(nbl), dim3(256), 0, ctx->stream, dbl, ctx->bl_arena.as<float>(), threshold, stat);
+        ColArgs ca;
+        ca.batches = ctx->bl_tab.as<BatchDev>();
+        ca.beamlets = reinterpret_cast<const BeamletDev*>(ctx->bl_tab.as<uint8_t>() + ((sz_b + 255) & ~size_t(255)));
+        ca.pdose = ctx->bl_pdose.as<float>();
+        ca.start = ld3(p.start);
+        ca.voxel = ld3(p.voxel);
+        ca.rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+        for (int k = 0; k < 3; k++) ca.bbs[k] = (int)p.bbox_start[k];
+        ca.nx = (int)p.size[0];
+        ca.ny = (int)p.size[1];
+        ca.thresh = threshold;
+        ca.stat = stat;
+        ca.off = offs;
+        ca.keys = nullptr;
+        ca.vals = nullptr;
+        int tok = prof_begin(ctx, K_BL_UNPACK);
+        hipLaunchKernelGGL(bl_colscan_kernel, dim3(nbl), dim3(256), 0, ctx->stream, ca);
+        prof_end(ctx, tok, 0.0, 0.0, (uint64_t)nbl);
         hipLaunchKernelGGL(bl_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, stat, nbl, offs);
         CCCS_HIP(hipGetLastError());
         // totals decide the output size
@@ -1625,10 +1722,12 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         std::memcpy(&nnz_total, ctx->h_small.p, sizeof(uint64_t));
         CCCS_TRY(ctx->bl_keys.ensure(sizeof(uint64_t) * std::max<uint64_t>(1, nnz_total)));
         CCCS_TRY(ctx->bl_vals.ensure(sizeof(float) * std::max<uint64_t>(1, nnz_total)));
-        hipLaunchKernelGGL(bl_compact_kernel, dim3(nbl), dim3(256), 0, ctx->stream, dbl, ctx->bl_arena.as<float>(),
-                           threshold, stat, offs, (int)p.size[0], (int)p.size[1], (int)p.bbox_start[0],
-                           (int)p.bbox_start[1], (int)p.bbox_start[2], ctx->bl_keys.as<uint64_t>(), ctx->bl_vals.as<float>());
-        prof_end(ctx, tok, 8.0 * (double)J.arena + 12.0 * (double)nnz_total, 0.0, J.arena);
+        ca.keys = ctx->bl_keys.as<uint64_t>();
+        ca.vals = ctx->bl_vals.as<float>();
+        int tk = prof_begin(ctx, K_BL_COMPACT);
+        hipLaunchKernelGGL(bl_emit_kernel, dim3(nbl), dim3(256), 0, ctx->stream, ca);
+        // algorithmic traffic: the kept entries (8 B key + 4 B value)
+        prof_end(ctx, tk, 12.0 * (double)nnz_total, 0.0, nnz_total);
         CCCS_HIP(hipGetLastError());
         if (!sink) continue;  // columns stay on the device (overwritten by the next job)
         keys.resize(nnz_total);

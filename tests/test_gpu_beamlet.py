@@ -156,3 +156,31 @@ def test_column_linearity(case):
     b = eng.beamlet_dense(w.beams[0], g, abi.CCCS_BEAMLET_FAITHFUL)
     np.testing.assert_allclose(b[0], 3.0 * a[0], rtol=2e-5, atol=1e-6 * float(a[0].max()))
     np.testing.assert_array_equal(b[1:], a[1:])
+
+
+def test_columns_identical_alone_and_in_a_multi_beam_job():
+    """Beams of different geometry and field sizes share launches in one job; every column must be
+    bit-identical to the column of the same beam computed alone (no cross-batch interference)."""
+    ws = [W.small(n=28, voxel=0.5, gantry=g, couch=c, hetero=True, fmap=fm, beamlet=(1.0, 1.0))
+          for g, c, fm in ((30.0, 0.0, (6, 6)), (215.0, -45.0, (8, 8)), (120.0, 20.0, (4, 6)))]
+    spec = ws[0].spec
+    spec.kernel_extent = 1.0
+    beams = [w.beams[0] for w in ws]
+    fmaps = []
+    for i, w in enumerate(ws):
+        f = w.fmaps[0].copy()
+        f.ravel()[::(i + 2)] = 0.0
+        fmaps.append(f)
+    eng = Engine(spec)
+    try:
+        mixed = eng.beamlet_dose(beams, fmaps, threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
+        alone = []
+        for i in range(len(beams)):
+            alone += [(i,) + tuple(c[1:]) for c in eng.beamlet_dose([beams[i]], [fmaps[i]], threshold=1e-4,
+                                                                     mode=abi.CCCS_BEAMLET_FAITHFUL)]
+        assert len(mixed) == len(alone) == sum(int((f > 0).sum()) for f in fmaps)
+        for m, a in zip(mixed, alone):
+            assert m[0] == a[0] and m[1] == a[1]
+            assert np.array_equal(m[2], a[2]) and np.array_equal(m[3].view(np.uint32), a[3].view(np.uint32))
+    finally:
+        eng.close()
