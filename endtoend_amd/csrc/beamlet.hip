@@ -30,6 +30,7 @@ constexpr int kBuffer = 2;      // pillar rim excluded from unpacking (same stru
 constexpr int kMaxBatches = 100;  // batch-count retry limit of the reference (host.cu:692-834)
 constexpr uint32_t kSubQueues = 64;  // TERMA sub-queues per cone direction
 constexpr uint32_t kLineWaves = 8192;  // waves of the per-line kernels (each takes a stride of lines)
+constexpr uint32_t kPillarBits = 4096;  // batches up to this many pillars track footprint pillars per line
 
 // One Rodrigues rotation with host-evaluated sine/cosine.
 struct Rot1 {
@@ -251,8 +252,11 @@ constexpr uint32_t kOpsCull = 12, kOpsLocate = 70, kOpsFluence = 150, kOpsDensit
 // samples are queued for their Siddon TERMA (fast mode evaluates TERMA here instead).
 __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restrict__ H, TermaReq* __restrict__ q,
                                                        uint32_t subcap, uint32_t* __restrict__ qcnt) {
-    __shared__ uint8_t s_fp[kMaxLine];
+    __shared__ uint8_t s_fp[kMaxLine];            // 1: footprint sample
+    __shared__ short s_list[kMaxLine];            // compacted work list of the current pass
+    __shared__ uint32_t s_pil[kPillarBits / 32];  // pillars with footprint samples on this line
     const int tid = threadIdx.x;
+    const uint64_t lt = (1ull << tid) - 1ull;
     uint32_t ops = 0;
     // each wave walks a stride of lines; waves append to one of kSubQueues queues (fewer
     // collisions on the counters)
@@ -267,68 +271,130 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
     const int i0 = meta.i0, m = meta.len;
     // affine form of the packed coordinate along the line (matrix rotation): used only to skip
-    // samples that are far from every footprint; the exact mapping decides all others
+    // samples that are far from every footprint or inside pillars without footprint samples;
+    // the exact mapping decides every other sample
     const v3 q0 = mk3(fmaf(c.minc.x, a.rrv.x, (float)tZ), c.minc.y * a.rrv.y, fmaf(c.minc.z, a.rrv.z, (float)tY));
     const v3 p0 = c.kinv_m(q0) + 0.5f;
     const v3 dl = mk3(c.kinv_m.m[1], c.kinv_m.m[4], c.kinv_m.m[7]);
     const float pdx = (float)B.pd[0], pdz = (float)B.pd[2], ipdx = 1.f / pdx, ipdz = 1.f / pdz;
     const float cull = B.fw + 0.05f;
-    bool any = false;
+    const bool track = B.n <= (int)kPillarBits;
+    for (int k = tid; k < (int)(kPillarBits / 32); k += 64) s_pil[k] = 0u;
+    // pass A.1: affine cull of every sample, candidates compacted into s_list
+    int ncand = 0;
     for (int base = 0; base < m; base += 64) {
         const int j = base + tid;
-        bool fp = false;
-        int pidx;
-        v3 inp;
-        bool cand = j < m;
-        if (cand) {
+        bool cand = false;
+        if (j < m) {
             const float t = (float)(i0 + j);
             const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z;
             const float xl = px - floorf(px * ipdx) * pdx, zl = pz - floorf(pz * ipdz) * pdz;
             cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull;
+            s_fp[j] = 0;
         }
-        ops += (j < m ? kOpsCull : 0u) + (cand ? kOpsLocate : 0u);
-        if (cand && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
-            fabsf(inp.z - B.fc[1]) <= B.fw) {
-            const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
-            fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
-            ops += kOpsFluence;
+        const uint64_t mask = __ballot(cand);
+        if (cand) s_list[ncand + __popcll(mask & lt)] = (short)j;
+        ncand += __popcll(mask);
+    }
+    ops += (uint32_t)m * kOpsCull / 64u + 1u;
+    if (ncand == 0) {
+        if (tid == 0) a.lines[line].len = 0;
+        __syncthreads();
+        continue;
+    }
+    __syncthreads();
+    // pass A.2: exact pillar mapping and fluence-cell test of the candidates, full waves
+    bool any = false;
+    for (int r = 0; r < ncand; r += 64) {
+        bool fp = false;
+        if (r + tid < ncand) {
+            const int j = s_list[r + tid];
+            int pidx;
+            v3 inp;
+            ops += kOpsLocate;
+            if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
+                fabsf(inp.z - B.fc[1]) <= B.fw) {
+                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+                fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
+                ops += kOpsFluence;
+                if (fp) {
+                    s_fp[j] = 1;
+                    if (track) atomicOr(&s_pil[pidx >> 5], 1u << (pidx & 31));
+                }
+            }
         }
-        if (j < m) s_fp[j] = fp ? 1 : 0;
         any = any || __ballot(fp) != 0ull;
     }
     if (!any) {
         if (tid == 0) a.lines[line].len = 0;
+        __syncthreads();
         continue;
     }
+    __syncthreads();
+    // pass B.1: a pillar without footprint samples on this line gets no dose on it, so its samples
+    // act as walls (a pillar crossing is exactly one segment of the walk).  Samples confidently
+    // in such pillars (or outside the grid's pillars) are written as walls here; the others go to
+    // the exact list.
     float* T = a.rev + meta.off;
     float* Hl = H + meta.off;
+    int nex = 0;
     for (int base = 0; base < m; base += 64) {
         const int j = base + tid;
-        float h = -1.f, t = 0.f;
-        bool need = false;
-        TermaReq r;
-        int pidx;
-        v3 inp;
-        ops += j < m ? kOpsLocate : 0u;
-        if (j < m && bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp)) {
-            const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
-            h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
-            ops += kOpsDensity;
-            if (s_fp[j]) {
-                const v3 gcs = bl_terma_point(a, B, bl, inp);
-                if (a.fast) {
-                    const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
-                    const v3 gi = (gcs - a.vd.start) / a.vd.voxel;
-                    t = fl * tgrid_fetch(B.tgrid, a.tg, gi) * a.bhc * inv_square(gcs, B.beam);
-                } else {
-                    need = true;
-                    r.slot = (uint32_t)(meta.off + j);
-                    r.beamlet = (uint32_t)(B.beamlet0 + pidx);
-                    r.x = gcs.x; r.y = gcs.y; r.z = gcs.z;
+        bool exact = false;
+        if (j < m) {
+            exact = true;
+            if (track) {
+                const float t = (float)(i0 + j);
+                const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z;
+                const float fx = floorf(px * ipdx), fz = floorf(pz * ipdz);
+                const float xl = px - fx * pdx, zl = pz - fz * pdz;
+                if (xl > 0.05f && xl < pdx - 0.05f && zl > 0.05f && zl < pdz - 0.05f) {
+                    const int pa = (int)fx + (int)fz * B.npx;
+                    exact = !(fx < 0.f || fz < 0.f || (int)fx >= B.npx || pa >= B.n ||
+                              !((s_pil[pa >> 5] >> (pa & 31)) & 1u));
                 }
             }
+            if (!exact) {
+                Hl[j] = -1.f;
+                T[j] = 0.f;
+            }
         }
-        if (j < m) {
+        const uint64_t mask = __ballot(exact);
+        if (exact) s_list[nex + __popcll(mask & lt)] = (short)j;
+        nex += __popcll(mask);
+    }
+    ops += (uint32_t)m * kOpsCull / 64u + 1u;
+    __syncthreads();
+    // pass B.2: exact mapping, density and TERMA request of the listed samples, full waves
+    for (int r = 0; r < nex; r += 64) {
+        float h = -1.f, t = 0.f;
+        bool need = false;
+        TermaReq rq;
+        int j = 0;
+        if (r + tid < nex) {
+            j = s_list[r + tid];
+            int pidx;
+            v3 inp;
+            ops += kOpsLocate;
+            if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) &&
+                (!track || ((s_pil[pidx >> 5] >> (pidx & 31)) & 1u))) {
+                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+                h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
+                ops += kOpsDensity;
+                if (s_fp[j]) {
+                    const v3 gcs = bl_terma_point(a, B, bl, inp);
+                    if (a.fast) {
+                        const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
+                        const v3 gi = (gcs - a.vd.start) / a.vd.voxel;
+                        t = fl * tgrid_fetch(B.tgrid, a.tg, gi) * a.bhc * inv_square(gcs, B.beam);
+                    } else {
+                        need = true;
+                        rq.slot = (uint32_t)(meta.off + j);
+                        rq.beamlet = (uint32_t)(B.beamlet0 + pidx);
+                        rq.x = gcs.x; rq.y = gcs.y; rq.z = gcs.z;
+                    }
+                }
+            }
             Hl[j] = h;
             T[j] = t;
         }
@@ -337,10 +403,11 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             uint32_t basepos = 0;
             if (tid == 0) basepos = atomicAdd(qcnt + sq, (uint32_t)__popcll(mask));
             basepos = __shfl(basepos, 0, 64);
-            const uint32_t pos = basepos + (uint32_t)__popcll(mask & ((1ull << tid) - 1ull));
-            if (need && pos < subcap) qs[pos] = r;
+            const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt);
+            if (need && pos < subcap) qs[pos] = rq;
         }
     }
+    __syncthreads();  // the next line reuses the LDS arrays
     }
     if (a.counts) wave_count(ops, a.counts, blockIdx.x);
 }
