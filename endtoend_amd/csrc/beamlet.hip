@@ -1583,12 +1583,34 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
     return CCCS_OK;
 }
 
-// Splits the beams' batches into jobs whose device footprint stays within the budget.
-size_t job_bytes(const Job& J) {
-    // packed grids, one cone's REV store and half steps, the line table, and the sparse output
-    // (at most one entry per pillar voxel: 12 B each)
-    return sizeof(float) * (J.pdose + 2 * J.max_rev) + sizeof(LineMeta) * J.max_lines + 12 * J.pvox;
+// One batch's share of a job's device footprint: REV samples and lines per cone direction, and
+// the packed grid plus the sparse-output bound (12 B per pillar voxel) in bytes.
+int batch_footprint(const cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const BatchRef& R,
+                    std::vector<uint64_t>& rev, std::vector<uint64_t>& lines, uint64_t& fixed) {
+    const cccs_problem& p = ctx->prob;
+    const BeamPlan& P = plans[R.plan];
+    const v3 rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
+    uint32_t mr[3] = {p.max_rev_size[0], p.max_rev_size[1], p.max_rev_size[2]};
+    if (mr[0] == 0) mr[0] = mr[1] = mr[2] = 500;
+    const int npx = (int)std::ceil(std::sqrt((float)R.n));
+    const int npy = (int)std::ceil((float)R.n / (float)npx);
+    const uint64_t gd0 = (uint64_t)npx * P.pd[0] + kWall, gd1 = (uint64_t)P.pd[1] + kWall,
+                   gd2 = (uint64_t)npy * P.pd[2] + kWall;
+    const v3 gsize = mk3((float)gd0, (float)gd1, (float)gd2) * rvs;
+    rev.assign(ctx->nrays, 0);
+    lines.assign(ctx->nrays, 0);
+    for (int r = 0; r < ctx->nrays; r++) {
+        const float th = ctx->theta[r] * 3.14159265358979323846f / 180.f;
+        const float ph = ctx->phi[r] * 3.14159265358979323846f / 180.f;
+        Rev rv;
+        if (!find_rev(p, mr, gsize, th, ph, rv)) return set_error(CCCS_EGEOM, "REV grid exceeds max_rev_size");
+        rev[r] = (uint64_t)rv.size[0] * rv.size[1] * rv.size[2];
+        lines[r] = (uint64_t)rv.size[1] * rv.size[2];
+    }
+    fixed = sizeof(float) * gd0 * gd1 * gd2 + 12ull * (uint64_t)R.n * P.pd[0] * P.pd[1] * P.pd[2];
+    return CCCS_OK;
 }
+
 
 }  // namespace
 
@@ -1734,26 +1756,37 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     const size_t nvox = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
     const size_t budget = size_t(24) << 30;  // device bytes per job
     size_t next = 0;
-    std::vector<uint64_t> keys;
-    std::vector<float> vals;
+    // device footprint of each batch, additive per cone direction: a job needs its batches' packed
+    // grids and sparse-output bounds, plus one direction's REV store, half steps and line table
+    // (summed over its batches per direction, maximised over directions)
+    const int nrays = ctx->nrays;
+    std::vector<std::vector<uint64_t>> b_rev(refs.size()), b_lines(refs.size());
+    std::vector<uint64_t> b_fixed(refs.size());
+    for (size_t i = 0; i < refs.size(); i++)
+        CCCS_TRY(batch_footprint(ctx, plans, refs[i], b_rev[i], b_lines[i], b_fixed[i]));
     while (next < refs.size()) {
         // grow the job batch by batch while it fits the budget
-        std::vector<BatchRef> sel{refs[next]};
-        Job J;
-        CCCS_TRY(build_job(ctx, plans, sel, d_fmaps, d_tgrids, J));
-        size_t k = next + 1;
+        std::vector<uint64_t> rev(nrays, 0), lines(nrays, 0);
+        uint64_t fixed = 0;
+        size_t k = next;
         while (k < refs.size()) {
-            std::vector<BatchRef> trial = sel;
-            trial.push_back(refs[k]);
-            Job T;
-            if (build_job(ctx, plans, trial, d_fmaps, d_tgrids, T) != CCCS_OK || job_bytes(T) > budget) {
-                clear_error();
-                break;
+            uint64_t mr = 0, ml = 0;
+            for (int r = 0; r < nrays; r++) {
+                mr = std::max(mr, rev[r] + b_rev[k][r]);
+                ml = std::max(ml, lines[r] + b_lines[k][r]);
             }
-            sel.swap(trial);
-            J = std::move(T);
+            const uint64_t bytes = fixed + b_fixed[k] + sizeof(float) * 2 * mr + sizeof(LineMeta) * ml;
+            if (k > next && bytes > budget) break;
+            for (int r = 0; r < nrays; r++) {
+                rev[r] += b_rev[k][r];
+                lines[r] += b_lines[k][r];
+            }
+            fixed += b_fixed[k];
             k++;
         }
+        std::vector<BatchRef> sel(refs.begin() + (long)next, refs.begin() + (long)k);
+        Job J;
+        CCCS_TRY(build_job(ctx, plans, sel, d_fmaps, d_tgrids, J));
         next = k;
         CCCS_TRY(run_job_cones(ctx, J, mode == CCCS_BEAMLET_FAST));
         const int nbl = (int)J.beamlets.size();
@@ -1797,14 +1830,22 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         prof_end(ctx, tk, 12.0 * (double)nnz_total, 0.0, nnz_total);
         CCCS_HIP(hipGetLastError());
         if (!sink) continue;  // columns stay on the device (overwritten by the next job)
-        keys.resize(nnz_total);
-        vals.resize(nnz_total);
+        // columns to pinned host memory (full PCIe rate), then the sink sees them in place
+        const size_t kbytes = sizeof(uint64_t) * nnz_total, vbytes = sizeof(float) * nnz_total;
+        const size_t sbytes = sizeof(uint32_t) * 4 * (size_t)nbl;
         CCCS_HIP(hipStreamSynchronize(ctx->stream));
-        CCCS_HIP(hipMemcpy(hstat.data(), stat, sizeof(uint32_t) * 4 * nbl, hipMemcpyDeviceToHost));
+        CCCS_TRY(ctx->h_bl_out.ensure(kbytes + vbytes + sbytes + 64));
+        uint8_t* hb = ctx->h_bl_out.as<uint8_t>();
+        uint64_t* hkeys = reinterpret_cast<uint64_t*>(hb);
+        float* hvals = reinterpret_cast<float*>(hb + kbytes);
+        uint32_t* hst = reinterpret_cast<uint32_t*>(hb + kbytes + vbytes);
+        CCCS_HIP(hipMemcpyAsync(hst, stat, sbytes, hipMemcpyDeviceToHost, ctx->stream));
         if (nnz_total) {
-            CCCS_HIP(hipMemcpy(keys.data(), ctx->bl_keys.p, sizeof(uint64_t) * nnz_total, hipMemcpyDeviceToHost));
-            CCCS_HIP(hipMemcpy(vals.data(), ctx->bl_vals.p, sizeof(float) * nnz_total, hipMemcpyDeviceToHost));
+            CCCS_HIP(hipMemcpyAsync(hkeys, ctx->bl_keys.p, kbytes, hipMemcpyDeviceToHost, ctx->stream));
+            CCCS_HIP(hipMemcpyAsync(hvals, ctx->bl_vals.p, vbytes, hipMemcpyDeviceToHost, ctx->stream));
         }
+        CCCS_HIP(hipStreamSynchronize(ctx->stream));
+        std::memcpy(hstat.data(), hst, sbytes);
         uint64_t pos = 0;
         for (int q = 0; q < nbl; q++) {
             const BeamletDev& d = J.beamlets[q];
@@ -1814,8 +1855,8 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
             col.beam_index = P.beam;
             col.beamlet_index = d.fidx;
             col.nnz = hstat[4 * q];
-            col.keys = keys.data() + pos;
-            col.values = vals.data() + pos;
+            col.keys = hkeys + pos;
+            col.values = hvals + pos;
             col.perc_nonzero = (float)(100.0 * (double)hstat[4 * q] / (double)nvox);
             col.perc_dropped = (float)(100.0 * (double)hstat[4 * q + 1] / (double)nvox);
             pos += hstat[4 * q];
