@@ -420,13 +420,15 @@ __global__ void __launch_bounds__(256) bl_siddon_kernel(BlArgs a, const TermaReq
     const uint32_t sq = blockIdx.x % kSubQueues, per = gridDim.x / kSubQueues;
     const uint32_t n = min(qcnt[sq], subcap);
     const TermaReq* qs = q + (size_t)sq * subcap;
+    const v3 ivox = mk3(1.f / a.vd.voxel.x, 1.f / a.vd.voxel.y, 1.f / a.vd.voxel.z);
     uint32_t steps = 0;
     for (uint32_t i = (blockIdx.x / kSubQueues) * 256 + threadIdx.x; i < n; i += per * 256) {
         const TermaReq r = qs[i];
         const BeamletDev& bl = a.beamlets[r.beamlet];
         const BatchDev& B = a.batches[bl.batch];
         int it = 0;
-        a.rev[r.slot] = point_terma(mk3(r.x, r.y, r.z), B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, &it);
+        a.rev[r.slot] = point_terma_rpl(mk3(r.x, r.y, r.z), B.fmap, B.beam, a.spec, a.vd, ivox, a.dens, a.bhc, bl.fidx,
+                                        &it);
         steps += (uint32_t)it;
     }
     if (counts) block_count(steps, counts, blockIdx.x);

@@ -112,6 +112,15 @@ CCCS_HD float tex3_border(const float* __restrict__ v, int nx, int ny, int nz, f
     float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
     int i = (int)fx, j = (int)fy, k = (int)fz;
     float a = xb - fx, b = yb - fy, c = zb - fz;
+    if (i >= 0 && j >= 0 && k >= 0 && i + 1 < nx && j + 1 < ny && k + 1 < nz) {
+        // all eight taps inside: the same interpolation without per-tap bounds tests
+        const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+        const float* p = v + (size_t)i + sy * (size_t)j + sz * (size_t)k;
+        float e0 = p[0] + a * (p[1] - p[0]), e1 = p[sy] + a * (p[sy + 1] - p[sy]);
+        float f0 = p[sz] + a * (p[sz + 1] - p[sz]), f1 = p[sz + sy] + a * (p[sz + sy + 1] - p[sz + sy]);
+        float g0 = e0 + b * (e1 - e0), g1 = f0 + b * (f1 - f0);
+        return g0 + c * (g1 - g0);
+    }
     float c00 = fetch_or0(v, nx, ny, nz, i, j, k), c10 = fetch_or0(v, nx, ny, nz, i + 1, j, k);
     float c01 = fetch_or0(v, nx, ny, nz, i, j + 1, k), c11 = fetch_or0(v, nx, ny, nz, i + 1, j + 1, k);
     float d00 = fetch_or0(v, nx, ny, nz, i, j, k + 1), d10 = fetch_or0(v, nx, ny, nz, i + 1, j, k + 1);
@@ -169,6 +178,53 @@ CCCS_HD float siddon_march(v3 this_voxel, v3 start, v3 end, v3 voxel, v3 src, co
                 float fz = (src.z + am * diff.z - start.z) / voxel.z;
                 rpl += inter * tex3_border(dens, nx, ny, nz, fx, fy, fz);
                 seg(fx, fy, fz, inter);
+            }
+            alpha = nextalpha;
+        }
+    }
+    if (iters) *iters = iter;
+    return rpl;
+}
+
+// Radiological path length of the same segment structure as siddon_march (identical plane
+// sequence and segment lengths), with the midpoint fetch coordinate formed by multiplying with the
+// reciprocal voxel size instead of dividing: the fetch coordinates may differ in the last bit, so
+// rpl agrees with siddon_march to FP32 rounding, not bit for bit.  Used for beamlet TERMA.
+CCCS_HD float siddon_rpl(v3 this_voxel, v3 start, v3 end, v3 voxel, v3 ivoxel, v3 src, const float* __restrict__ dens,
+                         int nx, int ny, int nz, int* iters) {
+    v3 diff = this_voxel - src;
+    v3 startdiff = start - src;
+    v3 enddiff = end - src;
+    float ax0 = ((startdiff.x) - 0.5f * voxel.x) / (diff.x);
+    float ay0 = ((startdiff.y) - 0.5f * voxel.y) / (diff.y);
+    float az0 = ((startdiff.z) - 0.5f * voxel.z) / (diff.z);
+    float ax1 = ((enddiff.x) + 0.5f * voxel.x) / (diff.x);
+    float ay1 = ((enddiff.y) + 0.5f * voxel.y) / (diff.y);
+    float az1 = ((enddiff.z) + 0.5f * voxel.z) / (diff.z);
+    float a_min = fmaxf(0.f, fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fminf(az0, az1)));
+    float a_max = fminf(1.f, fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1)));
+    float rpl = 0.0f;
+    int iter = 0;
+    if (!(a_min >= a_max)) {
+        float d12 = len3(diff);
+        float step_x = fabsf(voxel.x / diff.x);
+        float step_y = fabsf(voxel.y / diff.y);
+        float step_z = fabsf(voxel.z / diff.z);
+        const v3 o = (src - start) * ivoxel, dd = diff * ivoxel;
+        float alpha = a_min, alpha_x = a_min, alpha_y = a_min, alpha_z = a_min;
+        float nextalpha = 0.f;
+        while (alpha < a_max && ++iter < 5000) {
+            bool vx = (alpha_x >= 0.0f && alpha_x < 1.0f);
+            bool vy = (alpha_y >= 0.0f && alpha_y < 1.0f);
+            bool vz = (alpha_z >= 0.0f && alpha_z < 1.0f);
+            if (!(vx || vy || vz)) break;
+            if (vx && (!vy || alpha_x <= alpha_y) && (!vz || alpha_x <= alpha_z)) { nextalpha = alpha_x; alpha_x += step_x; }
+            if (vy && (!vx || alpha_y <= alpha_x) && (!vz || alpha_y <= alpha_z)) { nextalpha = alpha_y; alpha_y += step_y; }
+            if (vz && (!vx || alpha_z <= alpha_x) && (!vy || alpha_z <= alpha_y)) { nextalpha = alpha_z; alpha_z += step_z; }
+            float inter = fabsf(d12 * (nextalpha - alpha));
+            if (inter >= 0.001f) {
+                float am = (nextalpha + alpha) * 0.5f;
+                rpl += inter * tex3_border(dens, nx, ny, nz, o.x + am * dd.x, o.y + am * dd.y, o.z + am * dd.z);
             }
             alpha = nextalpha;
         }
@@ -241,6 +297,22 @@ CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b
     if (fluence <= 0.f) return 0.f;
     float rpl = siddon_march(p, vd.start, vd.end, vd.voxel, b.src, dens, vd.nx, vd.ny, vd.nz,
                              [](float, float, float, float) {}, iters);
+    float tsum = 0.f;
+    for (int e = 0; e < sp.n; e++) {
+        float this_fluence = fluence * sp.flu[e];
+        tsum += this_fluence * sp.en[e] * sp.mu[e] * expf(-1.f * sp.mu[e] * rpl);
+    }
+    return tsum * bhc * inv_square(p, b);
+}
+
+// point_terma with the rpl of siddon_rpl (beamlet TERMA).
+CCCS_HD float point_terma_rpl(v3 p, const float* __restrict__ fmap, const BeamDev& b, const SpecDev& sp,
+                              const VolDesc& vd, v3 ivoxel, const float* __restrict__ dens, float bhc, int beamletnum,
+                              int* iters) {
+    if (iters) *iters = 0;
+    const float fluence = point_fluence(p, fmap, b, beamletnum);
+    if (fluence <= 0.f) return 0.f;
+    float rpl = siddon_rpl(p, vd.start, vd.end, vd.voxel, ivoxel, b.src, dens, vd.nx, vd.ny, vd.nz, iters);
     float tsum = 0.f;
     for (int e = 0; e < sp.n; e++) {
         float this_fluence = fluence * sp.flu[e];
