@@ -267,10 +267,9 @@ __global__ void __launch_bounds__(256) line_interval_kernel(ConeArgs a) {
     int i0, i1;
     line_interval(a, c, (int)(l % (uint64_t)c.ny), (int)(l / (uint64_t)c.ny), i0, i1);
     LineMeta m;
-    m.i0 = i0;
-    m.len = i1 - i0;
+    m.i0 = (int16_t)i0;
+    m.len = (int16_t)(i1 - i0);
     m.off = 0;
-    m.pad = 0;
     a.lines[line] = m;
 }
 

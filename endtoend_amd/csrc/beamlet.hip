@@ -91,6 +91,7 @@ struct BlConeDev {
     Rot2 kinv;       // REV -> packed BEV (invRotateKernel)
     Rot2 kfwd;       // packed BEV -> REV (rotateKernel)
     m33 kinv_m;      // matrix form of kinv, for line intervals only
+    m33 kfwd_m;      // matrix form of kfwd, for the packed-BEV gather
     v3 minc;         // REV grid origin [cm]
     int nx, ny, nz;  // line length, lines along y, lines along z
     int batch;
@@ -155,11 +156,11 @@ __global__ void __launch_bounds__(256) bl_interval_kernel(BlArgs a) {
     m.i0 = 0;
     m.len = 0;
     m.off = 0;
-    m.pad = 0;
     if (t0 <= t1) {
-        m.i0 = max(0, (int)floorf(t0) - 2);
+        const int i0 = max(0, (int)floorf(t0) - 2);
         const int i1 = min(c.nx, (int)ceilf(t1) + 3);
-        m.len = max(0, i1 - m.i0);
+        m.i0 = (int16_t)i0;
+        m.len = (int16_t)max(0, i1 - i0);
     }
     a.lines[line] = m;
 }
@@ -600,7 +601,33 @@ __global__ void __launch_bounds__(256) bl_gather_kernel(BlArgs a, uint32_t ntile
     const int tid = threadIdx.x;
     const int pY = ty * 16 + (tid & 15), pZ = tz * 4 + ((tid >> 4) & 3), pX = tx * 4 + (tid >> 6);
     if (pX >= B.gd[0] || pY >= B.gd[1] || pZ >= B.gd[2]) return;
-    const v3 rk = rot2(c.kfwd, mk3((float)pX, (float)pY, (float)pZ));
+    // Only REV lines with footprint samples in a pillar carry dose in it, and those lines pass
+    // within sqrt(2)*fw of that pillar's beamlet axis (footprint samples satisfy |x-fc|,|z-fc| <=
+    // fw).  The four lines this voxel reads lie within sqrt(2) of the line through it, so if that
+    // line passes farther than sqrt(2)*(fw + 1) + 0.5 from the axes of its pillar and of the
+    // neighbouring pillars it touches, the fetch is zero and is skipped.
+    if (B.fw < 1e29f) {
+        const float wx = c.kinv_m.m[1], wz = c.kinv_m.m[7];
+        const float wn = sqrtf(wx * wx + wz * wz);
+        const float bound = 1.41421356f * (B.fw + 1.f) + 0.5f;
+        const int pix = pX / B.pd[0], piy = pZ / B.pd[2];
+        const int rx = pX - pix * B.pd[0], rz = pZ - piy * B.pd[2];
+        const int sx = rx < 2 ? -1 : (rx >= B.pd[0] - 2 ? 1 : 0);
+        const int sz = rz < 2 ? -1 : (rz >= B.pd[2] - 2 ? 1 : 0);
+        bool hit = false;
+        for (int ox = 0; ox <= (sx != 0 ? 1 : 0) && !hit; ox++)
+            for (int oz = 0; oz <= (sz != 0 ? 1 : 0) && !hit; oz++) {
+                const int qx = pix + ox * sx, qz = piy + oz * sz;
+                if (qx < 0 || qz < 0 || qx >= B.npx || qz >= B.npy || qx + qz * B.npx >= B.n) continue;
+                const float dx = (float)(pX - qx * B.pd[0]) - B.fc[0], dz = (float)(pZ - qz * B.pd[2]) - B.fc[1];
+                const float dist = wn > 1e-6f ? fabsf(dx * wz - dz * wx) / wn : sqrtf(dx * dx + dz * dz);
+                hit = dist <= bound;
+            }
+        if (!hit) return;
+    }
+    // continuous resampling: the matrix form of rotateKernel (the Rodrigues form differs only in
+    // the last bits of the fetch coordinate)
+    const v3 rk = c.kfwd_m(mk3((float)pX, (float)pY, (float)pZ));
     const v3 ri = mk3(fmaf(-c.minc.x, a.rrv.x, rk.x), fmaf(-c.minc.y, a.rrv.y, rk.y), fmaf(-c.minc.z, a.rrv.z, rk.z));
     const float dc = rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
     if (dc > 0.f) a.pdose[B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX)] += dc;
@@ -949,9 +976,8 @@ __global__ void __launch_bounds__(256) plain_meta_kernel(LineMeta* __restrict__ 
     if (l >= nlines) return;
     LineMeta m;
     m.i0 = 0;
-    m.len = nx;
+    m.len = (int16_t)nx;
     m.off = (uint32_t)(l * (uint64_t)nx);
-    m.pad = 0;
     lines[l] = m;
 }
 
@@ -1232,6 +1258,7 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
             c.kinv = rot2_kernel_inv(th, ph);
             c.kfwd = rot2_kernel_fwd(th, ph);
             c.kinv_m = to_f(dm_kernel_inv(th, ph));
+            c.kfwd_m = to_f(dm_kernel_fwd(th, ph));
             c.minc = rv.minc;
             c.nx = (int)rv.size[0];
             c.ny = (int)rv.size[1];

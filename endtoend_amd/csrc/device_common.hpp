@@ -37,11 +37,11 @@ __device__ __forceinline__ void block_count(uint32_t v, uint32_t* __restrict__ p
 // One REV line of a cone grid in the compact store: samples [i0, i0 + len) of the line are kept at
 // rev[off ...]; every other sample of the line is zero.
 struct LineMeta {
-    int i0;
-    int len;
-    uint32_t off;
-    int pad;
+    uint32_t off;  // first stored sample in the store
+    int16_t i0;    // first stored sample index on the line (lines are <= 1024 samples)
+    int16_t len;   // stored samples
 };
+static_assert(sizeof(LineMeta) == 8, "line table entries are 8 bytes");
 
 // Trilinear fetch from a compactly stored REV grid (nx samples per line, ny x nz lines) at texture
 // coordinates (u, v, w) = (sample, line y, line z), texel centres at +0.5, border 0: the same
