@@ -115,6 +115,15 @@ class KernelStats(C.Structure):
 SPARSE_SINK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(SparseColumn))
 
 
+class ServerInfo(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("batches", C.c_uint64), ("mean_batch", C.c_double), ("queued", C.c_uint64)]
+
+
+CCCS_REQ_BEAM_DOSE = 0
+CCCS_REQ_BEAMLET = 1
+
+
 class SinkTotals(C.Structure):
     _fields_ = [("columns", C.c_uint64), ("nnz", C.c_uint64), ("value_sum", C.c_double)]
 
@@ -134,6 +143,12 @@ EXPORTS = [
                                     SPARSE_SINK, _P]),
     ("cccs_beamlet_dense", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int)]),
     ("cccs_sink_count", C.c_int, [_P, C.POINTER(SparseColumn)]),
+    ("cccs_beamlet_footprint", C.c_int, [_P, C.POINTER(Beam), _fp, C.POINTER(C.c_uint64)]),
+    ("cccs_server_create", C.c_int, [C.POINTER(Problem), C.c_int, C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    ("cccs_server_submit", C.c_int, [_P, C.c_int, C.POINTER(Beam), _fp, C.c_float, C.c_int, C.POINTER(C.c_uint64)]),
+    ("cccs_server_wait", C.c_int, [_P, C.c_uint64, _fp, SPARSE_SINK, _P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    ("cccs_server_stats", C.c_int, [_P, C.POINTER(ServerInfo)]),
+    ("cccs_server_destroy", C.c_int, [_P]),
     ("cccs_load_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_int, C.POINTER(C.c_int)]),
     ("cccs_load_omni_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_char_p, C.c_int,
                                            C.POINTER(C.c_int)]),

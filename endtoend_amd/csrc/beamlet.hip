@@ -1696,6 +1696,28 @@ std::vector<BatchRef> all_batches(const std::vector<BeamPlan>& plans) {
 
 }  // namespace
 
+extern "C" int cccs_beamlet_footprint(const cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, uint64_t* bytes) {
+    clear_error();
+    if (!ctx || !beam || !fluence || !bytes) return set_error(CCCS_EINVAL, "null argument");
+    *bytes = 0;
+    CCCS_TRY(check_beam(ctx, *beam));
+    std::vector<BeamPlan> plans(1);
+    CCCS_TRY(plan_beam(ctx, *beam, fluence, 0, plans[0]));
+    if (plans[0].bl.empty()) return CCCS_OK;
+    const std::vector<BatchRef> refs = all_batches(plans);
+    std::vector<uint64_t> rev(ctx->nrays, 0), lines(ctx->nrays, 0), r1, l1;
+    uint64_t fixed = 0, f1 = 0;
+    for (const BatchRef& R : refs) {
+        CCCS_TRY(batch_footprint(ctx, plans, R, r1, l1, f1));
+        for (int r = 0; r < ctx->nrays; r++) { rev[r] += r1[r]; lines[r] += l1[r]; }
+        fixed += f1;
+    }
+    uint64_t mr = 0, ml = 0;
+    for (int r = 0; r < ctx->nrays; r++) { mr = std::max(mr, rev[r]); ml = std::max(ml, lines[r]); }
+    *bytes = fixed + sizeof(float) * 2 * mr + sizeof(LineMeta) * ml;
+    return CCCS_OK;
+}
+
 extern "C" int cccs_sink_count(void* user, const cccs_sparse_column* column) {
     if (!user || !column) return 1;
     cccs_sink_totals* t = static_cast<cccs_sink_totals*>(user);

@@ -1,0 +1,312 @@
+// server.cpp -- request loop for single-beam dose and beamlet-column requests, the calls a
+// beam-orientation optimiser issues while it anneals.
+//
+// * Resident state: each slot owns a context (density, kernel tables and cone angles stay on
+//   the device) with its own stream.  Slots run concurrently, so their work overlaps on the GPU.
+// * Continuous batching: a free slot takes every queued request of the head request's kind that
+//   fits its share, not just one.  Requests that arrive while a batch runs join the next batch.
+//   Beamlet requests of one batch share launches (one multi-beam job).
+// * Admission by HBM footprint: a request is admitted only while the device bytes in flight plus
+//   its estimate stay within the budget.  A request larger than the whole budget runs alone.
+// * Latency: each request's submit-to-completion time.  Statistics give p50/p99.
+// Host code only; it drives the library through its C entry points.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cccs.h"
+#include "errors.hpp"
+
+using namespace cccs;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t id = 0;
+    int kind = 0;
+    cccs_beam beam{};
+    std::vector<float> fluence;
+    float threshold = 0.f;
+    int mode = 0;
+    uint64_t bytes = 0;  // admission estimate
+    Clock::time_point submitted, finished;
+    bool done = false;
+    int status = CCCS_OK;
+    std::string error;
+    int batch_size = 0;
+    std::vector<float> dose;  // beam-dose result, full volume
+    // beamlet result: columns back to back
+    std::vector<uint64_t> keys;
+    std::vector<float> values;
+    std::vector<cccs_sparse_column> cols;  // keys/values fields hold offsets until delivery
+};
+
+}  // namespace
+
+struct cccs_server {
+    std::vector<cccs_ctx*> slots;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
+    uint64_t next_id = 1;
+    uint64_t budget = 0, in_flight = 0;
+    int max_batch = 64;
+    bool stop = false;
+    size_t nvox = 0, nbbox = 0;
+    std::vector<double> latencies;
+    uint64_t batches = 0, batched = 0;
+};
+
+namespace {
+
+// Device bytes a request may need while it runs: for a beam dose the TERMA over the calc_bbox,
+// the compact REV store (bounded by a few volumes) and the dose volume; for a beamlet request the
+// library's own per-batch footprint.
+uint64_t estimate_bytes(cccs_server* s, cccs_ctx* ctx, const Request& r) {
+    if (r.kind == CCCS_REQ_BEAMLET) {
+        uint64_t b = 0;
+        if (cccs_beamlet_footprint(ctx, &r.beam, r.fluence.data(), &b) == CCCS_OK) return b;
+        clear_error();
+        return 0;
+    }
+    return sizeof(float) * (uint64_t)(s->nbbox + 4 * s->nvox);
+}
+
+struct Collect {
+    std::vector<std::shared_ptr<Request>>* batch;
+    std::vector<int> owner;  // batch member of each beam of the call
+};
+
+int collect_column(void* user, const cccs_sparse_column* c) {
+    Collect* k = static_cast<Collect*>(user);
+    Request& r = *(*k->batch)[k->owner[c->beam_index]];
+    cccs_sparse_column o = *c;
+    o.beam_index = 0;
+    o.keys = reinterpret_cast<const uint64_t*>((uintptr_t)r.keys.size());  // offset, fixed at delivery
+    r.keys.insert(r.keys.end(), c->keys, c->keys + c->nnz);
+    r.values.insert(r.values.end(), c->values, c->values + c->nnz);
+    r.cols.push_back(o);
+    return 0;
+}
+
+void run_batch(cccs_server* s, cccs_ctx* ctx, std::vector<std::shared_ptr<Request>>& batch) {
+    if (batch.front()->kind == CCCS_REQ_BEAM_DOSE) {
+        for (auto& r : batch) {
+            r->dose.assign(s->nvox, 0.f);
+            const float* fl = r->fluence.data();
+            r->status = cccs_beam_dose(ctx, &r->beam, 1, &fl, r->dose.data());
+            if (r->status != CCCS_OK) r->error = cccs_last_error();
+        }
+        return;
+    }
+    // beamlet requests: one call over all beams of the batch (shared launches)
+    std::vector<cccs_beam> beams;
+    std::vector<const float*> fl;
+    Collect k{&batch, {}};
+    for (size_t i = 0; i < batch.size(); i++) {
+        beams.push_back(batch[i]->beam);
+        fl.push_back(batch[i]->fluence.data());
+        k.owner.push_back((int)i);
+    }
+    // one call per (threshold, mode) group; in practice the batch shares them
+    const float thr = batch.front()->threshold;
+    const int mode = batch.front()->mode;
+    bool uniform = true;
+    for (auto& r : batch) uniform = uniform && r->threshold == thr && r->mode == mode;
+    if (uniform) {
+        const int st = cccs_beamlet_dose(ctx, beams.data(), (int)beams.size(), fl.data(), thr, mode, collect_column, &k);
+        if (st == CCCS_OK) {
+            for (auto& r : batch) r->status = CCCS_OK;
+            return;
+        }
+        // one bad request must not fail the others: run them one by one
+        for (auto& r : batch) { r->keys.clear(); r->values.clear(); r->cols.clear(); }
+    }
+    for (size_t i = 0; i < batch.size(); i++) {
+        std::vector<std::shared_ptr<Request>> one{batch[i]};
+        Collect k1{&one, {0}};
+        const float* f1 = batch[i]->fluence.data();
+        int st = cccs_beamlet_dose(ctx, &batch[i]->beam, 1, &f1, batch[i]->threshold, batch[i]->mode, collect_column, &k1);
+        batch[i]->status = st;
+        if (st != CCCS_OK) batch[i]->error = cccs_last_error();
+    }
+}
+
+void worker(cccs_server* s, int slot) {
+    cccs_ctx* ctx = s->slots[slot];
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> batch;
+        uint64_t bytes = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                if (s->stop) return true;
+                if (s->queue.empty()) return false;
+                const auto& h = s->queue.front();
+                return s->in_flight == 0 || s->in_flight + h->bytes <= s->budget;
+            });
+            if (s->stop && s->queue.empty()) return;
+            const int kind = s->queue.front()->kind;
+            while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
+                const auto& h = s->queue.front();
+                if (h->kind != kind) break;
+                if (!batch.empty() && s->in_flight + bytes + h->bytes > s->budget) break;
+                bytes += h->bytes;
+                batch.push_back(h);
+                s->queue.pop_front();
+            }
+            s->in_flight += bytes;
+        }
+        run_batch(s, ctx, batch);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->in_flight -= bytes;
+            const auto now = Clock::now();
+            for (auto& r : batch) {
+                r->finished = now;
+                r->done = true;
+                r->batch_size = (int)batch.size();
+                s->latencies.push_back(std::chrono::duration<double, std::milli>(now - r->submitted).count());
+            }
+            s->batches++;
+            s->batched += batch.size();
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" int cccs_server_create(const cccs_problem* problem, int device, int slots, uint64_t hbm_budget,
+                                  int max_batch, cccs_server** out) {
+    clear_error();
+    if (!problem || !out || slots < 1 || slots > 16 || max_batch < 1) return set_error(CCCS_EINVAL, "bad argument");
+    *out = nullptr;
+    std::unique_ptr<cccs_server> s(new cccs_server());
+    for (int i = 0; i < slots; i++) {
+        cccs_ctx* c = nullptr;
+        int st = cccs_ctx_create(problem, device, &c);
+        if (st != CCCS_OK) {
+            std::string e = cccs_last_error();
+            for (cccs_ctx* d : s->slots) cccs_ctx_destroy(d);
+            return set_error(st, "server slot " + std::to_string(i) + ": " + e);
+        }
+        s->slots.push_back(c);
+    }
+    s->budget = hbm_budget;
+    s->max_batch = max_batch;
+    s->nvox = (size_t)problem->size[0] * problem->size[1] * problem->size[2];
+    s->nbbox = (size_t)problem->bbox_size[0] * problem->bbox_size[1] * problem->bbox_size[2];
+    cccs_server* raw = s.release();
+    for (int i = 0; i < slots; i++) raw->workers.emplace_back(worker, raw, i);
+    *out = raw;
+    return CCCS_OK;
+}
+
+extern "C" int cccs_server_submit(cccs_server* s, int kind, const cccs_beam* beam, const float* fluence,
+                                  float threshold, int mode, uint64_t* id) {
+    clear_error();
+    if (!s || !beam || !fluence || !id) return set_error(CCCS_EINVAL, "null argument");
+    if (kind != CCCS_REQ_BEAM_DOSE && kind != CCCS_REQ_BEAMLET) return set_error(CCCS_EINVAL, "unknown request kind");
+    auto r = std::make_shared<Request>();
+    r->kind = kind;
+    r->beam = *beam;
+    const size_t nf = (size_t)beam->fmap_size[0] * beam->fmap_size[1];
+    if (nf == 0 || nf > (size_t)4096 * 4096) return set_error(CCCS_EINVAL, "fluence map dims must be in [1, 4096]");
+    r->fluence.assign(fluence, fluence + nf);
+    r->threshold = threshold;
+    r->mode = mode;
+    r->bytes = estimate_bytes(s, s->slots[0], *r);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return set_error(CCCS_EBUSY, "server is shutting down");
+        r->id = s->next_id++;
+        r->submitted = Clock::now();
+        s->requests[r->id] = r;
+        s->queue.push_back(r);
+        *id = r->id;
+    }
+    s->cv_work.notify_one();
+    return CCCS_OK;
+}
+
+extern "C" int cccs_server_wait(cccs_server* s, uint64_t id, float* dose, cccs_sparse_sink sink, void* user,
+                                double* latency_ms, int* batch_size) {
+    clear_error();
+    if (!s) return set_error(CCCS_EINVAL, "null server");
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->requests.find(id);
+        if (it == s->requests.end()) return set_error(CCCS_EINVAL, "unknown request id");
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->done; });
+        s->requests.erase(it);
+    }
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->finished - r->submitted).count();
+    if (batch_size) *batch_size = r->batch_size;
+    if (r->status != CCCS_OK) return set_error(r->status, r->error);
+    if (r->kind == CCCS_REQ_BEAM_DOSE) {
+        if (dose) std::memcpy(dose, r->dose.data(), sizeof(float) * r->dose.size());
+        return CCCS_OK;
+    }
+    if (sink) {
+        for (cccs_sparse_column c : r->cols) {
+            const size_t off = (size_t)(uintptr_t)c.keys;
+            c.keys = r->keys.data() + off;
+            c.values = r->values.data() + off;
+            if (sink(user, &c) != 0) return set_error(CCCS_EIO, "sparse sink aborted");
+        }
+    }
+    return CCCS_OK;
+}
+
+extern "C" int cccs_server_stats(cccs_server* s, cccs_server_info* out) {
+    clear_error();
+    if (!s || !out) return set_error(CCCS_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::vector<double> v = s->latencies;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) {
+        if (v.empty()) return 0.0;
+        const double x = q * (double)(v.size() - 1);
+        const size_t i = (size_t)x;
+        const double f = x - (double)i;
+        return i + 1 < v.size() ? v[i] + f * (v[i + 1] - v[i]) : v[i];
+    };
+    out->completed = v.size();
+    out->p50_ms = pct(0.50);
+    out->p99_ms = pct(0.99);
+    out->max_ms = v.empty() ? 0.0 : v.back();
+    out->batches = s->batches;
+    out->mean_batch = s->batches ? (double)s->batched / (double)s->batches : 0.0;
+    out->queued = s->queue.size();
+    return CCCS_OK;
+}
+
+extern "C" int cccs_server_destroy(cccs_server* s) {
+    clear_error();
+    if (!s) return CCCS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (std::thread& t : s->workers) t.join();
+    for (cccs_ctx* c : s->slots) cccs_ctx_destroy(c);
+    delete s;
+    return CCCS_OK;
+}

@@ -367,3 +367,62 @@ def read_f32(path: str, shape) -> np.ndarray:
 def write_f32(path: str, data: np.ndarray) -> None:
     d = np.ascontiguousarray(data, np.float32)
     abi.check(abi.load().cccs_write_f32(path.encode(), abi.fptr(d), d.size), "cccs_write_f32")
+
+class Server:
+    """Request loop over device-resident contexts (cccs_server_*): submit single-beam dose or
+    beamlet-column requests as they arrive, wait for each; p50/p99 from stats()."""
+
+    def __init__(self, spec: "ProblemSpec", slots: int = 2, hbm_budget: int = 64 << 30, max_batch: int = 64,
+                 device: int = 0):
+        self.lib = abi.load()
+        self.spec = spec
+        self._p = spec.struct()
+        h = C.c_void_p()
+        abi.check(self.lib.cccs_server_create(C.byref(self._p), device, slots, hbm_budget, max_batch, C.byref(h)),
+                  "cccs_server_create")
+        self.h = h
+
+    def submit(self, beam: abi.Beam, fmap: np.ndarray, kind: int = abi.CCCS_REQ_BEAM_DOSE, threshold: float = 1e-4,
+               mode: int = abi.CCCS_BEAMLET_FAITHFUL) -> int:
+        f = np.ascontiguousarray(fmap, np.float32)
+        rid = C.c_uint64(0)
+        abi.check(self.lib.cccs_server_submit(self.h, kind, C.byref(beam), abi.fptr(f), threshold, mode, C.byref(rid)),
+                  "cccs_server_submit")
+        return rid.value
+
+    def wait(self, rid: int, kind: int = abi.CCCS_REQ_BEAM_DOSE):
+        """(result, latency_ms, batch_size): a dose volume, or a list of (beamlet, keys, values)."""
+        lat, bs = C.c_double(0), C.c_int(0)
+        if kind == abi.CCCS_REQ_BEAM_DOSE:
+            dose = np.zeros(self.spec.density.shape, np.float32)
+            abi.check(self.lib.cccs_server_wait(self.h, rid, abi.fptr(dose), abi.SPARSE_SINK(), None, C.byref(lat),
+                                                C.byref(bs)), "cccs_server_wait")
+            return dose, lat.value, bs.value
+        cols = []
+
+        def sink_fn(user, colp):
+            c = colp.contents
+            k = np.ctypeslib.as_array(c.keys, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.uint64)
+            v = np.ctypeslib.as_array(c.values, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.float32)
+            cols.append((c.beamlet_index, k, v))
+            return 0
+
+        cb = abi.SPARSE_SINK(sink_fn)
+        abi.check(self.lib.cccs_server_wait(self.h, rid, None, cb, None, C.byref(lat), C.byref(bs)), "cccs_server_wait")
+        return cols, lat.value, bs.value
+
+    def stats(self) -> dict:
+        s = abi.ServerInfo()
+        abi.check(self.lib.cccs_server_stats(self.h, C.byref(s)), "cccs_server_stats")
+        return {k: getattr(s, k) for k, _ in abi.ServerInfo._fields_}
+
+    def close(self):
+        if self.h:
+            abi.check(self.lib.cccs_server_destroy(self.h), "cccs_server_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -181,6 +181,42 @@ int cccs_sink_count(void* user, const cccs_sparse_column* column);
 int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int mode,
                        float* out, int max_beamlets, int* nactive);
 
+/* Device bytes a cccs_beamlet_dose call for this one beam would hold while it runs (packed
+ * grids, one cone direction's REV store, half steps and line table, sparse-output bound); host
+ * planning only, no device work.  No counterpart in the reference. */
+int cccs_beamlet_footprint(const cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, uint64_t* bytes);
+
+/* ---- serving: a request loop over device-resident contexts ----------------------------------
+ * For the calls a beam-orientation optimiser issues while annealing (single-beam dose, or one
+ * beam's beamlet columns).  `slots` contexts (own stream each) run concurrently; a free slot
+ * takes every queued request of the head request's kind that fits (continuous batching: requests
+ * that arrive while a batch runs join the next one); a request is admitted only while the bytes
+ * in flight plus its estimate stay within hbm_budget (a request above the whole budget runs
+ * alone).  Results are the same as the direct calls (cccs_beam_dose / cccs_beamlet_dose) made
+ * alone.  No counterpart in the reference (its dose calculation is one batch program). */
+#define CCCS_REQ_BEAM_DOSE 0
+#define CCCS_REQ_BEAMLET   1
+typedef struct cccs_server cccs_server;
+typedef struct cccs_server_info {
+    uint64_t completed;
+    double   p50_ms, p99_ms, max_ms;   /* submit-to-completion latency                          */
+    uint64_t batches;
+    double   mean_batch;               /* requests per batch                                    */
+    uint64_t queued;
+} cccs_server_info;
+int cccs_server_create(const cccs_problem* problem, int device, int slots, uint64_t hbm_budget,
+                       int max_batch, cccs_server** out);
+/* Copies the beam and its fluence map.  threshold/mode are used by beamlet requests. */
+int cccs_server_submit(cccs_server* server, int kind, const cccs_beam* beam, const float* fluence,
+                       float threshold, int mode, uint64_t* request_id);
+/* Blocks until the request is done, then delivers it: a beam dose into `dose` (full volume, may
+ * be NULL), beamlet columns through `sink` (may be NULL).  The request is forgotten afterwards. */
+int cccs_server_wait(cccs_server* server, uint64_t request_id, float* dose, cccs_sparse_sink sink,
+                     void* user, double* latency_ms, int* batch_size);
+int cccs_server_stats(cccs_server* server, cccs_server_info* out);
+/* Finishes every queued request, then releases the slots. */
+int cccs_server_destroy(cccs_server* server);
+
 /* ---- the reference's file formats (host only) -----------------------------------------------
  * Beam list "az zen [coll] [iso: x y z] [sad: s] [dir: x y z]" in degrees/cm, '#' comments,
  * default SAD 100 cm; each beam is reconfigured (from angles, or from dir: when given).

@@ -1,0 +1,110 @@
+"""The serving loop (cccs_server_*): continuous batching, HBM-budget admission and concurrent
+slots must return, for every request, exactly what the direct call returns for it alone."""
+import numpy as np
+import pytest
+
+from endtoend_amd import abi
+from endtoend_amd import workloads as W
+from endtoend_amd.engine import Engine, Server
+
+
+def _requests():
+    ws = [W.small(n=28, voxel=0.5, gantry=g, couch=c, hetero=True, fmap=(6, 6), beamlet=(1.0, 1.0), weights="rand",
+                  seed=s) for g, c, s in ((30.0, 0.0, 1), (215.0, -45.0, 2), (120.0, 20.0, 3), (300.0, 0.0, 4))]
+    spec = ws[0].spec
+    spec.kernel_extent = 1.0
+    out = []
+    for i, w in enumerate(ws):
+        f = w.fmaps[0].copy()
+        f.ravel()[::(i + 2)] = 0.0
+        out.append((w.beams[0], f))
+    return spec, out
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slots", [1, 2])
+def test_results_equal_direct_calls_alone(slots):
+    spec, reqs = _requests()
+    with Engine(spec) as eng:
+        ref_dose = [eng.beam_dose([b], [f]) for b, f in reqs]
+        ref_cols = [[(c[1], c[2], c[3]) for c in eng.beamlet_dose([b], [f], threshold=1e-4)] for b, f in reqs]
+    with Server(spec, slots=slots) as srv:
+        ids = []
+        for b, f in reqs:     # interleaved kinds, submitted back to back
+            ids.append((abi.CCCS_REQ_BEAM_DOSE, srv.submit(b, f, abi.CCCS_REQ_BEAM_DOSE)))
+            ids.append((abi.CCCS_REQ_BEAMLET, srv.submit(b, f, abi.CCCS_REQ_BEAMLET)))
+        got = [srv.wait(rid, kind) for kind, rid in ids]
+        st = srv.stats()
+    for i in range(len(reqs)):
+        dose, lat, _ = got[2 * i]
+        assert _same(dose, ref_dose[i])
+        cols = got[2 * i + 1][0]
+        assert [c[0] for c in cols] == [c[0] for c in ref_cols[i]]
+        for (bl, k, v), (bl2, k2, v2) in zip(cols, ref_cols[i]):
+            assert np.array_equal(k, k2) and _same(v, v2)
+        assert lat > 0
+    assert st["completed"] == 2 * len(reqs)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+@pytest.mark.gpu
+def test_queued_requests_are_batched():
+    """One slot: requests that arrive while the first runs are taken together (continuous
+    batching), and the beamlet batch shares launches yet matches the lone calls."""
+    spec, reqs = _requests()
+    with Server(spec, slots=1) as srv:
+        ids = [srv.submit(b, f, abi.CCCS_REQ_BEAMLET) for b, f in reqs * 2]
+        res = [srv.wait(r, abi.CCCS_REQ_BEAMLET) for r in ids]
+        st = srv.stats()
+    assert st["completed"] == len(ids)
+    assert st["batches"] < len(ids) and st["mean_batch"] > 1.0
+    assert max(r[2] for r in res) > 1
+    # the same beam twice in one batch gives the same columns twice
+    for a, b in zip(res[:len(reqs)], res[len(reqs):]):
+        assert all(np.array_equal(x[1], y[1]) and _same(x[2], y[2]) for x, y in zip(a[0], b[0]))
+
+
+@pytest.mark.gpu
+def test_tight_hbm_budget_admits_one_request_at_a_time():
+    spec, reqs = _requests()
+    with Engine(spec) as eng:
+        ref = [eng.beam_dose([b], [f]) for b, f in reqs]
+    with Server(spec, slots=2, hbm_budget=1) as srv:
+        ids = [srv.submit(b, f) for b, f in reqs]
+        res = [srv.wait(r) for r in ids]
+        st = srv.stats()
+    assert all(r[2] == 1 for r in res)          # every request ran alone
+    assert st["mean_batch"] == 1.0
+    assert all(_same(r[0], d) for r, d in zip(res, ref))
+
+
+@pytest.mark.gpu
+def test_a_bad_request_fails_alone():
+    spec, reqs = _requests()
+    bad_beam = reqs[0][0].copy()
+    bad_f = np.ones((60, 60), np.float32)        # a 60 cm field: beamlet axes miss the 14 cm box
+    bad_beam.fmap_size[:] = (60, 60)
+    with Server(spec, slots=1) as srv:
+        good = [srv.submit(b, f, abi.CCCS_REQ_BEAMLET) for b, f in reqs[:2]]
+        bad = srv.submit(bad_beam, bad_f, abi.CCCS_REQ_BEAMLET)
+        with pytest.raises(abi.CCCSError) as e:
+            srv.wait(bad, abi.CCCS_REQ_BEAMLET)
+        assert e.value.code == abi.CCCS_EGEOM
+        for r in good:
+            cols, _, _ = srv.wait(r, abi.CCCS_REQ_BEAMLET)
+            assert len(cols) > 0
+
+
+def test_server_without_a_device_reports_edevice():
+    spec = W.small(n=16).spec
+    try:
+        srv = Server(spec, slots=1)
+    except abi.CCCSError as e:
+        assert e.code == abi.CCCS_EDEVICE
+        return
+    srv.close()
+    pytest.skip("a HIP device is present")
