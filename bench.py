@@ -138,6 +138,64 @@ def plain_beam_path(steps: int = 2):
             "ms_per_plan": 1e3 * dt, "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)}}
 
 
+def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
+    """Annealing-style request stream through the serving loop: Poisson arrivals of single-beam
+    dose requests on the 256x256x180 thorax phantom (beams from the 1162-beam 4pi list, 20^2-64^2
+    fluence maps).  Latency is submit-to-completion, measured inside the server."""
+    import threading
+
+    from endtoend_amd import abi
+    from endtoend_amd import workloads as W
+    from endtoend_amd.engine import Server
+    w = W.cfg4(nreq=nreq, rate=rate)
+    with Server(w.spec, slots=slots) as srv:
+        for i in range(2 * slots):                       # warm every slot
+            srv.wait(srv.submit(w.beams[i], w.fmaps[i]))
+        ids, lat, bs, failed = [], [], [], []
+        ready = threading.Semaphore(0)
+
+        def consume():
+            for k in range(nreq):
+                ready.acquire()
+                try:
+                    _, l, b = srv.wait(ids[k])
+                except abi.CCCSError as e:      # e.g. a REV grid beyond the limit (ArraySizeError)
+                    failed.append(e.code)
+                    continue
+                lat.append(l)
+                bs.append(b)
+
+        th = threading.Thread(target=consume)
+        th.start()
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            dt = w.arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(w.beams[i], w.fmaps[i]))
+            ready.release()
+        th.join()
+        span = time.perf_counter() - t0
+        # capacity: the same requests submitted at once (closed loop, queue never empty)
+        ncap = min(nreq, 60)
+        t1 = time.perf_counter()
+        cap_ids = [srv.submit(w.beams[i], w.fmaps[i]) for i in range(ncap)]
+        ok = 0
+        for r in cap_ids:
+            try:
+                srv.wait(r)
+                ok += 1
+            except abi.CCCSError:
+                pass
+        capacity = ok / (time.perf_counter() - t1)
+    return {"workload": "256x256x180 thorax @ 2 mm, single-beam dose requests, beams from the 1162-beam 4pi list, "
+                        f"20^2-64^2 x 5 mm U[0,1] fluence, Poisson arrivals at {rate:g}/s",
+            "requests": nreq, "failed": len(failed), "offered_rate": rate,
+            "completed_per_s": len(lat) / span, "capacity_per_s": capacity, "slots": slots,
+            "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99), "max": max(lat)},
+            "mean_batch": float(np.mean(bs)), "seed": w.seed}
+
+
 def copy_calibration(mib: int, reps: int = 5):
     """Streams mib MiB through the library's copy kernel `reps` times (reads = writes = mib MiB per
     launch), so rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -167,6 +225,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="headline only (profiling runs)")
     ap.add_argument("--plain-beam-only", action="store_true",
                     help="only the plain full-beam path on the thorax plan (profiling runs)")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving stream")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -179,6 +238,9 @@ def main():
         return
     if args.plain_beam_only:
         print(json.dumps(plain_beam_path(steps=args.steps)))
+        return
+    if args.serving_only:
+        print(json.dumps(serving()))
         return
 
     from endtoend_amd import abi
@@ -278,6 +340,7 @@ def main():
         out["cpu_baseline"] = cpu
         eng.close()
         out["plain_beam_path"] = plain_beam_path()
+        out["serving"] = serving()
     else:
         eng.close()
         out["cpu_baseline"] = None

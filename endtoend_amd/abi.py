@@ -133,6 +133,7 @@ EXPORTS = [
     ("cccs_ctx_create", C.c_int, [C.POINTER(Problem), C.c_int, C.POINTER(C.c_void_p)]),
     ("cccs_ctx_destroy", C.c_int, [_P]),
     ("cccs_ctx_memory", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("cccs_ctx_set_schedule", C.c_int, [_P, C.c_char_p, C.c_int]),
     ("cccs_beam_dose", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.POINTER(_fp), _fp]),
     ("cccs_beam_dose_device", C.c_int, [_P, C.POINTER(Beam), C.c_int, _P, _P]),
     ("cccs_beam_terma", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp]),

@@ -72,8 +72,10 @@ struct TermaArgs {
     uint32_t* counts;         // per-block Siddon step counts (profiling) or nullptr
 };
 
-// One lane per calc_bbox voxel (x fastest => coalesced store).  Each lane marches its own ray;
-// neighbouring lanes follow near-parallel rays, so the density taps stay in L1/L2.
+// One lane per calc_bbox voxel (x fastest => coalesced store).  Each lane marches its own ray
+// (the rpl-only march: same crossings and segments as cudaSiddon, fetch coordinates formed with
+// reciprocal voxel sizes); neighbouring lanes follow near-parallel rays, so the density taps stay
+// in L1/L2.
 __global__ void __launch_bounds__(256) terma_kernel(TermaArgs a) {
     const int X = blockIdx.x * 64 + threadIdx.x;
     const int Y = blockIdx.y * 4 + threadIdx.y;
@@ -81,6 +83,7 @@ __global__ void __launch_bounds__(256) terma_kernel(TermaArgs a) {
     const bool in = (X < a.bx && Y < a.by && Z < a.bz);
     uint32_t steps = 0;
     float acc = 0.f;
+    const v3 ivox = mk3(1.f / a.vd.voxel.x, 1.f / a.vd.voxel.y, 1.f / a.vd.voxel.z);
     if (!in) {
     } else if (a.ss > 1) {
         // supersampled: sub-voxel centres, summed in sub-voxel order then averaged
@@ -93,7 +96,7 @@ __global__ void __launch_bounds__(256) terma_kernel(TermaArgs a) {
                        a.vd.start.y + ((float)a.oy + Y - 0.5f) * a.vd.voxel.y + (sy + 0.5f) * subsz.y,
                        a.vd.start.z + ((float)a.oz + Z - 0.5f) * a.vd.voxel.z + (sz + 0.5f) * subsz.z);
             int it = 0;
-            acc += point_terma(p, a.fmap, a.beam, a.spec, a.vd, a.dens, a.bhc, a.beamletnum, &it);
+            acc += point_terma_rpl(p, a.fmap, a.beam, a.spec, a.vd, ivox, a.dens, a.bhc, a.beamletnum, &it);
             steps += (uint32_t)it;
         }
         acc /= (float)nsub;
@@ -101,7 +104,7 @@ __global__ void __launch_bounds__(256) terma_kernel(TermaArgs a) {
         v3 p = mk3(a.vd.start.x + ((float)a.ox + X) * a.vd.voxel.x, a.vd.start.y + ((float)a.oy + Y) * a.vd.voxel.y,
                    a.vd.start.z + ((float)a.oz + Z) * a.vd.voxel.z);
         int it = 0;
-        acc = point_terma(p, a.fmap, a.beam, a.spec, a.vd, a.dens, a.bhc, a.beamletnum, &it);
+        acc = point_terma_rpl(p, a.fmap, a.beam, a.spec, a.vd, ivox, a.dens, a.bhc, a.beamletnum, &it);
         steps = (uint32_t)it;
     }
     if (in) a.terma[(size_t)X + (size_t)a.bx * ((size_t)Y + (size_t)a.by * Z)] = acc;
@@ -212,6 +215,8 @@ struct ConeArgs {
     int ntheta, nradii, nphi;
     float delr;
     uint32_t* counts;         // per-workgroup walk iteration counts (profiling) or nullptr
+    int ramp_min;             // valid segments at least this long use the segment-sum walk
+    int maxn;                 // longest stored line (LDS sizing)
     VolDesc vd;
     m33 binv;                 // PVCS <- BEV rotation (index units)
     v3 off;                   // (bevStart - start) / voxel
@@ -257,6 +262,150 @@ __device__ __forceinline__ void line_interval(const ConeArgs& a, const ConeDev& 
     }
 }
 
+// ---- segment-sum form of the two walks --------------------------------------------------------
+// walk_both visits every sample of the segment for every destination (O(line^2) per line).  The
+// same sums follow in O(nradii) per destination from three facts:
+//  * the radiological distance from destination j to sample k is a difference of the line's
+//    radiological coordinates: rho_k = P - c_k backward (P = boundary between j-1 and j, c_k the
+//    centre of sample k) and c_k - P forward (P = boundary between j and j+1);
+//  * the cumulative kernel C is linear on each radial interval (R_{s-1}, R_s]:
+//    C(rho) = beta_s + sigma_s * rho;
+//  * summing by parts, sum_k T_k (C(rho_k) - C(rho_{k+1})) = sum_k (T_k - T_{k-1}) C(rho_k) + one
+//    end term, so a run of samples in one interval contributes
+//    (beta_s + sigma_s P)(T_hi - T_{lo-1}) - sigma_s (E[hi+1] - E[lo]) with E the prefix sum of
+//    (T_k - T_{k-1}) c_k (FP64).
+// The samples of each interval are found by a lookup in a uniform table over the radiological
+// coordinate.  The walk's exits (distance beyond the last radius, entering the last interval) become
+// the first/last sample of the sum.  Same result as the sequential walk up to FP32 rounding of the
+// accumulated distance.
+struct RampLine {
+    const float* c;       // centres, radiological cm from the segment start; c[n] = +inf
+    const double* E;      // E[k]  = sum_{m<k} (T_m - T_{m-1}) c_m   (T_{-1} = 0), k in [0, n]
+    const double* E2;     // E2[k] = sum_{m<k} (T_m - T_{m+1}) c_m   (T_n = 0),    k in [0, n]
+    const float2* s;      // {half step, TERMA} of the segment's samples
+    const uint16_t* lb;   // lb[u] = first k with c_k >= c0 + u * D, u in [0, U + 1]
+    int n, U;
+    float c0, cn, invD;
+};
+
+// first k with c_k >= y
+__device__ __forceinline__ int ramp_lb(const RampLine& L, float y) {
+    if (!(y > L.c0)) return 0;
+    if (y > L.cn) return L.n;
+    // biased by 1% of a cell so that the cell's threshold never exceeds y after rounding
+    int u = (int)((y - L.c0) * L.invD - 0.01f);
+    u = min(max(u, 0), L.U - 1);
+    int k = L.lb[u];
+    if (L.lb[u + 1] - k > 8) {  // a crowded cell (low density): bisect up to the next cell's end
+        int hi = L.lb[u + 2];
+        while (k < hi) {
+            const int mid = (k + hi) >> 1;
+            if (L.c[mid] < y) k = mid + 1; else hi = mid;
+        }
+    }
+    while (L.c[k] < y) k++;
+    return k;
+}
+
+// first k with c_k > y
+__device__ __forceinline__ int ramp_ub(const RampLine& L, float y) {
+    if (y < L.c0) return 0;
+    if (!(y < L.cn)) return L.n;
+    int u = (int)((y - L.c0) * L.invD - 0.01f);
+    u = min(max(u, 0), L.U - 1);
+    int k = L.lb[u];
+    if (L.lb[u + 1] - k > 8) {
+        int hi = L.lb[u + 2];
+        while (k < hi) {
+            const int mid = (k + hi) >> 1;
+            if (L.c[mid] <= y) k = mid + 1; else hi = mid;
+        }
+    }
+    while (L.c[k] <= y) k++;
+    return k;
+}
+
+// Backward walk of destination j (toward lower indices), kernel row given as sigma/beta.
+__device__ __forceinline__ double ramp_back(const RampLine& L, int j, const double* __restrict__ sig,
+                                            const double* __restrict__ beta, const float* __restrict__ R, int nr,
+                                            uint32_t& iters) {
+    if (j == 0) return 0.0;
+    const float2 ep = L.s[j - 1];
+    const double P = (double)L.c[j - 1] + (double)ep.x;
+    const int nlast = ramp_lb(L, (float)(P - (double)R[nr - 1]));
+    const int nprev = ramp_lb(L, (float)(P - (double)R[nr - 2]));
+    const int a = max(nlast, nprev - 1);  // last sample the walk reaches
+    if (a >= j) return 0.0;
+    double sum = 0.0, Thi = (double)ep.y, Ehi = L.E[j];
+    int hiend = j;
+    for (int s = 0; s < nr; s++) {
+        int ns = s == nr - 1 ? nlast : (s == nr - 2 ? nprev : ramp_lb(L, (float)(P - (double)R[s])));
+        ns = max(ns, a);
+        if (ns < hiend) {  // samples [ns, hiend) lie in interval s
+            const double Tlo = ns > 0 ? (double)L.s[ns - 1].y : 0.0;
+            const double Elo = L.E[ns];
+            sum += (beta[s] + sig[s] * P) * (Thi - Tlo) - sig[s] * (Ehi - Elo);
+            if (ns == a) sum += Tlo * (beta[s] + sig[s] * (P - (double)L.c[a]));
+            hiend = ns;
+            Thi = Tlo;
+            Ehi = Elo;
+            iters++;
+        }
+        if (hiend == a) break;
+    }
+    return sum;
+}
+
+// Forward walk of destination j (toward higher indices).
+__device__ __forceinline__ double ramp_fwd(const RampLine& L, int j, const double* __restrict__ sig,
+                                           const double* __restrict__ beta, const float* __restrict__ R, int nr,
+                                           uint32_t& iters) {
+    if (j >= L.n - 1) return 0.0;
+    const float2 en = L.s[j + 1];
+    const double P = (double)L.c[j + 1] - (double)en.x;
+    const int mlast = ramp_ub(L, (float)(P + (double)R[nr - 1])) - 1;
+    const int mprev = ramp_ub(L, (float)(P + (double)R[nr - 2])) - 1;
+    const int b = min(mlast, mprev + 1);  // last sample the walk reaches
+    if (b <= j) return 0.0;
+    double sum = 0.0, Tlo = (double)en.y, Elo = L.E2[j + 1];
+    int lo = j + 1;
+    for (int s = 0; s < nr; s++) {
+        int ms = s == nr - 1 ? mlast : (s == nr - 2 ? mprev : ramp_ub(L, (float)(P + (double)R[s])) - 1);
+        ms = min(ms, b);
+        if (ms >= lo) {  // samples [lo, ms] lie in interval s
+            const double Thi = ms + 1 < L.n ? (double)L.s[ms + 1].y : 0.0;
+            const double Ehi = L.E2[ms + 1];
+            sum += (beta[s] - sig[s] * P) * (Tlo - Thi) + sig[s] * (Ehi - Elo);
+            if (ms == b) sum += Thi * (beta[s] + sig[s] * ((double)L.c[b] - P));
+            lo = ms + 1;
+            Tlo = Thi;
+            Elo = Ehi;
+            iters++;
+        }
+        if (lo > b) break;
+    }
+    return sum;
+}
+
+// Exclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ double wave_excl_sum(double v) {
+    const int lane = threadIdx.x & 63;
+    double x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_up(x, o, 64);
+        if (lane >= o) x += u;
+    }
+    const double e = __shfl_up(x, 1, 64);
+    return lane == 0 ? 0.0 : e;
+}
+
+// LDS bytes of the cone kernel for lines of up to maxn samples (with the segment-sum arrays).
+inline size_t cone_lds_bytes(uint32_t maxn, bool ramp) {
+    if (!ramp) return sizeof(float2) * maxn;
+    const size_t n1 = (size_t)maxn + 1, nc = ((size_t)maxn + 2 + 1) & ~(size_t)1;
+    return 2 * sizeof(double) * n1 + sizeof(float) * nc + sizeof(float2) * maxn + sizeof(uint16_t) * (maxn + 2);
+}
+
 // Pass 1 of the cone pipeline: the stored interval of every REV line (offsets come from a scan).
 __global__ void __launch_bounds__(256) line_interval_kernel(ConeArgs a) {
     const uint64_t line = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -278,8 +427,9 @@ __global__ void __launch_bounds__(256) line_interval_kernel(ConeArgs a) {
 // cudaRaytrace; then every valid sample runs both collapsed-cone walks from LDS and the interval's
 // cone dose is written to the compact REV store.
 __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
-    extern __shared__ float2 s_line[];
+    extern __shared__ double s_dyn[];
     __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
+    __shared__ double sSig[2][kMaxRadii], sBeta[2][kMaxRadii];
     __shared__ int s_lim[2];
     const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -288,6 +438,15 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
     if (meta.len > 0) {
         const ConeDev& c = a.cones[cone_of_line(a.cones, a.ncones, line)];
         const int nr = a.nradii;
+        const int m = meta.len;
+        const bool ramp_lds = a.ramp_min <= a.maxn;
+        // LDS: [E | E2 | c | line | lb] with the segment-sum arrays, else [line]
+        const int n1 = a.maxn + 1, nc = (a.maxn + 3) & ~1;
+        double* rE = s_dyn;
+        double* rE2 = rE + n1;
+        float* rc = reinterpret_cast<float*>(rE2 + n1);
+        float2* s_line = ramp_lds ? reinterpret_cast<float2*>(rc + nc) : reinterpret_cast<float2*>(s_dyn);
+        uint16_t* rlb = reinterpret_cast<uint16_t*>(s_line + a.maxn);
         if (tid < nr) {
             const float* K = a.tables;
             const float* R = a.tables + (size_t)a.ntheta * nr;
@@ -299,7 +458,7 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
         const uint64_t l = line - c.line0;
         const int tY = (int)(l % (uint64_t)c.ny);
         const int tZ = (int)(l / (uint64_t)c.ny);
-        const int i0 = meta.i0, m = meta.len;
+        const int i0 = meta.i0;
         float* out = a.rev + meta.off;
         for (int j = tid; j < m; j += 64) {
             const int i = i0 + j;
@@ -320,22 +479,99 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
         }
         if (tid == 0) { s_lim[0] = m; s_lim[1] = -1; }
         __syncthreads();
+        int nvalid = 0;
         for (int j = tid; j < m; j += 64)
             if (s_line[j].x >= 0.f) {
                 atomicMin(&s_lim[0], j);
                 atomicMax(&s_lim[1], j);
+                nvalid++;
             }
+        for (int o = 32; o > 0; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
         __syncthreads();
         // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
         const int vlo = s_lim[0], vhi = s_lim[1];
         const float nphi_f = (float)a.nphi;
         for (int j = tid; j < m; j += 64)
             if (j < vlo || j > vhi) out[j] = 0.f;
-        for (int base = vlo; base <= vhi; base += 64) {
-            const int j = base + tid;
-            const bool act = j <= vhi && s_line[min(j, vhi)].x >= 0.f;
-            const float v = walk_both(s_line, min(j, vhi), vlo, vhi, sR, sIW, sF, sB, nr, act, iters);
-            if (j <= vhi) out[j] = act ? v / nphi_f : 0.f;
+        const int n = vhi - vlo + 1;
+        if (n >= a.ramp_min && nvalid == n) {
+            // kernel rows as C(rho) = beta_s + sigma_s * rho on (R_{s-1}, R_s]
+            if (tid < 2 * nr) {
+                const int d = tid / nr, q = tid - d * nr;
+                const float* row = d == 0 ? sF : sB;
+                const float prev = q == 0 ? 0.f : row[q - 1];
+                const float sg = (row[q] - prev) * sIW[q];
+                sSig[d][q] = (double)sg;
+                sBeta[d][q] = (double)prev - (double)sg * (q == 0 ? 0.0 : (double)sR[q - 1]);
+            }
+            const float2* sv = s_line + vlo;
+            // radiological centres (FP64 scan) and the two FP64 prefix sums, chunk per lane
+            const int per = (n + 63) >> 6;
+            const int ca = min(n, tid * per), cb = min(n, ca + per);
+            double loc = 0.0;
+            for (int k = ca; k < cb; k++) loc += 2.0 * (double)sv[k].x;
+            double P = wave_excl_sum(loc);
+            double le = 0.0, le2 = 0.0;
+            for (int k = ca; k < cb; k++) {
+                const float2 e = sv[k];
+                const float ck = (float)(P + (double)e.x);
+                rc[k] = ck;
+                P += 2.0 * (double)e.x;
+                const double tp = k > 0 ? (double)sv[k - 1].y : 0.0, tn = k + 1 < n ? (double)sv[k + 1].y : 0.0;
+                le += ((double)e.y - tp) * (double)ck;
+                le2 += ((double)e.y - tn) * (double)ck;
+            }
+            double e1 = wave_excl_sum(le), e2 = wave_excl_sum(le2);
+            for (int k = ca; k < cb; k++) {
+                const double ck = (double)rc[k], t = (double)sv[k].y;
+                const double tp = k > 0 ? (double)sv[k - 1].y : 0.0, tn = k + 1 < n ? (double)sv[k + 1].y : 0.0;
+                rE[k] = e1;
+                rE2[k] = e2;
+                e1 += (t - tp) * ck;
+                e2 += (t - tn) * ck;
+            }
+            if (ca < cb && cb == n) {
+                rE[n] = e1;
+                rE2[n] = e2;
+                rc[n] = INFINITY;
+            }
+            __syncthreads();
+            // lookup table: U = n cells over [c0, c_{n-1}]
+            RampLine L;
+            L.c = rc;
+            L.E = rE;
+            L.E2 = rE2;
+            L.s = sv;
+            L.lb = rlb;
+            L.n = n;
+            L.U = n;
+            L.c0 = rc[0];
+            L.cn = rc[n - 1];
+            const float range = L.cn - L.c0;
+            const float D = range > 0.f ? range / (float)n : 0.f;
+            L.invD = range > 0.f ? (float)n / range : 0.f;
+            for (int u = tid; u <= n + 1; u += 64) {
+                const float t = L.c0 + (float)u * D;
+                int lo = 0, hi = n;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (rc[mid] < t) lo = mid + 1; else hi = mid;
+                }
+                rlb[u] = (uint16_t)lo;
+            }
+            __syncthreads();
+            for (int j = tid; j < n; j += 64) {
+                const double v = ramp_back(L, j, sSig[0], sBeta[0], sR, nr, iters) +
+                                 ramp_fwd(L, j, sSig[1], sBeta[1], sR, nr, iters);
+                out[vlo + j] = (float)v / nphi_f;
+            }
+        } else {
+            for (int base = vlo; base <= vhi; base += 64) {
+                const int j = base + tid;
+                const bool act = j <= vhi && s_line[min(j, vhi)].x >= 0.f;
+                const float v = walk_both(s_line, min(j, vhi), vlo, vhi, sR, sIW, sF, sB, nr, act, iters);
+                if (j <= vhi) out[j] = act ? v / nphi_f : 0.f;
+            }
         }
     }
     if (a.counts) block_count(iters, a.counts, blockIdx.x);
@@ -613,8 +849,11 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     ca.counts = prof_counter(ctx, g.total_lines);
     uint32_t maxn = 1;
     for (int r = 0; r < g.nrays; r++) maxn = std::max(maxn, g.rev_size[r][0]);
+    ca.maxn = (int)maxn;
+    ca.ramp_min = ctx->sched.cone_ramp_min;
+    const size_t lds = cone_lds_bytes(maxn, ca.ramp_min <= ca.maxn);
     int tok = prof_begin(ctx, K_CONE);
-    hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), sizeof(float2) * maxn, ctx->stream, ca);
+    hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), lds, ctx->stream, ca);
     // algorithmic traffic: density + TERMA in, cone dose out, per stored sample
     prof_end(ctx, tok, 12.0 * (double)stored, 0.0, stored);
     CCCS_HIP(hipGetLastError());

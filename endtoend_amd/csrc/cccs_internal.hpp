@@ -189,7 +189,7 @@ CCCS_HD float siddon_march(v3 this_voxel, v3 start, v3 end, v3 voxel, v3 src, co
 // Radiological path length of the same segment structure as siddon_march (identical plane
 // sequence and segment lengths), with the midpoint fetch coordinate formed by multiplying with the
 // reciprocal voxel size instead of dividing: the fetch coordinates may differ in the last bit, so
-// rpl agrees with siddon_march to FP32 rounding, not bit for bit.  Used for beamlet TERMA.
+// rpl agrees with siddon_march to FP32 rounding, not bit for bit.  Used for TERMA (both paths).
 CCCS_HD float siddon_rpl(v3 this_voxel, v3 start, v3 end, v3 voxel, v3 ivoxel, v3 src, const float* __restrict__ dens,
                          int nx, int ny, int nz, int* iters) {
     v3 diff = this_voxel - src;

@@ -325,6 +325,17 @@ extern "C" int cccs_ctx_memory(const cccs_ctx* c, uint64_t* resident, uint64_t* 
     return CCCS_OK;
 }
 
+extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
+    clear_error();
+    if (!c || !key) return set_error(CCCS_EINVAL, "null argument");
+    if (std::strcmp(key, "cone_ramp_min") == 0) {
+        if (value < 1) return set_error(CCCS_EINVAL, "cone_ramp_min must be >= 1");
+        c->sched.cone_ramp_min = value;
+        return CCCS_OK;
+    }
+    return set_error(CCCS_EINVAL, std::string("unknown schedule key: ") + key);
+}
+
 // ---- memory helpers ----------------------------------------------------------------------------
 extern "C" int cccs_dev_alloc(cccs_ctx* ctx, size_t bytes, void** ptr) {
     clear_error();

@@ -90,6 +90,16 @@ struct Profiler {
 struct LineMeta;
 }  // namespace cccs
 
+namespace cccs {
+// Launch-schedule knobs (cccs_ctx_set_schedule).  They change how the work is scheduled, never
+// which samples contribute.
+struct Schedule {
+    // plain-path cone lines whose valid segment has at least this many samples use the
+    // segment-sum walk (O(nradii) per destination) instead of the sequential walk (O(line))
+    int cone_ramp_min = 64;
+};
+}  // namespace cccs
+
 struct cccs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -109,6 +119,7 @@ struct cccs_ctx {
     cccs::HostBuf h_bl, h_bl_out;
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
+    cccs::Schedule sched;
 };
 
 namespace cccs {

@@ -3,9 +3,10 @@
 //
 // * Resident state: each slot owns a context (density, kernel tables and cone angles stay on
 //   the device) with its own stream.  Slots run concurrently, so their work overlaps on the GPU.
-// * Continuous batching: a free slot takes every queued request of the head request's kind that
-//   fits its share, not just one.  Requests that arrive while a batch runs join the next batch.
-//   Beamlet requests of one batch share launches (one multi-beam job).
+// * Continuous batching: a free slot takes every queued beamlet request that fits its share, not
+//   just one, and runs them as one multi-beam job (shared launches).  Requests that arrive while
+//   a batch runs join the next batch.  A full-beam request fills the device alone and shares no
+//   launches, so a slot takes one at a time; concurrent slots overlap them.
 // * Admission by HBM footprint: a request is admitted only while the device bytes in flight plus
 //   its estimate stay within the budget.  A request larger than the whole budget runs alone.
 // * Latency: each request's submit-to-completion time.  Statistics give p50/p99.
@@ -101,12 +102,34 @@ int collect_column(void* user, const cccs_sparse_column* c) {
     return 0;
 }
 
-void run_batch(cccs_server* s, cccs_ctx* ctx, std::vector<std::shared_ptr<Request>>& batch) {
+// Per-slot device buffers of the beam-dose requests (fluence map and dose volume).
+struct SlotBuffers {
+    void* fl = nullptr;
+    size_t fl_bytes = 0;
+    void* dose = nullptr;
+};
+
+int beam_dose_request(cccs_server* s, cccs_ctx* ctx, SlotBuffers& sb, Request& r) {
+    const size_t fb = sizeof(float) * r.fluence.size(), db = sizeof(float) * s->nvox;
+    if (fb > sb.fl_bytes) {
+        if (sb.fl) cccs_dev_free(ctx, sb.fl);
+        sb.fl = nullptr;
+        sb.fl_bytes = 0;
+        CCCS_TRY(cccs_dev_alloc(ctx, fb, &sb.fl));
+        sb.fl_bytes = fb;
+    }
+    if (!sb.dose) CCCS_TRY(cccs_dev_alloc(ctx, db, &sb.dose));
+    CCCS_TRY(cccs_dev_upload(ctx, sb.fl, r.fluence.data(), fb));
+    CCCS_TRY(cccs_dev_zero(ctx, sb.dose, db));
+    CCCS_TRY(cccs_beam_dose_device(ctx, &r.beam, 1, static_cast<const float*>(sb.fl), static_cast<float*>(sb.dose)));
+    r.dose.resize(s->nvox);
+    return cccs_dev_download(ctx, r.dose.data(), sb.dose, db);
+}
+
+void run_batch(cccs_server* s, cccs_ctx* ctx, SlotBuffers& sb, std::vector<std::shared_ptr<Request>>& batch) {
     if (batch.front()->kind == CCCS_REQ_BEAM_DOSE) {
         for (auto& r : batch) {
-            r->dose.assign(s->nvox, 0.f);
-            const float* fl = r->fluence.data();
-            r->status = cccs_beam_dose(ctx, &r->beam, 1, &fl, r->dose.data());
+            r->status = beam_dose_request(s, ctx, sb, *r);
             if (r->status != CCCS_OK) r->error = cccs_last_error();
         }
         return;
@@ -146,6 +169,7 @@ void run_batch(cccs_server* s, cccs_ctx* ctx, std::vector<std::shared_ptr<Reques
 
 void worker(cccs_server* s, int slot) {
     cccs_ctx* ctx = s->slots[slot];
+    SlotBuffers sb;
     for (;;) {
         std::vector<std::shared_ptr<Request>> batch;
         uint64_t bytes = 0;
@@ -157,9 +181,16 @@ void worker(cccs_server* s, int slot) {
                 const auto& h = s->queue.front();
                 return s->in_flight == 0 || s->in_flight + h->bytes <= s->budget;
             });
-            if (s->stop && s->queue.empty()) return;
+            if (s->stop && s->queue.empty()) {
+                if (sb.fl) cccs_dev_free(ctx, sb.fl);
+                if (sb.dose) cccs_dev_free(ctx, sb.dose);
+                return;
+            }
             const int kind = s->queue.front()->kind;
-            while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
+            // a full-beam request fills the device by itself and shares no launches with another:
+            // it runs alone on its slot so that it completes as soon as its own work does
+            const int cap = kind == CCCS_REQ_BEAM_DOSE ? 1 : s->max_batch;
+            while (!s->queue.empty() && (int)batch.size() < cap) {
                 const auto& h = s->queue.front();
                 if (h->kind != kind) break;
                 if (!batch.empty() && s->in_flight + bytes + h->bytes > s->budget) break;
@@ -169,7 +200,7 @@ void worker(cccs_server* s, int slot) {
             }
             s->in_flight += bytes;
         }
-        run_batch(s, ctx, batch);
+        run_batch(s, ctx, sb, batch);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->in_flight -= bytes;

@@ -186,6 +186,10 @@ class Engine:
         abi.check(self.lib.cccs_ctx_memory(self.ctx, C.byref(r), C.byref(s)), "cccs_ctx_memory")
         return r.value, s.value
 
+    def set_schedule(self, key: str, value: int):
+        """Launch-schedule knob (cccs_ctx_set_schedule); e.g. "cone_ramp_min"."""
+        abi.check(self.lib.cccs_ctx_set_schedule(self.ctx, key.encode(), int(value)), "cccs_ctx_set_schedule")
+
     def beam_dose(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray]) -> np.ndarray:
         dose = np.zeros(self.spec.density.shape, np.float32)
         fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 
 import numpy as np
 
@@ -21,6 +22,7 @@ from . import engine
 SEED_WEIGHTS = 20261202
 SEED_ANGLES = 1162
 SEED_ARRIVALS = 200
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 # ---- rotations in double (same convention as the library's dm_beam_inv) ----------------------
@@ -238,3 +240,34 @@ def small(n=32, voxel=0.5, gantry=0.0, couch=0.0, fmap=(8, 8), beamlet=(1.0, 1.0
     else:
         f = np.random.default_rng(seed).uniform(0.0, 1.0, (fmap[1], fmap[0])).astype(np.float32)
     return Workload(f"small{n}-g{gantry}-c{couch}", spec, [b], [f], seed, ph)
+
+@dataclasses.dataclass
+class ServingWorkload:
+    name: str
+    spec: engine.ProblemSpec
+    beams: list
+    fmaps: list
+    arrivals: np.ndarray  # seconds from the start of the stream
+    rate: float
+    seed: int
+
+
+def cfg4(seed: int = SEED_ARRIVALS, nreq: int = 300, rate: float = 200.0) -> ServingWorkload:
+    """Serving: the thorax tissue classes on a 256x256x180 phantom at 2 mm; Poisson arrivals at
+    `rate` requests/s; each request is one beam's dose, the beam drawn uniformly from the 1162-beam
+    4pi candidate list (the reference's CCCS/tests/input_files/beamlist_4pi.txt), with an n x n
+    fluence map of 0.5 cm beamlets, n ~ U{20..64}, weights U[0,1]."""
+    ph = thorax(256, 256, 180, 0.2)
+    spec = problem(ph, nphi=8, ntheta=8)
+    cands = engine.load_beam_list(os.path.join(DATA, "beamlist_4pi.txt"))
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    beams, fmaps = [], []
+    for _ in range(nreq):
+        b = cands[int(rng.integers(len(cands)))].copy()
+        n = int(rng.integers(20, 65))
+        b.fmap_size[:] = (n, n)
+        b.beamlet_size[:] = (0.5, 0.5)
+        beams.append(b)
+        fmaps.append(rng.uniform(0.0, 1.0, (n, n)).astype(np.float32))
+    return ServingWorkload("cfg4-thorax256x256x180-serving", spec, beams, fmaps, arrivals, rate, seed)

@@ -129,6 +129,12 @@ int cccs_ctx_create(const cccs_problem* problem, int device, cccs_ctx** out);
 int cccs_ctx_destroy(cccs_ctx* ctx);
 /* HBM bytes held by the context (resident state plus cached work buffers). */
 int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scratch_bytes);
+/* Launch-schedule knob (no counterpart in the reference; results do not depend on it beyond FP32
+ * rounding).  Keys:
+ *   "cone_ramp_min"  full-beam cone lines with at least this many valid samples use the
+ *                    segment-sum walk, O(nradii) per sample, instead of the sequential walk,
+ *                    O(line) per sample (default 64; a value above the longest line disables it). */
+int cccs_ctx_set_schedule(cccs_ctx* ctx, const char* key, int value);
 
 /* ---- full-beam dose: replaces radconvolveTexture (dosecalc-beam) + cuda_sum_device_float_dose.
  * Computes the dose of `nbeams` beams and adds their sum into `dose` (Nz*Ny*Nx, host memory).

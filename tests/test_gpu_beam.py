@@ -124,6 +124,35 @@ def test_dose_parity_small(small_case):
     assert gp >= GAMMA_MIN
 
 
+def test_segment_sum_walk_small(small_case):
+    """Every cone line through the segment-sum walk (cone_ramp_min = 1): same oracle bars, and
+    equal to the sequential walk up to FP32 accumulation of the radiological distance."""
+    name, w, eng = small_case
+    seq = eng.beam_dose(w.beams, w.fmaps)
+    eng.set_schedule("cone_ramp_min", 1)
+    try:
+        ramp = eng.beam_dose(w.beams, w.fmaps)
+    finally:
+        eng.set_schedule("cone_ramp_min", 64)
+    d_orc, _ = O.beam_dose(w.spec, w.beams, w.fmaps)
+    err = max_err_pct(d_orc, ramp)
+    gp = gamma_pass_rate(d_orc, ramp, voxel_mm=10 * w.spec.voxel[0])
+    diff = max_err_pct(seq, ramp)
+    print(f"{name}: segment-sum max err {err:.4f} % Dmax, gamma {gp:.5f}, vs sequential {diff:.2e} %", flush=True)
+    assert err <= DOSE_TOL_PCT
+    assert gp >= GAMMA_MIN
+    assert diff <= 1e-3
+    np.testing.assert_array_equal(ramp != 0, seq != 0)
+
+
+def test_schedule_rejects_bad_keys(small_case):
+    _, _, eng = small_case
+    with pytest.raises(Exception):
+        eng.set_schedule("no_such_knob", 1)
+    with pytest.raises(Exception):
+        eng.set_schedule("cone_ramp_min", 0)
+
+
 def test_dose_linear_and_additive(small_case):
     name, w, eng = small_case
     b, f = w.beams[0], w.fmaps[0]
@@ -135,16 +164,21 @@ def test_dose_linear_and_additive(small_case):
 
 
 def test_dose_parity_cfg1_full():
+    """128^3 water at 2.5 mm: most cone lines are long enough for the segment-sum walk."""
     print("[case] cfg1 full size", flush=True)
     w = W.cfg1()
     with Engine(w.spec) as eng:
         d_gpu = eng.beam_dose(w.beams, w.fmaps)
+        eng.set_schedule("cone_ramp_min", 1 << 20)
+        d_seq = eng.beam_dose(w.beams, w.fmaps)
     d_orc, _ = O.beam_dose(w.spec, w.beams, w.fmaps)
     err = max_err_pct(d_orc, d_gpu)
     gp = gamma_pass_rate(d_orc, d_gpu, voxel_mm=2.5, step_mm=0.5)
-    print(f"cfg1: max err {err:.4f} % Dmax, gamma {gp:.5f}", flush=True)
+    diff = max_err_pct(d_seq, d_gpu)
+    print(f"cfg1: max err {err:.4f} % Dmax, gamma {gp:.5f}, segment-sum vs sequential {diff:.2e} %", flush=True)
     assert err <= DOSE_TOL_PCT
     assert gp >= GAMMA_MIN
+    assert diff <= 1e-3
 
 
 @pytest.mark.parametrize("beam", [0, 7])
