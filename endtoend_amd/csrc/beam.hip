@@ -204,8 +204,7 @@ __global__ void __launch_bounds__(256) crop_final_kernel(const float* __restrict
 }
 
 struct ConeArgs {
-    const float* dens;
-    const float* terma;
+    const float2* dt;         // {density, TERMA (0 outside the calc_bbox)} over the CT grid
     const float* tables;      // kernel | radii | inverse widths
     const ConeDev* cones;
     LineMeta* lines;          // one entry per REV line of every cone
@@ -216,7 +215,8 @@ struct ConeArgs {
     float delr;
     uint32_t* counts;         // per-workgroup walk iteration counts (profiling) or nullptr
     int ramp_min;             // valid segments at least this long use the segment-sum walk
-    int maxn;                 // longest stored line (LDS sizing)
+    int maxn;                 // longest stored line of this launch (LDS sizing)
+    int len_lo, len_hi;       // this launch takes the lines whose stored length is in [len_lo, len_hi]
     VolDesc vd;
     m33 binv;                 // PVCS <- BEV rotation (index units)
     v3 off;                   // (bevStart - start) / voxel
@@ -430,12 +430,12 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
     extern __shared__ double s_dyn[];
     __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
     __shared__ double sSig[2][kMaxRadii], sBeta[2][kMaxRadii];
-    __shared__ int s_lim[2];
     const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     const LineMeta meta = a.lines[line];
+    const bool mine = meta.len >= a.len_lo && meta.len <= a.len_hi;
     uint32_t iters = 0;
-    if (meta.len > 0) {
+    if (mine) {
         const ConeDev& c = a.cones[cone_of_line(a.cones, a.ncones, line)];
         const int nr = a.nradii;
         const int m = meta.len;
@@ -469,27 +469,27 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
                 v3 g = a.binv(bev) + a.off;
                 if (g.x >= a.bb0.x && g.x < a.bb1.x && g.y >= a.bb0.y && g.y < a.bb1.y && g.z >= a.bb0.z &&
                     g.z < a.bb1.z) {
-                    float d = tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, g.x + 0.5f, g.y + 0.5f, g.z + 0.5f);
-                    v3 tq = g - a.bb0;
-                    e.x = 0.5f * (a.delr * d);
-                    e.y = tex3_border(a.terma, a.bbx, a.bby, a.bbz, tq.x + 0.5f, tq.y + 0.5f, tq.z + 0.5f);
+                    const float2 dt = tex3_pair(a.dt, a.vd.nx, a.vd.ny, a.vd.nz, g.x + 0.5f, g.y + 0.5f, g.z + 0.5f);
+                    e.x = 0.5f * (a.delr * dt.x);
+                    e.y = dt.y;
                 }
             }
             s_line[j] = e;
         }
-        if (tid == 0) { s_lim[0] = m; s_lim[1] = -1; }
-        __syncthreads();
-        int nvalid = 0;
-        for (int j = tid; j < m; j += 64)
-            if (s_line[j].x >= 0.f) {
-                atomicMin(&s_lim[0], j);
-                atomicMax(&s_lim[1], j);
-                nvalid++;
-            }
-        for (int o = 32; o > 0; o >>= 1) nvalid += __shfl_xor(nvalid, o, 64);
         __syncthreads();
         // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
-        const int vlo = s_lim[0], vhi = s_lim[1];
+        int vlo = m, vhi = -1, nvalid = 0;
+        for (int j = tid; j < m; j += 64)
+            if (s_line[j].x >= 0.f) {
+                vlo = min(vlo, j);
+                vhi = max(vhi, j);
+                nvalid++;
+            }
+        for (int o = 32; o > 0; o >>= 1) {
+            vlo = min(vlo, __shfl_xor(vlo, o, 64));
+            vhi = max(vhi, __shfl_xor(vhi, o, 64));
+            nvalid += __shfl_xor(nvalid, o, 64);
+        }
         const float nphi_f = (float)a.nphi;
         for (int j = tid; j < m; j += 64)
             if (j < vlo || j > vhi) out[j] = 0.f;
@@ -574,7 +574,8 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
             }
         }
     }
-    if (a.counts) block_count(iters, a.counts, blockIdx.x);
+    // every line's slot is written by exactly one of the length-class launches
+    if (a.counts && (mine || (meta.len == 0 && a.len_lo == 1))) wave_count(iters, a.counts, blockIdx.x);
 }
 
 struct PvcsArgs {
@@ -619,6 +620,21 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
         sum += rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
     }
     if (sum != 0.f) a.dose[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] += sum;
+}
+
+// Interleaves density and TERMA over the CT grid for the cone kernel's paired fetch; TERMA is 0
+// outside the calc_bbox, which is what the bbox-bounded TERMA fetch returns there.
+__global__ void __launch_bounds__(256) pack_dt_kernel(const float* __restrict__ dens, const float* __restrict__ terma,
+                                                      int nx, int ny, int nz, int bx0, int by0, int bz0, int bx, int by,
+                                                      int bz, float2* __restrict__ out) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    const size_t v = (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z);
+    const int tx = x - bx0, ty = y - by0, tz = z - bz0;
+    const float t = (tx >= 0 && ty >= 0 && tz >= 0 && tx < bx && ty < by && tz < bz)
+                        ? terma[(size_t)tx + (size_t)bx * ((size_t)ty + (size_t)by * tz)]
+                        : 0.f;
+    out[v] = make_float2(dens[v], t);
 }
 
 __global__ void add_kernel(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
@@ -810,9 +826,11 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
         roff += (uint64_t)c.nx * c.ny * c.nz;
     }
     CCCS_HIP(hipMemcpyAsync(ctx->cones.p, hc, sizeof(ConeDev) * g.nrays, hipMemcpyHostToDevice, ctx->stream));
+    const size_t nvox = vol_voxels(ctx);
+    if (nvox >= (size_t(1) << 31)) return set_error(CCCS_EGEOM, "CT grid beyond 2^31 voxels");
+    CCCS_TRY(ctx->dt.ensure(sizeof(float2) * nvox));
     ConeArgs ca;
-    ca.dens = ctx->d_dens;
-    ca.terma = d_terma;
+    ca.dt = ctx->dt.as<float2>();
     ca.tables = ctx->d_tables;
     ca.cones = ctx->cones.as<ConeDev>();
     ca.lines = ctx->lines.as<LineMeta>();
@@ -832,6 +850,13 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     ca.bb1 = ca.bb0 + ldu3(p.bbox_size);
     ca.bbx = (int)p.bbox_size[0]; ca.bby = (int)p.bbox_size[1]; ca.bbz = (int)p.bbox_size[2];
     if (g.total_lines == 0) return CCCS_OK;
+    {
+        dim3 blk(64, 4, 1), grd((unsigned)((p.size[0] + 63) / 64), (unsigned)((p.size[1] + 3) / 4), p.size[2]);
+        hipLaunchKernelGGL(pack_dt_kernel, grd, blk, 0, ctx->stream, (const float*)ctx->d_dens, d_terma,
+                           (int)p.size[0], (int)p.size[1], (int)p.size[2], (int)p.bbox_start[0],
+                           (int)p.bbox_start[1], (int)p.bbox_start[2], ca.bbx, ca.bby, ca.bbz, ctx->dt.as<float2>());
+        CCCS_HIP(hipGetLastError());
+    }
     // pass 1: stored interval of every line, then offsets into the compact store
     hipLaunchKernelGGL(line_interval_kernel, dim3((unsigned)((g.total_lines + 255) / 256)), dim3(256), 0, ctx->stream,
                        ca);
@@ -849,14 +874,24 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     ca.counts = prof_counter(ctx, g.total_lines);
     uint32_t maxn = 1;
     for (int r = 0; r < g.nrays; r++) maxn = std::max(maxn, g.rev_size[r][0]);
-    ca.maxn = (int)maxn;
     ca.ramp_min = ctx->sched.cone_ramp_min;
-    const size_t lds = cone_lds_bytes(maxn, ca.ramp_min <= ca.maxn);
+    // One launch per stored-length class, each with LDS for its longest line, so that the many
+    // short lines are not held to the occupancy of the longest; a workgroup whose line belongs to
+    // another class leaves at once.
     int tok = prof_begin(ctx, K_CONE);
-    hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), lds, ctx->stream, ca);
+    for (int ci = 0, lo = 1; ci < kConeClasses && lo <= (int)maxn; ci++) {
+        const int hi = std::min<int>(ctx->sched.cone_class_len[ci], (int)maxn);
+        if (hi < lo) continue;
+        ca.len_lo = lo;
+        ca.len_hi = ci == kConeClasses - 1 ? (int)maxn : hi;
+        ca.maxn = ca.len_hi;
+        const size_t lds = cone_lds_bytes((uint32_t)ca.maxn, ca.ramp_min <= ca.maxn);
+        hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), lds, ctx->stream, ca);
+        CCCS_HIP(hipGetLastError());
+        lo = ca.len_hi + 1;
+    }
     // algorithmic traffic: density + TERMA in, cone dose out, per stored sample
     prof_end(ctx, tok, 12.0 * (double)stored, 0.0, stored);
-    CCCS_HIP(hipGetLastError());
     PvcsArgs pa;
     pa.cones = ctx->cones.as<ConeDev>();
     pa.lines = ca.lines;
@@ -947,11 +982,7 @@ extern "C" int cccs_beam_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
         CCCS_TRY(upload_fmap(ctx, beams[i], fluence[i], &dfm));
         CCCS_TRY(beam_dose_one(ctx, beams[i], dfm, d));
     }
-    std::vector<float> tmp(nv);
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    CCCS_HIP(hipMemcpy(tmp.data(), d, nv * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < nv; i++) dose[i] += tmp[i];
-    return CCCS_OK;
+    return download_staged(ctx, dose, d, nv * sizeof(float), true);
 }
 
 extern "C" int cccs_beam_terma(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int beamletnum,

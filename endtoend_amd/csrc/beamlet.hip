@@ -234,11 +234,6 @@ __device__ __forceinline__ float kc_at(float r, const float4* __restrict__ tab, 
     return iv.y + (r - iv.x) * iv.z;
 }
 
-// Sums one uint32 per lane over a 64-lane workgroup into part[slot] (no LDS).
-__device__ __forceinline__ void wave_count(uint32_t v, uint32_t* __restrict__ part, size_t slot) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (threadIdx.x == 0) part[slot] = v;
-}
 
 // FP32 operations counted per sample by the sample stage: the affine pre-cull, the exact pillar
 // mapping (two Rodrigues rotations, two divisions, three fmods, bounds), the fluence-cell test of

@@ -131,6 +131,45 @@ CCCS_HD float tex3_border(const float* __restrict__ v, int nx, int ny, int nz, f
     return g0 + c * (g1 - g0);
 }
 
+// tex3_border over an interleaved {density, TERMA} volume: both channels at one point with one set
+// of taps (8-byte loads, one address per tap).  Each channel follows tex3_border's operation
+// sequence, so a channel equals tex3_border of that channel's volume at the same coordinates.
+CCCS_HD float2 fetch2_or0(const float2* __restrict__ v, int nx, int ny, int nz, int i, int j, int k) {
+    return (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz)
+               ? make_float2(0.f, 0.f)
+               : v[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k)];
+}
+CCCS_HD float lerp3(float c00, float c10, float c01, float c11, float d00, float d10, float d01, float d11, float a,
+                    float b, float c) {
+    float e0 = c00 + a * (c10 - c00), e1 = c01 + a * (c11 - c01);
+    float f0 = d00 + a * (d10 - d00), f1 = d01 + a * (d11 - d01);
+    float g0 = e0 + b * (e1 - e0), g1 = f0 + b * (f1 - f0);
+    return g0 + c * (g1 - g0);
+}
+CCCS_HD float2 tex3_pair(const float2* __restrict__ v, int nx, int ny, int nz, float u, float w, float t) {
+    float xb = u - 0.5f, yb = w - 0.5f, zb = t - 0.5f;
+    if (!(xb > -1.f && yb > -1.f && zb > -1.f && xb < (float)nx && yb < (float)ny && zb < (float)nz))
+        return make_float2(0.f, 0.f);
+    float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+    int i = (int)fx, j = (int)fy, k = (int)fz;
+    float a = xb - fx, b = yb - fy, c = zb - fz;
+    float2 c00, c10, c01, c11, d00, d10, d01, d11;
+    if (i >= 0 && j >= 0 && k >= 0 && i + 1 < nx && j + 1 < ny && k + 1 < nz) {
+        // volumes below 2^31 voxels (checked by the caller): 32-bit tap offsets
+        const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
+        const float2* p = v + ((uint32_t)i + sy * (uint32_t)j + sz * (uint32_t)k);
+        c00 = p[0]; c10 = p[1]; c01 = p[sy]; c11 = p[sy + 1];
+        d00 = p[sz]; d10 = p[sz + 1]; d01 = p[sz + sy]; d11 = p[sz + sy + 1];
+    } else {
+        c00 = fetch2_or0(v, nx, ny, nz, i, j, k); c10 = fetch2_or0(v, nx, ny, nz, i + 1, j, k);
+        c01 = fetch2_or0(v, nx, ny, nz, i, j + 1, k); c11 = fetch2_or0(v, nx, ny, nz, i + 1, j + 1, k);
+        d00 = fetch2_or0(v, nx, ny, nz, i, j, k + 1); d10 = fetch2_or0(v, nx, ny, nz, i + 1, j, k + 1);
+        d01 = fetch2_or0(v, nx, ny, nz, i, j + 1, k + 1); d11 = fetch2_or0(v, nx, ny, nz, i + 1, j + 1, k + 1);
+    }
+    return make_float2(lerp3(c00.x, c10.x, c01.x, c11.x, d00.x, d10.x, d01.x, d11.x, a, b, c),
+                       lerp3(c00.y, c10.y, c01.y, c11.y, d00.y, d10.y, d01.y, d11.y, a, b, c));
+}
+
 // ---- Siddon march (DoseCalcAlgo/cudaSiddon.cu:20-115) -------------------------------------------
 // The float operation sequence is fixed (no contraction: every file is built with
 // -ffp-contract=off) so the segment list is reproducible bit for bit on any IEEE FP32 machine.

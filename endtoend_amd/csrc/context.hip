@@ -303,8 +303,10 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     if (c->d_dens) (void)hipFree(c->d_dens);
     if (c->d_tables) (void)hipFree(c->d_tables);
     c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
-    c->dose.release(); c->col.release(); c->lines.release(); c->scan.release();
-    c->h_small.release(); c->h_cones.release();
+    c->dose.release(); c->col.release(); c->lines.release(); c->scan.release(); c->dt.release();
+    c->h_small.release(); c->h_cones.release(); c->h_stage.release();
+    for (hipEvent_t& e : c->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     c->bl_tab.release(); c->bl_pdose.release(); c->bl_tgrid.release(); c->bl_arena.release();
     c->bl_cnt.release(); c->bl_keys.release(); c->bl_vals.release(); c->bl_fmap.release();
     c->bl_h.release(); c->bl_queue.release(); c->bl_qcnt.release();
@@ -321,7 +323,7 @@ extern "C" int cccs_ctx_memory(const cccs_ctx* c, uint64_t* resident, uint64_t* 
     if (resident) *resident = c->resident;
     if (scratch)
         *scratch = c->terma.bytes + c->fmap.bytes + c->partial.bytes + c->rev.bytes + c->cones.bytes + c->dose.bytes +
-                   c->col.bytes;
+                   c->col.bytes + c->dt.bytes;
     return CCCS_OK;
 }
 
@@ -332,6 +334,14 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
         if (value < 1) return set_error(CCCS_EINVAL, "cone_ramp_min must be >= 1");
         c->sched.cone_ramp_min = value;
         return CCCS_OK;
+    }
+    for (int i = 0; i < kConeClasses - 1; i++) {
+        const std::string k = "cone_class_len" + std::to_string(i);
+        if (k == key) {
+            if (value < 1 || value > kMaxLine) return set_error(CCCS_EINVAL, "class length out of range");
+            c->sched.cone_class_len[i] = value;
+            return CCCS_OK;
+        }
     }
     return set_error(CCCS_EINVAL, std::string("unknown schedule key: ") + key);
 }
@@ -364,13 +374,45 @@ extern "C" int cccs_dev_upload(cccs_ctx* ctx, void* dst, const void* src, size_t
     CCCS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return CCCS_OK;
 }
+namespace cccs {
+int download_staged(cccs_ctx* c, void* dst, const void* src, size_t bytes, bool add_f32) {
+    constexpr size_t kChunk = size_t(8) << 20;
+    CCCS_TRY(c->h_stage.ensure(2 * kChunk));
+    for (hipEvent_t& e : c->stage_ev)
+        if (!e) CCCS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    char* stage = c->h_stage.as<char>();
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    const size_t n = (bytes + kChunk - 1) / kChunk;
+    auto issue = [&](size_t i) -> int {
+        const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+        CCCS_HIP(hipMemcpyAsync(stage + (i & 1) * kChunk, s + off, len, hipMemcpyDeviceToHost, c->stream));
+        CCCS_HIP(hipEventRecord(c->stage_ev[i & 1], c->stream));
+        return CCCS_OK;
+    };
+    if (n > 0) CCCS_TRY(issue(0));
+    for (size_t i = 0; i < n; i++) {
+        if (i + 1 < n) CCCS_TRY(issue(i + 1));
+        CCCS_HIP(hipEventSynchronize(c->stage_ev[i & 1]));
+        const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+        const char* from = stage + (i & 1) * kChunk;
+        if (add_f32) {
+            float* o = reinterpret_cast<float*>(d + off);
+            const float* f = reinterpret_cast<const float*>(from);
+            for (size_t k = 0; k < len / sizeof(float); k++) o[k] += f[k];
+        } else {
+            std::memcpy(d + off, from, len);
+        }
+    }
+    return CCCS_OK;
+}
+}  // namespace cccs
+
 extern "C" int cccs_dev_download(cccs_ctx* ctx, void* dst, const void* src, size_t bytes) {
     clear_error();
     if (!ctx || !dst || !src) return set_error(CCCS_EINVAL, "null argument");
     CCCS_HIP(hipSetDevice(ctx->device));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    CCCS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return CCCS_OK;
+    return download_staged(ctx, dst, src, bytes, false);
 }
 extern "C" int cccs_dev_zero(cccs_ctx* ctx, void* ptr, size_t bytes) {
     clear_error();

@@ -91,12 +91,15 @@ struct LineMeta;
 }  // namespace cccs
 
 namespace cccs {
+constexpr int kConeClasses = 4;
 // Launch-schedule knobs (cccs_ctx_set_schedule).  They change how the work is scheduled, never
 // which samples contribute.
 struct Schedule {
     // plain-path cone lines whose valid segment has at least this many samples use the
     // segment-sum walk (O(nradii) per destination) instead of the sequential walk (O(line))
     int cone_ramp_min = 64;
+    // upper stored-line lengths of the cone-kernel launch classes (the last class takes the rest)
+    int cone_class_len[kConeClasses] = {64, 160, 320, kMaxLine};
 };
 }  // namespace cccs
 
@@ -112,8 +115,9 @@ struct cccs_ctx {
     float* d_dens = nullptr;
     float* d_tables = nullptr;  // kernel (ntheta*nradii) | radii | inverse widths | mean radii | inverse widths
     size_t resident = 0;
-    cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan;
-    cccs::HostBuf h_small, h_cones;
+    cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan, dt;
+    cccs::HostBuf h_small, h_cones, h_stage;  // h_stage: pinned double buffer of download_staged
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // beamlet path: job tables, packed grids, unit-TERMA grids, column arena, sparse output
     cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt;
     cccs::HostBuf h_bl, h_bl_out;
@@ -138,6 +142,11 @@ BeamDev make_beam_dev(const cccs_beam& b);
 uint32_t* prof_counter(cccs_ctx* c, size_t blocks);
 int prof_begin(cccs_ctx* c, int kind);
 void prof_end(cccs_ctx* c, int token, double bytes, double flops, uint64_t items);
+
+// Device-to-host copy of `bytes` (a multiple of 4) through a pinned double buffer: the DMA of one
+// chunk overlaps the host copy (or FP32 add, when add_f32) of the previous one.  Waits for the
+// context stream first.
+int download_staged(cccs_ctx* c, void* dst, const void* src, size_t bytes, bool add_f32);
 
 // Exclusive scan of LineMeta::len into LineMeta::off over n lines (on the context stream).
 int launch_line_scan(cccs_ctx* c, LineMeta* meta, uint64_t n);

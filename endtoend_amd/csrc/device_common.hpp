@@ -21,6 +21,12 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
 }
 
 // Sums one uint32 per thread over the workgroup (LDS tree, <= 256 threads) into part[slot].
+// Sums one uint32 per lane over a 64-lane workgroup into part[slot] (no LDS).
+__device__ __forceinline__ void wave_count(uint32_t v, uint32_t* __restrict__ part, size_t slot) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x == 0) part[slot] = v;
+}
+
 __device__ __forceinline__ void block_count(uint32_t v, uint32_t* __restrict__ part, size_t slot) {
     __shared__ uint32_t s_cnt[256];
     const int t = threadIdx.x + threadIdx.y * blockDim.x + threadIdx.z * blockDim.x * blockDim.y;

@@ -68,6 +68,9 @@ struct cccs_server {
     size_t nvox = 0, nbbox = 0;
     std::vector<double> latencies;
     uint64_t batches = 0, batched = 0;
+    // full-volume result buffers returned by cccs_server_wait, reused so that a request does not
+    // pay for first-touch page faults of a fresh 4 * nvox byte allocation
+    std::vector<std::vector<float>> dose_pool;
 };
 
 namespace {
@@ -122,6 +125,13 @@ int beam_dose_request(cccs_server* s, cccs_ctx* ctx, SlotBuffers& sb, Request& r
     CCCS_TRY(cccs_dev_upload(ctx, sb.fl, r.fluence.data(), fb));
     CCCS_TRY(cccs_dev_zero(ctx, sb.dose, db));
     CCCS_TRY(cccs_beam_dose_device(ctx, &r.beam, 1, static_cast<const float*>(sb.fl), static_cast<float*>(sb.dose)));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->dose_pool.empty()) {
+            r.dose = std::move(s->dose_pool.back());
+            s->dose_pool.pop_back();
+        }
+    }
     r.dose.resize(s->nvox);
     return cccs_dev_download(ctx, r.dose.data(), sb.dose, db);
 }
@@ -292,6 +302,8 @@ extern "C" int cccs_server_wait(cccs_server* s, uint64_t id, float* dose, cccs_s
     if (r->status != CCCS_OK) return set_error(r->status, r->error);
     if (r->kind == CCCS_REQ_BEAM_DOSE) {
         if (dose) std::memcpy(dose, r->dose.data(), sizeof(float) * r->dose.size());
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->dose_pool.size() < 2 * s->slots.size() + 2) s->dose_pool.push_back(std::move(r->dose));
         return CCCS_OK;
     }
     if (sink) {

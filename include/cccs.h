@@ -133,7 +133,11 @@ int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scr
  * rounding).  Keys:
  *   "cone_ramp_min"  full-beam cone lines with at least this many valid samples use the
  *                    segment-sum walk, O(nradii) per sample, instead of the sequential walk,
- *                    O(line) per sample (default 64; a value above the longest line disables it). */
+ *                    O(line) per sample (default 64; a value above the longest line disables it).
+ *   "cone_class_len0".."cone_class_len2"
+ *                    upper stored-line lengths of the first three cone-kernel launch classes
+ *                    (defaults 64, 160, 320; the last class takes the longer lines).  Each class
+ *                    launch sizes its LDS for its longest line. */
 int cccs_ctx_set_schedule(cccs_ctx* ctx, const char* key, int value);
 
 /* ---- full-beam dose: replaces radconvolveTexture (dosecalc-beam) + cuda_sum_device_float_dose.
