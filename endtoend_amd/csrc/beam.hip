@@ -426,12 +426,38 @@ __global__ void __launch_bounds__(256) line_interval_kernel(ConeArgs a) {
 // resampled (density and TERMA, trilinear) into LDS with the exact per-sample test of
 // cudaRaytrace; then every valid sample runs both collapsed-cone walks from LDS and the interval's
 // cone dose is written to the compact REV store.
-__global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
+// Workgroup-wide exclusive prefix sum of one double per thread (NT/64 waves; s_tot holds NT/64
+// doubles).  Contains barriers: every thread of the workgroup must call it.
+template <int NT>
+__device__ __forceinline__ double block_excl_sum(double v, double* s_tot) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_up(x, o, 64);
+        if (lane >= o) x += u;
+    }
+    double e = __shfl_up(x, 1, 64);
+    e = lane == 0 ? 0.0 : e;
+    if (NT == 64) return e;
+    if (lane == 63) s_tot[wid] = x;
+    __syncthreads();
+    double base = 0.0;
+    for (int w = 0; w < wid; w++) base += s_tot[w];
+    __syncthreads();
+    return base + e;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
+    constexpr int NW = NT / 64;
     extern __shared__ double s_dyn[];
     __shared__ float sR[kMaxRadii], sIW[kMaxRadii], sF[kMaxRadii], sB[kMaxRadii];
     __shared__ double sSig[2][kMaxRadii], sBeta[2][kMaxRadii];
+    __shared__ double s_tot[NW];
+    __shared__ int s_red[NW][3];
+    __shared__ uint32_t s_cnt[NW];
     const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const LineMeta meta = a.lines[line];
     const bool mine = meta.len >= a.len_lo && meta.len <= a.len_hi;
     uint32_t iters = 0;
@@ -460,7 +486,7 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
         const int tZ = (int)(l / (uint64_t)c.ny);
         const int i0 = meta.i0;
         float* out = a.rev + meta.off;
-        for (int j = tid; j < m; j += 64) {
+        for (int j = tid; j < m; j += NT) {
             const int i = i0 + j;
             float2 e = make_float2(-1.f, 0.f);
             v3 bev = c.kinv(mk3(c.rs.x + (float)tZ, c.rs.y + (float)i, c.rs.z + (float)tY));
@@ -479,7 +505,7 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
         __syncthreads();
         // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
         int vlo = m, vhi = -1, nvalid = 0;
-        for (int j = tid; j < m; j += 64)
+        for (int j = tid; j < m; j += NT)
             if (s_line[j].x >= 0.f) {
                 vlo = min(vlo, j);
                 vhi = max(vhi, j);
@@ -490,8 +516,24 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
             vhi = max(vhi, __shfl_xor(vhi, o, 64));
             nvalid += __shfl_xor(nvalid, o, 64);
         }
+        if (NW > 1) {
+            if (lane == 0) {
+                s_red[wid][0] = vlo;
+                s_red[wid][1] = vhi;
+                s_red[wid][2] = nvalid;
+            }
+            __syncthreads();
+            vlo = s_red[0][0];
+            vhi = s_red[0][1];
+            nvalid = s_red[0][2];
+            for (int w = 1; w < NW; w++) {
+                vlo = min(vlo, s_red[w][0]);
+                vhi = max(vhi, s_red[w][1]);
+                nvalid += s_red[w][2];
+            }
+        }
         const float nphi_f = (float)a.nphi;
-        for (int j = tid; j < m; j += 64)
+        for (int j = tid; j < m; j += NT)
             if (j < vlo || j > vhi) out[j] = 0.f;
         const int n = vhi - vlo + 1;
         if (n >= a.ramp_min && nvalid == n) {
@@ -505,12 +547,12 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
                 sBeta[d][q] = (double)prev - (double)sg * (q == 0 ? 0.0 : (double)sR[q - 1]);
             }
             const float2* sv = s_line + vlo;
-            // radiological centres (FP64 scan) and the two FP64 prefix sums, chunk per lane
-            const int per = (n + 63) >> 6;
+            // radiological centres (FP64 scan) and the two FP64 prefix sums, one chunk per thread
+            const int per = (n + NT - 1) / NT;
             const int ca = min(n, tid * per), cb = min(n, ca + per);
             double loc = 0.0;
             for (int k = ca; k < cb; k++) loc += 2.0 * (double)sv[k].x;
-            double P = wave_excl_sum(loc);
+            double P = block_excl_sum<NT>(loc, s_tot);
             double le = 0.0, le2 = 0.0;
             for (int k = ca; k < cb; k++) {
                 const float2 e = sv[k];
@@ -521,7 +563,8 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
                 le += ((double)e.y - tp) * (double)ck;
                 le2 += ((double)e.y - tn) * (double)ck;
             }
-            double e1 = wave_excl_sum(le), e2 = wave_excl_sum(le2);
+            double e1 = block_excl_sum<NT>(le, s_tot);
+            double e2 = block_excl_sum<NT>(le2, s_tot);
             for (int k = ca; k < cb; k++) {
                 const double ck = (double)rc[k], t = (double)sv[k].y;
                 const double tp = k > 0 ? (double)sv[k - 1].y : 0.0, tn = k + 1 < n ? (double)sv[k + 1].y : 0.0;
@@ -550,7 +593,7 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
             const float range = L.cn - L.c0;
             const float D = range > 0.f ? range / (float)n : 0.f;
             L.invD = range > 0.f ? (float)n / range : 0.f;
-            for (int u = tid; u <= n + 1; u += 64) {
+            for (int u = tid; u <= n + 1; u += NT) {
                 const float t = L.c0 + (float)u * D;
                 int lo = 0, hi = n;
                 while (lo < hi) {
@@ -560,13 +603,13 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
                 rlb[u] = (uint16_t)lo;
             }
             __syncthreads();
-            for (int j = tid; j < n; j += 64) {
+            for (int j = tid; j < n; j += NT) {
                 const double v = ramp_back(L, j, sSig[0], sBeta[0], sR, nr, iters) +
                                  ramp_fwd(L, j, sSig[1], sBeta[1], sR, nr, iters);
                 out[vlo + j] = (float)v / nphi_f;
             }
         } else {
-            for (int base = vlo; base <= vhi; base += 64) {
+            for (int base = vlo; base <= vhi; base += NT) {
                 const int j = base + tid;
                 const bool act = j <= vhi && s_line[min(j, vhi)].x >= 0.f;
                 const float v = walk_both(s_line, min(j, vhi), vlo, vhi, sR, sIW, sF, sB, nr, act, iters);
@@ -575,8 +618,23 @@ __global__ void __launch_bounds__(64) cone_kernel(ConeArgs a) {
         }
     }
     // every line's slot is written by exactly one of the length-class launches
-    if (a.counts && (mine || (meta.len == 0 && a.len_lo == 1))) wave_count(iters, a.counts, blockIdx.x);
+    if (a.counts && (mine || (meta.len == 0 && a.len_lo == 1))) {
+        for (int o = 32; o > 0; o >>= 1) iters += __shfl_xor(iters, o, 64);
+        if (NW == 1) {
+            if (tid == 0) a.counts[blockIdx.x] = iters;
+        } else {
+            if (lane == 0) s_cnt[wid] = iters;
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t t = 0;
+                for (int w = 0; w < NW; w++) t += s_cnt[w];
+                a.counts[blockIdx.x] = t;
+            }
+        }
+    }
 }
+
+constexpr int kPvcsGroup = 4;
 
 struct PvcsArgs {
     const ConeDev* cones;
@@ -612,12 +670,62 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
     if (!(bv.x >= a.bev_lo.x && bv.y >= a.bev_lo.y && bv.z >= a.bev_lo.z && bv.x < a.bev_hi.x && bv.y < a.bev_hi.y &&
           bv.z < a.bev_hi.z))
         return;
+    // Cones in groups of kPvcsGroup: the line-table loads of a group are issued together, then its
+    // REV loads, so each lane keeps several dependent gathers in flight.  The arithmetic per cone is
+    // rev_fetch's and the sum runs in cone order.
     float sum = 0.f;
-    for (int q = 0; q < a.ncones; q++) {
-        const ConeDev& c = sc[q];
-        const v3 r = c.a(g) + c.t;
-        if (r.y >= (float)c.nx || r.z >= (float)c.ny || r.x >= (float)c.nz || r.x < 0.f || r.y < 0.f || r.z < 0.f) continue;
-        sum += rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, r.y + 0.5f, r.z + 0.5f, r.x + 0.5f);
+    for (int q0 = 0; q0 < a.ncones; q0 += kPvcsGroup) {
+        LineMeta mt[kPvcsGroup][4];
+        float fa[kPvcsGroup], fb[kPvcsGroup], fc[kPvcsGroup];
+        int di[kPvcsGroup];
+        bool ok[kPvcsGroup];
+#pragma unroll
+        for (int u = 0; u < kPvcsGroup; u++) {
+            ok[u] = false;
+            di[u] = 0;
+            fa[u] = fb[u] = fc[u] = 0.f;
+#pragma unroll
+            for (int t = 0; t < 4; t++) mt[u][t] = LineMeta{0u, 0, 0};
+            const int q = q0 + u;
+            if (q >= a.ncones) continue;
+            const ConeDev& c = sc[q];
+            const v3 r = c.a(g) + c.t;
+            if (r.y >= (float)c.nx || r.z >= (float)c.ny || r.x >= (float)c.nz || r.x < 0.f || r.y < 0.f || r.z < 0.f)
+                continue;
+            const float ub = (r.y + 0.5f) - 0.5f, vb = (r.z + 0.5f) - 0.5f, wb = (r.x + 0.5f) - 0.5f;
+            if (!(ub > -1.f && vb > -1.f && wb > -1.f && ub < (float)c.nx && vb < (float)c.ny && wb < (float)c.nz))
+                continue;
+            const float fu = floorf(ub), fv = floorf(vb), fw = floorf(wb);
+            const int i = (int)fu, j = (int)fv, k = (int)fw;
+            ok[u] = true;
+            di[u] = i;
+            fa[u] = ub - fu;
+            fb[u] = vb - fv;
+            fc[u] = wb - fw;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int jj = j + (t & 1), kk = k + (t >> 1);
+                if (jj >= 0 && jj < c.ny && kk >= 0 && kk < c.nz)
+                    mt[u][t] = a.lines[c.line0 + (uint64_t)jj + (uint64_t)c.ny * kk];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPvcsGroup; u++) {
+            if (!ok[u]) continue;
+            float tv[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const LineMeta m = mt[u][t];
+                const int d = di[u] - m.i0;
+                float lo = 0.f, hi = 0.f;
+                if (d >= 0 && d < m.len) lo = a.rev[m.off + d];
+                if (d + 1 >= 0 && d + 1 < m.len) hi = a.rev[m.off + d + 1];
+                tv[t] = lo + fa[u] * (hi - lo);
+            }
+            const float e0 = tv[0] + fb[u] * (tv[1] - tv[0]);
+            const float e1 = tv[2] + fb[u] * (tv[3] - tv[2]);
+            sum += e0 + fc[u] * (e1 - e0);
+        }
     }
     if (sum != 0.f) a.dose[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] += sum;
 }
@@ -886,7 +994,12 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
         ca.len_hi = ci == kConeClasses - 1 ? (int)maxn : hi;
         ca.maxn = ca.len_hi;
         const size_t lds = cone_lds_bytes((uint32_t)ca.maxn, ca.ramp_min <= ca.maxn);
-        hipLaunchKernelGGL(cone_kernel, dim3((unsigned)g.total_lines), dim3(64), lds, ctx->stream, ca);
+        const dim3 grid((unsigned)g.total_lines);
+        switch (ctx->sched.cone_class_threads[ci]) {
+            case 256: hipLaunchKernelGGL(cone_kernel<256>, grid, dim3(256), lds, ctx->stream, ca); break;
+            case 128: hipLaunchKernelGGL(cone_kernel<128>, grid, dim3(128), lds, ctx->stream, ca); break;
+            default: hipLaunchKernelGGL(cone_kernel<64>, grid, dim3(64), lds, ctx->stream, ca); break;
+        }
         CCCS_HIP(hipGetLastError());
         lo = ca.len_hi + 1;
     }

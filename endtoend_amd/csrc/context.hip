@@ -335,6 +335,14 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
         c->sched.cone_ramp_min = value;
         return CCCS_OK;
     }
+    for (int i = 0; i < kConeClasses; i++) {
+        const std::string k = "cone_class_threads" + std::to_string(i);
+        if (k == key) {
+            if (value != 64 && value != 128 && value != 256) return set_error(CCCS_EINVAL, "threads must be 64, 128 or 256");
+            c->sched.cone_class_threads[i] = value;
+            return CCCS_OK;
+        }
+    }
     for (int i = 0; i < kConeClasses - 1; i++) {
         const std::string k = "cone_class_len" + std::to_string(i);
         if (k == key) {

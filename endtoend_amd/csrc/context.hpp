@@ -100,6 +100,8 @@ struct Schedule {
     int cone_ramp_min = 64;
     // upper stored-line lengths of the cone-kernel launch classes (the last class takes the rest)
     int cone_class_len[kConeClasses] = {64, 160, 320, kMaxLine};
+    // threads per line (64, 128 or 256) of each class: long lines spread over more waves
+    int cone_class_threads[kConeClasses] = {64, 128, 128, 128};
 };
 }  // namespace cccs
 

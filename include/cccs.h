@@ -137,7 +137,10 @@ int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scr
  *   "cone_class_len0".."cone_class_len2"
  *                    upper stored-line lengths of the first three cone-kernel launch classes
  *                    (defaults 64, 160, 320; the last class takes the longer lines).  Each class
- *                    launch sizes its LDS for its longest line. */
+ *                    launch sizes its LDS for its longest line.
+ *   "cone_class_threads0".."cone_class_threads3"
+ *                    threads per cone line in each class: 64, 128 or 256 (defaults 64, 128, 128,
+ *                    128). */
 int cccs_ctx_set_schedule(cccs_ctx* ctx, const char* key, int value);
 
 /* ---- full-beam dose: replaces radconvolveTexture (dosecalc-beam) + cuda_sum_device_float_dose.
