@@ -202,8 +202,9 @@ int cccs_beamlet_footprint(const cccs_ctx* ctx, const cccs_beam* beam, const flo
 /* ---- serving: a request loop over device-resident contexts ----------------------------------
  * For the calls a beam-orientation optimiser issues while annealing (single-beam dose, or one
  * beam's beamlet columns).  `slots` contexts (own stream each) run concurrently; a free slot
- * takes every queued request of the head request's kind that fits (continuous batching: requests
- * that arrive while a batch runs join the next one); a request is admitted only while the bytes
+ * takes every queued beamlet request that fits and runs them as one multi-beam job (continuous
+ * batching: requests that arrive while a batch runs join the next one), and full-beam requests
+ * one at a time (each fills the device alone); a request is admitted only while the bytes
  * in flight plus its estimate stay within hbm_budget (a request above the whole budget runs
  * alone).  Results are the same as the direct calls (cccs_beam_dose / cccs_beamlet_dose) made
  * alone.  No counterpart in the reference (its dose calculation is one batch program). */
