@@ -98,6 +98,9 @@ class SparseColumn(C.Structure):
         ("values", _fp),
         ("perc_nonzero", C.c_float),
         ("perc_dropped", C.c_float),
+        ("nblocks", C.c_int),
+        ("row_block_sizes", C.POINTER(C.c_uint64)),
+        ("perc_reducedrop", C.c_float),
     ]
 
 
@@ -134,6 +137,12 @@ EXPORTS = [
     ("cccs_ctx_destroy", C.c_int, [_P]),
     ("cccs_ctx_memory", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("cccs_ctx_set_schedule", C.c_int, [_P, C.c_char_p, C.c_int]),
+    ("cccs_fluence_projection", C.c_int, [_P, C.c_void_p, C.POINTER(Beam), C.c_int, _fp]),
+    ("cccs_set_roi_masks", C.c_int, [_P, C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("cccs_fmap_aperture_fill", C.c_int, [_fp, C.c_int, C.c_int, C.c_float]),
+    ("cccs_fmap_is_aperture_ready", C.c_int, [_fp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    ("cccs_write_omni_beam_list", C.c_int, [C.c_char_p, C.POINTER(Beam), C.c_int, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
     ("cccs_beam_dose", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.POINTER(_fp), _fp]),
     ("cccs_beam_dose_device", C.c_int, [_P, C.POINTER(Beam), C.c_int, _P, _P]),
     ("cccs_beam_terma", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp]),

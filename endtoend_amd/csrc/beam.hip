@@ -745,6 +745,54 @@ __global__ void __launch_bounds__(256) pack_dt_kernel(const float* __restrict__ 
     out[v] = make_float2(dens[v], t);
 }
 
+// ---- structure projection onto fluence maps (preprocess: findFluenceProjection + cudaRaycast,
+// dosecalc-preprocess/cudaMemFunctions.cu:308-393, kernels.cuh:136-224) ---------------------------
+struct ProjBeam {
+    v3 src, iso;
+    Rot2 inv;         // FCS -> PVCS (inverseRotateBeamAtOriginRHS)
+    int fx, fy;       // fluence map size
+    float bsx, bsy;   // beamlet size [cm]
+    uint32_t off;     // first beamlet of this beam in the output
+};
+
+// One lane per beamlet of any beam.  Nine sub-rays (offsets -1/0/+1 half beamlets along each
+// fluence axis) run from the source to three times the source-to-cell distance through the
+// trilinearly sampled mask with the TERMA path's Siddon march; the beamlet is 1 when any sub-ray
+// accumulates a positive path integral.
+__global__ void __launch_bounds__(64) projection_kernel(const float* __restrict__ mask, VolDesc vd,
+                                                        const ProjBeam* __restrict__ beams, int nbeams, uint32_t total,
+                                                        float* __restrict__ out) {
+    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= total) return;
+    int lo = 0, hi = nbeams - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (beams[mid].off <= t) lo = mid; else hi = mid - 1;
+    }
+    const ProjBeam& b = beams[lo];
+    const int pix = (int)(t - b.off);
+    const int rx = pix % b.fx, rz = pix / b.fx;
+    bool hit = false;
+    for (int sr = 0; sr < 9 && !hit; sr++) {
+        const int sx = sr % 3 - 1, sz = sr / 3 - 1;
+        const v3 fcs = mk3(b.bsx * ((float)rx + 0.5f * (float)sx - (float)b.fx * 0.5f + 0.5f), 0.f,
+                           b.bsy * ((float)rz + 0.5f * (float)sz - (float)b.fy * 0.5f + 0.5f));
+        v3 ctr = rot2(b.inv, fcs) + b.iso;
+        ctr = ctr + 1e-9f;
+        const v3 sd = ctr - b.src;
+        const v3 sink = b.src + 3.f * sd;
+        const float rpl = siddon_march(sink, vd.start, vd.end, vd.voxel, b.src, mask, vd.nx, vd.ny, vd.nz,
+                                       [](float, float, float, float) {}, nullptr);
+        hit = rpl > 0.f;
+    }
+    out[t] = hit ? 1.f : 0.f;
+}
+
+__global__ void mask_to_float_kernel(const uint8_t* __restrict__ m, float* __restrict__ f, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        f[i] = (float)m[i];
+}
+
 __global__ void add_kernel(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] += src[i];
 }
@@ -1138,6 +1186,59 @@ extern "C" int cccs_beam_geometry_of(cccs_ctx* ctx, const cccs_beam* beam, const
         geom->rev_min[r][0] = g.rev_min[r].x; geom->rev_min[r][1] = g.rev_min[r].y; geom->rev_min[r][2] = g.rev_min[r].z;
         for (int k = 0; k < 3; k++) geom->rev_size[r][k] = g.rev_size[r][k];
     }
+    return CCCS_OK;
+}
+
+extern "C" int cccs_fluence_projection(cccs_ctx* ctx, const uint8_t* mask, const cccs_beam* beams, int nbeams,
+                                       float* fluence) {
+    clear_error();
+    if (!ctx || !mask || (!beams && nbeams > 0) || nbeams < 0 || (!fluence && nbeams > 0))
+        return set_error(CCCS_EINVAL, "null argument");
+    if (nbeams == 0) return CCCS_OK;
+    CCCS_HIP(hipSetDevice(ctx->device));
+    std::vector<ProjBeam> pb((size_t)nbeams);
+    uint64_t total = 0;
+    for (int i = 0; i < nbeams; i++) {
+        const cccs_beam& b = beams[i];
+        if (b.fmap_size[0] == 0 || b.fmap_size[1] == 0) return set_error(CCCS_EINVAL, "empty fluence map");
+        if (!(b.beamlet_size[0] > 0.f) || !(b.beamlet_size[1] > 0.f)) return set_error(CCCS_EINVAL, "beamlet size");
+        ProjBeam& o = pb[(size_t)i];
+        o.src = ld3(b.source);
+        o.iso = ld3(b.isocenter);
+        o.inv = rot2_beam_inv(b.azimuth, b.zenith, b.coll);
+        o.fx = (int)b.fmap_size[0];
+        o.fy = (int)b.fmap_size[1];
+        o.bsx = b.beamlet_size[0];
+        o.bsy = b.beamlet_size[1];
+        o.off = (uint32_t)total;
+        total += (uint64_t)b.fmap_size[0] * b.fmap_size[1];
+        if (total > 0x7fffffffULL) return set_error(CCCS_EINVAL, "too many beamlets");
+    }
+    const size_t nv = vol_voxels(ctx);
+    // scratch: the mask (bytes, then float), the beam table and the maps; released on every return
+    DevBuf mb, mf, tb, ob;
+    struct Release {
+        DevBuf* b[4];
+        ~Release() {
+            for (DevBuf* x : b) x->release();
+        }
+    } release{{&mb, &mf, &tb, &ob}};
+    CCCS_TRY(mb.ensure(nv));
+    CCCS_TRY(mf.ensure(nv * sizeof(float)));
+    CCCS_TRY(tb.ensure(sizeof(ProjBeam) * pb.size()));
+    CCCS_TRY(ob.ensure(sizeof(float) * total));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(mb.p, mask, nv, hipMemcpyHostToDevice));
+    CCCS_HIP(hipMemcpy(tb.p, pb.data(), sizeof(ProjBeam) * pb.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mask_to_float_kernel, dim3(1024), dim3(256), 0, ctx->stream, (const uint8_t*)mb.p,
+                       mf.as<float>(), nv);
+    CCCS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(projection_kernel, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, ctx->stream,
+                       (const float*)mf.as<float>(), ctx->vol, (const ProjBeam*)tb.as<ProjBeam>(), nbeams,
+                       (uint32_t)total, ob.as<float>());
+    CCCS_HIP(hipGetLastError());
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(fluence, ob.p, sizeof(float) * total, hipMemcpyDeviceToHost));
     return CCCS_OK;
 }
 

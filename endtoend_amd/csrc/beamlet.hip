@@ -1562,7 +1562,7 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
         const int set = (g0 / G) % 2;
         for (int q = g0; q < std::min(nbl, g0 + G); q++) {
             const int sl = set * G + (q - g0);
-            cccs_sparse_column col;
+            cccs_sparse_column col{};
             col.beam_index = plans[R.plan].beam;
             col.beamlet_index = J.beamlets[q].fidx;
             col.nnz = keys[sl].size();
@@ -1691,6 +1691,178 @@ std::vector<BatchRef> all_batches(const std::vector<BeamPlan>& plans) {
 
 }  // namespace
 
+namespace {
+
+// ---- ROI reduction (_reduce_by_roi, DoseCalcIO/sparse_io.cpp:62-158) ------------------------------
+constexpr int kMaxRoi = 64;
+
+struct RoiDev {
+    int c0[3], cs[3];    // crop box start and size on the CT grid
+    uint64_t rank_off;   // first rank entry (crop box, x fastest; -1 outside the mask)
+    uint64_t mark;       // voxels of all earlier ROIs (first key of this block)
+};
+
+struct RoiArgs {
+    const RoiDev* rois;
+    const int32_t* rank;
+    int nroi;
+    int nx, ny;                  // CT grid
+    const uint64_t* keys;        // the job's kept entries, column after column
+    const float* vals;
+    const uint64_t* off;         // first entry of each column (nbl + 1)
+    uint32_t* cnt;               // [column][roi] reduced counts
+    const uint64_t* roff;        // [column][roi] first reduced entry (exclusive scan of cnt)
+    uint64_t* rkeys;
+    float* rvals;
+};
+
+// ROI rank of CT voxel `key` in ROI r (or -1).
+__device__ __forceinline__ int32_t roi_rank(const RoiArgs& a, const RoiDev& R, int x, int y, int z) {
+    const int lx = x - R.c0[0], ly = y - R.c0[1], lz = z - R.c0[2];
+    if (lx < 0 || ly < 0 || lz < 0 || lx >= R.cs[0] || ly >= R.cs[1] || lz >= R.cs[2]) return -1;
+    return a.rank[R.rank_off + (uint64_t)lx + (uint64_t)R.cs[0] * ((uint64_t)ly + (uint64_t)R.cs[1] * lz)];
+}
+
+// Pass 1: per column and ROI, the number of kept entries inside the ROI (one LDS add per wave,
+// chunk and ROI).
+__global__ void __launch_bounds__(256) roi_count_kernel(RoiArgs a) {
+    __shared__ uint32_t s_c[kMaxRoi];
+    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    for (int r = tid; r < a.nroi; r += 256) s_c[r] = 0u;
+    __syncthreads();
+    const uint64_t e0 = a.off[q], e1 = a.off[q + 1];
+    const uint64_t plane = (uint64_t)a.nx * (uint64_t)a.ny;
+    for (uint64_t base = e0; base < e1; base += 256) {
+        const uint64_t e = base + tid;
+        int x = -1, y = -1, z = -1;
+        if (e < e1) {
+            const uint64_t key = a.keys[e];
+            z = (int)(key / plane);
+            const int rem = (int)(key - (uint64_t)z * plane);
+            y = rem / a.nx;
+            x = rem - y * a.nx;
+        }
+        for (int r = 0; r < a.nroi; r++) {
+            const uint64_t m = __ballot(e < e1 && roi_rank(a, a.rois[r], x, y, z) >= 0);
+            if (lane == 0 && m) atomicAdd(&s_c[r], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    for (int r = tid; r < a.nroi; r += 256) a.cnt[(size_t)q * a.nroi + r] = s_c[r];
+}
+
+// Pass 2: ROI by ROI, the column's entries inside the ROI in the column's order (an
+// order-preserving compaction per 256-entry chunk), keyed mark + rank.
+__global__ void __launch_bounds__(256) roi_emit_kernel(RoiArgs a) {
+    __shared__ uint32_t s_w[4];
+    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t e0 = a.off[q], e1 = a.off[q + 1];
+    const uint64_t plane = (uint64_t)a.nx * (uint64_t)a.ny;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int r = 0; r < a.nroi; r++) {
+        const RoiDev R = a.rois[r];
+        uint64_t pos = a.roff[(size_t)q * a.nroi + r];
+        for (uint64_t base = e0; base < e1; base += 256) {
+            const uint64_t e = base + tid;
+            int32_t rk = -1;
+            if (e < e1) {
+                const uint64_t key = a.keys[e];
+                const int z = (int)(key / plane), rem = (int)(key - (uint64_t)z * plane);
+                const int y = rem / a.nx, x = rem - y * a.nx;
+                rk = roi_rank(a, R, x, y, z);
+            }
+            const uint64_t m = __ballot(rk >= 0);
+            if (lane == 0) s_w[wid] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (int w = 0; w < 4; w++) {
+                before += w < wid ? s_w[w] : 0u;
+                total += s_w[w];
+            }
+            if (rk >= 0) {
+                const uint64_t o = pos + before + (uint32_t)__popcll(m & lt);
+                a.rkeys[o] = R.mark + (uint64_t)rk;
+                a.rvals[o] = a.vals[e];
+            }
+            pos += total;
+            __syncthreads();
+        }
+    }
+}
+
+// Exclusive scan of n uint32 counts into n+1 uint64 offsets (one workgroup, serial chunks).
+__global__ void __launch_bounds__(1024) roi_scan_kernel(const uint32_t* __restrict__ cnt, size_t n,
+                                                        uint64_t* __restrict__ out) {
+    __shared__ uint64_t s_p[1024];
+    const int tid = threadIdx.x;
+    const size_t per = (n + 1023) / 1024;
+    const size_t c0 = std::min(n, (size_t)tid * per), c1 = std::min(n, c0 + per);
+    uint64_t loc = 0;
+    for (size_t i = c0; i < c1; i++) loc += cnt[i];
+    s_p[tid] = loc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint64_t v = tid >= o ? s_p[tid - o] : 0;
+        __syncthreads();
+        s_p[tid] += v;
+        __syncthreads();
+    }
+    uint64_t run = s_p[tid] - loc;
+    for (size_t i = c0; i < c1; i++) {
+        out[i] = run;
+        run += cnt[i];
+    }
+    if (tid == 1023) out[n] = s_p[1023];
+}
+
+}  // namespace
+
+extern "C" int cccs_set_roi_masks(cccs_ctx* ctx, int nroi, const uint8_t* const* masks, const uint32_t* crop_start,
+                                  const uint32_t* crop_size) {
+    clear_error();
+    if (!ctx || nroi < 0 || (nroi > 0 && (!masks || !crop_start || !crop_size))) return set_error(CCCS_EINVAL, "null argument");
+    if (nroi > kMaxRoi) return set_error(CCCS_EINVAL, "at most " + std::to_string(kMaxRoi) + " ROIs");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    ctx->nroi = 0;
+    if (nroi == 0) return CCCS_OK;
+    const cccs_problem& p = ctx->prob;
+    std::vector<RoiDev> tab((size_t)nroi);
+    std::vector<int32_t> rank;
+    uint64_t mark = 0;
+    for (int r = 0; r < nroi; r++) {
+        RoiDev& R = tab[(size_t)r];
+        for (int k = 0; k < 3; k++) {
+            R.c0[k] = (int)crop_start[3 * r + k];
+            R.cs[k] = (int)crop_size[3 * r + k];
+            if (crop_start[3 * r + k] + (uint64_t)crop_size[3 * r + k] > p.size[k])
+                return set_error(CCCS_EINVAL, "ROI crop box outside the CT grid");  // sparse_io.cpp:111-113
+        }
+        if (!masks[r]) return set_error(CCCS_EINVAL, "null ROI mask");
+        R.rank_off = rank.size();
+        R.mark = mark;
+        const size_t n = (size_t)R.cs[0] * R.cs[1] * R.cs[2];
+        rank.resize(rank.size() + n, -1);
+        int32_t* rk = rank.data() + R.rank_off;
+        int32_t elem = 0;
+        // the reference's iteration: ii (x) outer, jj (y), kk (z) inner over the crop box
+        for (int ii = 0; ii < R.cs[0]; ii++)
+            for (int jj = 0; jj < R.cs[1]; jj++)
+                for (int kk = 0; kk < R.cs[2]; kk++) {
+                    const size_t ck = ((size_t)kk * R.cs[1] + jj) * R.cs[0] + ii;
+                    if (masks[r][ck]) rk[ck] = elem++;
+                }
+        mark += (uint64_t)elem;
+    }
+    CCCS_TRY(ctx->roi_tab.ensure(sizeof(RoiDev) * tab.size()));
+    CCCS_TRY(ctx->roi_rank.ensure(sizeof(int32_t) * std::max<size_t>(1, rank.size())));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(ctx->roi_tab.p, tab.data(), sizeof(RoiDev) * tab.size(), hipMemcpyHostToDevice));
+    if (!rank.empty())
+        CCCS_HIP(hipMemcpy(ctx->roi_rank.p, rank.data(), sizeof(int32_t) * rank.size(), hipMemcpyHostToDevice));
+    ctx->nroi = nroi;
+    return CCCS_OK;
+}
+
 extern "C" int cccs_beamlet_footprint(const cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, uint64_t* bytes) {
     clear_error();
     if (!ctx || !beam || !fluence || !bytes) return set_error(CCCS_EINVAL, "null argument");
@@ -1775,6 +1947,8 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || (!fluence && nbeams > 0))
         return set_error(CCCS_EINVAL, "null argument");
     if (!sink && mode == CCCS_BEAMLET_PLAIN) return set_error(CCCS_EINVAL, "the plain path needs a sink");
+    if (ctx && ctx->nroi > 0 && mode == CCCS_BEAMLET_PLAIN)
+        return set_error(CCCS_EINVAL, "ROI reduction runs in the batched modes");
     if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST && mode != CCCS_BEAMLET_PLAIN)
         return set_error(CCCS_EINVAL, "unknown beamlet mode");
     if (!(threshold >= 0.f)) return set_error(CCCS_EINVAL, "threshold must be >= 0");
@@ -1875,37 +2049,91 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         // algorithmic traffic: the kept entries (8 B key + 4 B value)
         prof_end(ctx, tk, 12.0 * (double)nnz_total, 0.0, nnz_total);
         CCCS_HIP(hipGetLastError());
+        // ROI reduction of the job's columns (count, scan, order-preserving emit)
+        const int nroi = ctx->nroi;
+        uint64_t rtotal = 0;
+        uint32_t* rcnt = nullptr;
+        if (nroi > 0) {
+            const size_t ncr = (size_t)nbl * (size_t)nroi;
+            const size_t cbytes = (sizeof(uint32_t) * ncr + 255) & ~size_t(255);
+            CCCS_TRY(ctx->roi_cnt.ensure(cbytes + sizeof(uint64_t) * (ncr + 1)));
+            rcnt = ctx->roi_cnt.as<uint32_t>();
+            uint64_t* roff = reinterpret_cast<uint64_t*>(ctx->roi_cnt.as<uint8_t>() + cbytes);
+            RoiArgs ra;
+            ra.rois = ctx->roi_tab.as<RoiDev>();
+            ra.rank = ctx->roi_rank.as<int32_t>();
+            ra.nroi = nroi;
+            ra.nx = (int)p.size[0];
+            ra.ny = (int)p.size[1];
+            ra.keys = ctx->bl_keys.as<uint64_t>();
+            ra.vals = ctx->bl_vals.as<float>();
+            ra.off = offs;
+            ra.cnt = rcnt;
+            ra.roff = roff;
+            ra.rkeys = nullptr;
+            ra.rvals = nullptr;
+            hipLaunchKernelGGL(roi_count_kernel, dim3(nbl), dim3(256), 0, ctx->stream, ra);
+            hipLaunchKernelGGL(roi_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t*)rcnt, ncr, roff);
+            CCCS_HIP(hipGetLastError());
+            CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, roff + ncr, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+            CCCS_HIP(hipStreamSynchronize(ctx->stream));
+            std::memcpy(&rtotal, ctx->h_small.p, sizeof(uint64_t));
+            CCCS_TRY(ctx->roi_keys.ensure(sizeof(uint64_t) * std::max<uint64_t>(1, rtotal)));
+            CCCS_TRY(ctx->roi_vals.ensure(sizeof(float) * std::max<uint64_t>(1, rtotal)));
+            ra.rkeys = ctx->roi_keys.as<uint64_t>();
+            ra.rvals = ctx->roi_vals.as<float>();
+            hipLaunchKernelGGL(roi_emit_kernel, dim3(nbl), dim3(256), 0, ctx->stream, ra);
+            CCCS_HIP(hipGetLastError());
+        }
         if (!sink) continue;  // columns stay on the device (overwritten by the next job)
         // columns to pinned host memory (full PCIe rate), then the sink sees them in place
-        const size_t kbytes = sizeof(uint64_t) * nnz_total, vbytes = sizeof(float) * nnz_total;
+        const uint64_t nout = nroi > 0 ? rtotal : nnz_total;
+        const size_t kbytes = sizeof(uint64_t) * nout, vbytes = sizeof(float) * nout;
         const size_t sbytes = sizeof(uint32_t) * 4 * (size_t)nbl;
+        const size_t rbytes = sizeof(uint32_t) * (size_t)nbl * (size_t)nroi;
         CCCS_HIP(hipStreamSynchronize(ctx->stream));
-        CCCS_TRY(ctx->h_bl_out.ensure(kbytes + vbytes + sbytes + 64));
+        CCCS_TRY(ctx->h_bl_out.ensure(kbytes + vbytes + sbytes + rbytes + 64));
         uint8_t* hb = ctx->h_bl_out.as<uint8_t>();
         uint64_t* hkeys = reinterpret_cast<uint64_t*>(hb);
         float* hvals = reinterpret_cast<float*>(hb + kbytes);
         uint32_t* hst = reinterpret_cast<uint32_t*>(hb + kbytes + vbytes);
+        uint32_t* hrc = hst + 4 * (size_t)nbl;
         CCCS_HIP(hipMemcpyAsync(hst, stat, sbytes, hipMemcpyDeviceToHost, ctx->stream));
-        if (nnz_total) {
-            CCCS_HIP(hipMemcpyAsync(hkeys, ctx->bl_keys.p, kbytes, hipMemcpyDeviceToHost, ctx->stream));
-            CCCS_HIP(hipMemcpyAsync(hvals, ctx->bl_vals.p, vbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (nroi > 0) CCCS_HIP(hipMemcpyAsync(hrc, rcnt, rbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (nout) {
+            CCCS_HIP(hipMemcpyAsync(hkeys, nroi > 0 ? ctx->roi_keys.p : ctx->bl_keys.p, kbytes, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+            CCCS_HIP(hipMemcpyAsync(hvals, nroi > 0 ? ctx->roi_vals.p : ctx->bl_vals.p, vbytes, hipMemcpyDeviceToHost,
+                                    ctx->stream));
         }
         CCCS_HIP(hipStreamSynchronize(ctx->stream));
         std::memcpy(hstat.data(), hst, sbytes);
+        std::vector<uint64_t> blocks((size_t)nroi);
         uint64_t pos = 0;
         for (int q = 0; q < nbl; q++) {
             const BeamletDev& d = J.beamlets[q];
             const BatchRef& R = sel[d.batch];
             const BeamPlan& P = plans[R.plan];
-            cccs_sparse_column col;
+            cccs_sparse_column col{};
             col.beam_index = P.beam;
             col.beamlet_index = d.fidx;
             col.nnz = hstat[4 * q];
-            col.keys = hkeys + pos;
-            col.values = hvals + pos;
             col.perc_nonzero = (float)(100.0 * (double)hstat[4 * q] / (double)nvox);
             col.perc_dropped = (float)(100.0 * (double)hstat[4 * q + 1] / (double)nvox);
-            pos += hstat[4 * q];
+            if (nroi > 0) {
+                // _reduce_by_roi's statistics (sparse_io.cpp:150-156)
+                uint64_t kept = 0;
+                for (int r = 0; r < nroi; r++) kept += blocks[(size_t)r] = hrc[(size_t)q * nroi + r];
+                const double pn = 100.0 * (double)kept / (double)nvox;
+                col.perc_reducedrop = (float)((double)col.perc_nonzero - pn);
+                col.perc_nonzero = (float)pn;
+                col.nnz = kept;
+                col.nblocks = nroi;
+                col.row_block_sizes = blocks.data();
+            }
+            col.keys = hkeys + pos;
+            col.values = hvals + pos;
+            pos += col.nnz;
             if (sink(user, &col) != 0) return set_error(CCCS_EIO, "sparse sink aborted the call");
         }
     }

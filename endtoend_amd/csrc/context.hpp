@@ -126,6 +126,10 @@ struct cccs_ctx {
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
     cccs::Schedule sched;
+    // ROI reduction (cccs_set_roi_masks): per-ROI table, rank volumes over the crop boxes, and the
+    // reduced output of the current job
+    int nroi = 0;
+    cccs::DevBuf roi_tab, roi_rank, roi_cnt, roi_keys, roi_vals;
 };
 
 namespace cccs {

@@ -1,6 +1,8 @@
 // io.cpp -- the reference's input/output file formats for the dose path: beam lists
-// (DoseCalcIO/beam.cpp:369-576), the omni header (DoseCalcIO/calc_header.cpp:12-161), fluence maps
-// (DoseCalcIO/fluence.cpp:19-44) and raw float32 volumes (density in, dose out).  Host code only.
+// (DoseCalcIO/beam.cpp:369-607, reader and writer), the omni header
+// (DoseCalcIO/calc_header.cpp:12-161), fluence maps (DoseCalcIO/fluence.cpp:19-50), raw float32
+// volumes (density in, dose out), and the preprocessor's aperture-ready fill
+// (dosecalc-preprocess/fmapProcessing.cpp:7-46).  Host code only.
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -284,4 +286,80 @@ extern "C" int cccs_write_f32(const char* path, const float* data, uint64_t n) {
     const size_t put = std::fwrite(data, sizeof(float), (size_t)n, o);
     std::fclose(o);
     return put == (size_t)n ? CCCS_OK : set_error(CCCS_EIO, "short write");
+}
+
+extern "C" int cccs_write_omni_beam_list(const char* path, const cccs_beam* beams, int beam_count,
+                                         const char* const* iso_type, const char* const* iso_loc,
+                                         const char* const* orient_type, const char* const* fmap_names) {
+    clear_error();
+    if (!path || (!beams && beam_count > 0) || beam_count < 0) return set_error(CCCS_EINVAL, "null argument");
+    FILE* o = std::fopen(path, "w");
+    if (!o) return set_error(CCCS_EIO, std::string("cannot write ") + path);
+    std::fprintf(o, "%d\n", beam_count);
+    char name[32];
+    for (int b = 0; b < beam_count; b++) {
+        const cccs_beam& B = beams[b];
+        // deg_azimuth() and friends: 180 * angle / PI in FP32 (beam.h:52-54)
+        const float az = 180.0f * B.azimuth / kPiF, zen = 180.0f * B.zenith / kPiF, coll = 180.0f * B.coll / kPiF;
+        std::snprintf(name, sizeof(name), "fmap-%06d.raw", b);  // write_fluence_map's name (fluence.cpp:46-50)
+        std::fprintf(o, "%d %f %f %f %f %s %s %f %f %f %f %f %f %s %f %f %f %f %f %d %d %s\n", b, az, zen, coll, B.sad,
+                     iso_type && iso_type[b] ? iso_type[b] : "unspec", iso_loc && iso_loc[b] ? iso_loc[b] : "unspec",
+                     B.isocenter[0], B.isocenter[1], B.isocenter[2], B.source[0], B.source[1], B.source[2],
+                     orient_type && orient_type[b] ? orient_type[b] : "auto", B.direction[0], B.direction[1],
+                     B.direction[2], B.beamlet_size[0], B.beamlet_size[1], (int)B.fmap_size[0], (int)B.fmap_size[1],
+                     fmap_names && fmap_names[b] ? fmap_names[b] : name);
+    }
+    std::fprintf(o, "\n#beam_id  azimuth zenith coll sad | iso_type iso_loc isocenter.x isocenter.y isocenter.z | "
+                    "source.x source.y source.z | orient_type direction.x direction.y direction.z | beamlet_size.x "
+                    "beamlet_size.y fluence_dim.x fluence_dim.y | fmap_fname\n");
+    const bool bad = std::ferror(o) != 0;
+    std::fclose(o);
+    return bad ? set_error(CCCS_EIO, "write failed") : CCCS_OK;
+}
+
+namespace {
+// get_active_segments (dosecalc-preprocess/fmapProcessing.cpp:7-24): a non-zero beamlet restarts
+// the run at itself, so each run is recorded as {last, last} of its non-zero beamlets.
+struct Run {
+    int first, last;
+};
+void row_runs(const float* f, int w, int yy, std::vector<Run>& out) {
+    out.clear();
+    int begin = -1, end = -1;
+    for (int xx = 0; xx < w; ++xx) {
+        const float v = f[(size_t)w * yy + xx];
+        if (v != 0.f) begin = end = xx;
+        if (begin >= 0) {
+            if (v != 0.f) end = xx;
+            if (v == 0.f || xx >= w - 1) {
+                out.push_back(Run{begin, end});
+                begin = end = -1;
+            }
+        }
+    }
+}
+}  // namespace
+
+extern "C" int cccs_fmap_aperture_fill(float* fluence, int width, int height, float fill_val) {
+    clear_error();
+    if (!fluence || width < 0 || height < 0) return set_error(CCCS_EINVAL, "bad argument");
+    std::vector<Run> runs;
+    for (int yy = 0; yy < height; ++yy) {
+        row_runs(fluence, width, yy, runs);
+        for (int ii = 0; ii + 1 < (int)runs.size(); ++ii)
+            for (int xx = runs[ii].last + 1; xx < runs[ii + 1].first; ++xx) fluence[(size_t)width * yy + xx] = fill_val;
+    }
+    return CCCS_OK;
+}
+
+extern "C" int cccs_fmap_is_aperture_ready(const float* fluence, int width, int height, int* ready) {
+    clear_error();
+    if (!fluence || !ready || width < 0 || height < 0) return set_error(CCCS_EINVAL, "bad argument");
+    std::vector<Run> runs;
+    *ready = 1;
+    for (int yy = 0; yy < height && *ready; ++yy) {
+        row_runs(fluence, width, yy, runs);
+        if (runs.size() > 1) *ready = 0;
+    }
+    return CCCS_OK;
 }

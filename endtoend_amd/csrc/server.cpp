@@ -98,6 +98,8 @@ int collect_column(void* user, const cccs_sparse_column* c) {
     Request& r = *(*k->batch)[k->owner[c->beam_index]];
     cccs_sparse_column o = *c;
     o.beam_index = 0;
+    o.nblocks = 0;  // slot contexts carry no ROI masks
+    o.row_block_sizes = nullptr;
     o.keys = reinterpret_cast<const uint64_t*>((uintptr_t)r.keys.size());  // offset, fixed at delivery
     r.keys.insert(r.keys.end(), c->keys, c->keys + c->nnz);
     r.values.insert(r.values.end(), c->values, c->values + c->nnz);

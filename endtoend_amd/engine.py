@@ -186,6 +186,37 @@ class Engine:
         abi.check(self.lib.cccs_ctx_memory(self.ctx, C.byref(r), C.byref(s)), "cccs_ctx_memory")
         return r.value, s.value
 
+    def fluence_projection(self, mask: np.ndarray, beams) -> list:
+        """Structure mask (z, y, x; non-zero inside) projected onto each beam's fluence map
+        (cccs_fluence_projection); returns one (fy, fx) float map per beam."""
+        m = np.ascontiguousarray(mask, np.uint8)
+        if m.shape != self.spec.density.shape:
+            raise ValueError("mask must have the density's shape")
+        arr = (abi.Beam * len(beams))(*beams)
+        sizes = [(int(b.fmap_size[1]), int(b.fmap_size[0])) for b in beams]
+        out = np.zeros(sum(a * b for a, b in sizes), np.float32)
+        abi.check(self.lib.cccs_fluence_projection(self.ctx, m.ctypes.data, arr, len(beams), abi.fptr(out)),
+                  "cccs_fluence_projection")
+        maps, o = [], 0
+        for fy, fx in sizes:
+            maps.append(out[o:o + fy * fx].reshape(fy, fx).copy())
+            o += fy * fx
+        return maps
+
+    def set_roi_masks(self, rois) -> None:
+        """ROI reduction of later beamlet columns (cccs_set_roi_masks).  rois: list of
+        (mask over the crop box as (z, y, x) bytes, crop_start (x, y, z), crop_size (x, y, z));
+        an empty list turns it off."""
+        n = len(rois)
+        masks = [np.ascontiguousarray(m, np.uint8) for m, _, _ in rois]
+        for m, (_, _, size) in zip(masks, rois):
+            if m.shape != (size[2], size[1], size[0]):
+                raise ValueError("mask shape must be (crop_size z, y, x)")
+        ptrs = (C.c_void_p * max(1, n))(*[m.ctypes.data for m in masks])
+        st = (C.c_uint32 * max(3, 3 * n))(*[int(v) for _, s0, _ in rois for v in s0])
+        sz = (C.c_uint32 * max(3, 3 * n))(*[int(v) for _, _, s1 in rois for v in s1])
+        abi.check(self.lib.cccs_set_roi_masks(self.ctx, n, C.cast(ptrs, C.c_void_p), st, sz), "cccs_set_roi_masks")
+
     def set_schedule(self, key: str, value: int):
         """Launch-schedule knob (cccs_ctx_set_schedule); e.g. "cone_ramp_min"."""
         abi.check(self.lib.cccs_ctx_set_schedule(self.ctx, key.encode(), int(value)), "cccs_ctx_set_schedule")
@@ -308,7 +339,11 @@ class Engine:
             c = colp.contents
             k = np.ctypeslib.as_array(c.keys, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.uint64)
             v = np.ctypeslib.as_array(c.values, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.float32)
-            cols.append((c.beam_index, c.beamlet_index, k, v))
+            if c.nblocks:  # ROI-reduced column: its row-block sizes as a fifth element
+                blocks = np.ctypeslib.as_array(c.row_block_sizes, shape=(c.nblocks,)).copy()
+                cols.append((c.beam_index, c.beamlet_index, k, v, blocks))
+            else:
+                cols.append((c.beam_index, c.beamlet_index, k, v))
             return 0
 
         cb = abi.SPARSE_SINK(sink_fn)
@@ -360,6 +395,38 @@ def load_omni_header(path: str) -> dict:
 def write_omni_header(path: str, p, beam_count: int, beam_spec: str, target: str, reduce: bool = False) -> None:
     abi.check(abi.load().cccs_write_omni_header(path.encode(), C.byref(p), beam_count, beam_spec.encode(),
                                                 target.encode(), int(reduce)), "cccs_write_omni_header")
+
+
+def write_omni_beam_list(path: str, beams, iso_type=None, iso_loc=None, orient_type=None, fmap_names=None) -> None:
+    """omni_beam_list.txt (write_omni_beam_list semantics); string lists default per beam."""
+    n = len(beams)
+    arr = (abi.Beam * max(1, n))(*beams)
+
+    def strs(v):
+        if v is None:
+            return None
+        a = (C.c_char_p * n)(*[x.encode() for x in v])
+        return C.cast(a, C.c_void_p), a
+
+    keep = [strs(v) for v in (iso_type, iso_loc, orient_type, fmap_names)]
+    ptrs = [k[0] if k else None for k in keep]
+    abi.check(abi.load().cccs_write_omni_beam_list(path.encode(), arr, n, *ptrs), "cccs_write_omni_beam_list")
+
+
+def fmap_aperture_fill(fmap: np.ndarray, fill_val: float = 1.0) -> np.ndarray:
+    """Aperture-ready fill (fmap_post_apertureready semantics) of a (fy, fx) map; returns a copy."""
+    f = np.ascontiguousarray(fmap, np.float32).copy()
+    abi.check(abi.load().cccs_fmap_aperture_fill(abi.fptr(f), f.shape[1], f.shape[0], fill_val),
+              "cccs_fmap_aperture_fill")
+    return f
+
+
+def fmap_is_aperture_ready(fmap: np.ndarray) -> bool:
+    f = np.ascontiguousarray(fmap, np.float32)
+    r = C.c_int(0)
+    abi.check(abi.load().cccs_fmap_is_aperture_ready(abi.fptr(f), f.shape[1], f.shape[0], C.byref(r)),
+              "cccs_fmap_is_aperture_ready")
+    return bool(r.value)
 
 
 def read_f32(path: str, shape) -> np.ndarray:

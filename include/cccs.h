@@ -102,6 +102,12 @@ typedef struct cccs_sparse_column {
     const float*    values;
     float     perc_nonzero;
     float     perc_dropped;
+    /* ROI-reduced columns (cccs_set_roi_masks): one block per ROI, row_block_sizes[r] entries in
+     * block r; nblocks = 0 and row_block_sizes = NULL otherwise.  perc_reducedrop is the
+     * percentage of calc_bbox voxels the reduction removed (sparse_io.cpp:154). */
+    int       nblocks;
+    const uint64_t* row_block_sizes;
+    float     perc_reducedrop;
 } cccs_sparse_column;
 
 /* Called once per beamlet column; return 0 to continue, non-zero to abort the call. */
@@ -182,6 +188,16 @@ int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
 
 /* Counting sink for callers that need totals only (throughput runs, dry runs): pass a
  * cccs_sink_totals as `user`; no column data is kept.  No counterpart in the reference. */
+/* ROI reduction of beamlet columns (the `reduce` option of dosecalc-beamlet; _reduce_by_roi,
+ * DoseCalcIO/sparse_io.cpp:62-158).  ROI r is a byte mask over its crop box: crop_size[3r..3r+2]
+ * voxels starting at crop_start[3r..3r+2] on the CT grid, x fastest, non-zero = inside.  Once set,
+ * cccs_beamlet_dose delivers every column of the batched modes reduced: block r holds the
+ * column's kept entries inside ROI r, in the column's order, keyed (voxels of all ROIs before r) +
+ * (rank of the voxel among ROI r's voxels in x-outer, y, z-inner order).  A voxel in two ROIs
+ * appears in both blocks.  nroi = 0 turns the reduction off.  The plain mode rejects ROIs. */
+int cccs_set_roi_masks(cccs_ctx* ctx, int nroi, const uint8_t* const* masks, const uint32_t* crop_start,
+                       const uint32_t* crop_size);
+
 typedef struct cccs_sink_totals {
     uint64_t columns;
     uint64_t nnz;
@@ -231,6 +247,25 @@ int cccs_server_stats(cccs_server* server, cccs_server_info* out);
 /* Finishes every queued request, then releases the slots. */
 int cccs_server_destroy(cccs_server* server);
 
+/* ---- preprocess inputs -------------------------------------------------------------------------
+ * Projection of a structure mask onto each beam's fluence map (findFluenceProjection + cudaRaycast,
+ * dosecalc-preprocess/cudaMemFunctions.cu:308-393, kernels.cuh:136-224).  `mask` holds
+ * size[2]*size[1]*size[0] bytes on the context's CT grid (non-zero = inside the structure).  For
+ * each beamlet, nine sub-rays (the cell centre and the points half a beamlet away along each
+ * fluence axis and diagonal) run from the source to three times the source-to-cell distance
+ * through the trilinearly sampled mask with the TERMA Siddon march; the beamlet is 1 when any
+ * sub-ray accumulates a positive path integral, else 0.  `fluence` receives the maps of all beams
+ * back to back (fmap_size[1] rows of fmap_size[0] each). */
+int cccs_fluence_projection(cccs_ctx* ctx, const uint8_t* mask, const cccs_beam* beams, int nbeams,
+                            float* fluence);
+/* Aperture-ready fill of a fluence map, row by row (fmap_post_apertureready,
+ * dosecalc-preprocess/fmapProcessing.cpp:7-46, with its segment bookkeeping: a run of non-zero
+ * beamlets is recorded by its last beamlet, so the fill between two runs covers the gap and all
+ * but the last beamlet of the next run).  fill_val is 1 in the reference. */
+int cccs_fmap_aperture_fill(float* fluence, int width, int height, float fill_val);
+/* 1 when every row has at most one recorded run (fmap_is_apertureready, fmapProcessing.cpp:26-32). */
+int cccs_fmap_is_aperture_ready(const float* fluence, int width, int height, int* ready);
+
 /* ---- the reference's file formats (host only) -----------------------------------------------
  * Beam list "az zen [coll] [iso: x y z] [sad: s] [dir: x y z]" in degrees/cm, '#' comments,
  * default SAD 100 cm; each beam is reconfigured (from angles, or from dir: when given).
@@ -251,6 +286,11 @@ int cccs_write_omni_header(const char* path, const cccs_problem* p, int beam_cou
                            const char* beam_spec, const char* target_structure, int reduce);
 /* Raw little-endian float32 files: density.raw, fluence maps (fluence.cpp:19-44, exact size
  * required), dose output (<out>.raw, DoseCalcIO/binary_io.h:47-86). */
+/* omni_beam_list.txt writer (write_omni_beam_list, DoseCalcIO/beam.cpp:577-607).  The string
+ * arrays may be NULL (defaults "unspec", "unspec", "auto" and "fmap-%06d.raw"). */
+int cccs_write_omni_beam_list(const char* path, const cccs_beam* beams, int beam_count,
+                              const char* const* iso_type, const char* const* iso_loc,
+                              const char* const* orient_type, const char* const* fmap_names);
 int cccs_read_f32(const char* path, float* out, uint64_t n);
 int cccs_write_f32(const char* path, const float* data, uint64_t n);
 

@@ -1194,4 +1194,83 @@ int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beamp, const float
     return CCCS_OK;
 }
 
+// _reduce_by_roi (DoseCalcIO/sparse_io.cpp:62-158): the column scattered into a dense volume, then
+// for each ROI the crop box walked ii (x) outer, jj (y), kk (z) inner; every mask voxel advances the
+// ROI's element counter and a non-zero dose there is emitted as (mark + element, dose).
+int64_t orc_reduce_by_roi(const uint64_t* keys, const float* vals, int64_t nnz, const uint32_t* size, int nroi,
+                          const uint8_t* const* masks, const uint32_t* crop_start, const uint32_t* crop_size,
+                          uint64_t* out_keys, float* out_vals, int64_t cap, uint64_t* row_block_sizes) {
+    const size_t nv = (size_t)size[0] * size[1] * size[2];
+    std::vector<float> dense(nv, 0.f);
+    for (int64_t i = 0; i < nnz; i++) dense.at(keys[i]) = vals[i];
+    int64_t n = 0;
+    uint64_t mark = 0;
+    for (int r = 0; r < nroi; r++) {
+        const uint32_t* c0 = crop_start + 3 * r;
+        const uint32_t* cs = crop_size + 3 * r;
+        uint64_t kept = 0, elem = 0;
+        for (uint32_t ii = 0; ii < cs[0]; ii++)
+            for (uint32_t jj = 0; jj < cs[1]; jj++)
+                for (uint32_t kk = 0; kk < cs[2]; kk++) {
+                    const uint64_t crop_key = ((uint64_t)kk * cs[1] + jj) * cs[0] + ii;
+                    const uint64_t orig_key = (((uint64_t)kk + c0[2]) * size[1] + (jj + c0[1])) * size[0] + (ii + c0[0]);
+                    if (!masks[r][crop_key]) continue;
+                    const float d = dense.at(orig_key);
+                    if (d != 0.f) {
+                        if (n < cap) {
+                            out_keys[n] = mark + elem;
+                            out_vals[n] = d;
+                        }
+                        n++;
+                        kept++;
+                    }
+                    elem++;
+                }
+        mark += elem;
+        row_block_sizes[r] = kept;
+    }
+    return n;
+}
+
+// Preprocess: findFluenceProjection + cudaRaycast (dosecalc-preprocess/cudaMemFunctions.cu:308-393,
+// kernels.cuh:136-224).  Nine sub-rays per beamlet (subray index i: offsets (i%3-1, i/3-1) half
+// beamlets), bixel centre rotated by inverseRotateBeamAtOriginRHS (geometry.cu:85-92) plus the
+// isocentre plus 1e-9, the ray extended to 3x the source-to-centre distance, cudaRaytrace_device
+// through the mask (trilinear, border 0, no +0.5 shift).  Beamlet = 1 when any sub-ray's rpl > 0.
+int orc_fluence_projection(const cccs_problem* p, const uint8_t* mask, const cccs_beam* beams, int nbeams,
+                           float* out) {
+    if (!p || !mask || !beams || !out || nbeams < 0) return CCCS_EINVAL;
+    const size_t nv = (size_t)p->size[0] * p->size[1] * p->size[2];
+    std::vector<float> mf(nv);
+    for (size_t i = 0; i < nv; i++) mf[i] = (float)mask[i];
+    const Vol v{mf.data(), (int)p->size[0], (int)p->size[1], (int)p->size[2]};
+    const f3 start = from_f(p->start), voxel = from_f(p->voxel);
+    const f3 end = mk(start.x + (float)p->size[0] * voxel.x, start.y + (float)p->size[1] * voxel.y,
+                      start.z + (float)p->size[2] * voxel.z);
+    size_t off = 0;
+    for (int bi = 0; bi < nbeams; bi++) {
+        const cccs_beam& b = beams[bi];
+        const int fx = (int)b.fmap_size[0], fy = (int)b.fmap_size[1];
+        const f3 src = from_f(b.source), iso = from_f(b.isocenter);
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int pix = 0; pix < fx * fy; pix++) {
+            const int rx = pix % fx, rz = pix / fx;
+            bool hit = false;
+            for (int sr = 0; sr < 9; sr++) {
+                const int sx = sr % 3 - 1, sz = sr / 3 - 1;
+                f3 fcs = mk(b.beamlet_size[0] * ((float)rx + 0.5f * (float)sx - (float)fx * 0.5f + 0.5f), 0.f,
+                            b.beamlet_size[1] * ((float)rz + 0.5f * (float)sz - (float)fy * 0.5f + 0.5f));
+                f3 ctr = invRotateBeam(fcs, b.azimuth, b.zenith, b.coll) + iso;
+                ctr = ctr + 1e-9f;
+                const f3 sink = src + 3.f * (ctr - src);
+                if (siddon_rpl(sink, start, end, voxel, src, v, [](float, float, float, float) {}, nullptr) > 0.f)
+                    hit = true;
+            }
+            out[off + (size_t)pix] = hit ? 1.f : 0.f;
+        }
+        off += (size_t)fx * fy;
+    }
+    return CCCS_OK;
+}
+
 }  // extern "C"

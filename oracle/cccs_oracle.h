@@ -95,6 +95,17 @@ int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beam, const float*
 int orc_beamlet_anchor(const cccs_problem* p, const cccs_beam* beam, int idx, float* start3, float* end3,
                        float* biso3, float* angles2);
 
+/* Preprocess: PTV projection onto fluence maps (findFluenceProjection + cudaRaycast,
+ * dosecalc-preprocess/cudaMemFunctions.cu:308-393, kernels.cuh:136-224); maps back to back. */
+int orc_fluence_projection(const cccs_problem* p, const uint8_t* mask, const cccs_beam* beams, int nbeams,
+                           float* out);
+
+/* ROI reduction of one sparse column (_reduce_by_roi, DoseCalcIO/sparse_io.cpp:62-158).  ROI r:
+ * byte mask over its crop box (x fastest).  Returns the reduced count (writes at most cap). */
+int64_t orc_reduce_by_roi(const uint64_t* keys, const float* vals, int64_t nnz, const uint32_t* size, int nroi,
+                          const uint8_t* const* masks, const uint32_t* crop_start, const uint32_t* crop_size,
+                          uint64_t* out_keys, float* out_vals, int64_t cap, uint64_t* row_block_sizes);
+
 int64_t orc_sparsify(const float* dense, const uint32_t* size, const uint32_t* crop_start,
                      const uint32_t* crop_size, float thresh, uint64_t* keys, float* vals,
                      int64_t cap, float* perc_nonzero, float* perc_dropped);

@@ -61,6 +61,7 @@ def load() -> C.CDLL:
         "orc_pack_row_convolve": [_fp, _fp, C.c_int, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_float, C.c_int, _fp],
         "orc_beamlet_dense": [P, B, _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int), C.POINTER(Stats)],
         "orc_beamlet_anchor": [P, B, C.c_int, _fp, _fp, _fp, _fp],
+        "orc_fluence_projection": [P, C.c_void_p, B, C.c_int, _fp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -69,6 +70,10 @@ def load() -> C.CDLL:
     lib.orc_sparsify.argtypes = [_fp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_float,
                                  C.POINTER(C.c_uint64), _fp, C.c_int64, _fp, _fp]
     lib.orc_sparsify.restype = C.c_int64
+    lib.orc_reduce_by_roi.argtypes = [C.POINTER(C.c_uint64), _fp, C.c_int64, C.POINTER(C.c_uint32), C.c_int,
+                                      C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint64), _fp, C.c_int64, C.POINTER(C.c_uint64)]
+    lib.orc_reduce_by_roi.restype = C.c_int64
     _lib = lib
     return lib
 
@@ -122,6 +127,44 @@ def terma(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, beamletnum: int =
     _chk(load().orc_terma(C.byref(p), C.byref(beam), fp(np.ascontiguousarray(fmap, np.float32)), beamletnum, fp(out),
                           C.byref(st)), "terma")
     return out, st.as_dict()
+
+
+def reduce_by_roi(keys: np.ndarray, vals: np.ndarray, size, rois):
+    """_reduce_by_roi of one column.  size = CT (nx, ny, nz); rois = list of (crop mask (z, y, x)
+    bytes, crop_start (x, y, z), crop_size (x, y, z)).  Returns (keys, values, row_block_sizes)."""
+    k = np.ascontiguousarray(keys, np.uint64)
+    v = np.ascontiguousarray(vals, np.float32)
+    masks = [np.ascontiguousarray(m, np.uint8) for m, _, _ in rois]
+    n = len(rois)
+    ptrs = (C.c_void_p * max(1, n))(*[m.ctypes.data for m in masks])
+    st = (C.c_uint32 * max(3, 3 * n))(*[int(x) for _, s0, _ in rois for x in s0])
+    sz = (C.c_uint32 * max(3, 3 * n))(*[int(x) for _, _, s1 in rois for x in s1])
+    cap = int(sum(int(m.astype(bool).sum()) for m in masks))
+    ok = np.zeros(max(1, cap), np.uint64)
+    ov = np.zeros(max(1, cap), np.float32)
+    blocks = np.zeros(max(1, n), np.uint64)
+    size3 = (C.c_uint32 * 3)(*[int(x) for x in size])
+    got = load().orc_reduce_by_roi(k.ctypes.data_as(C.POINTER(C.c_uint64)), fp(v), len(k), size3, n,
+                                   C.cast(ptrs, C.c_void_p), st, sz, ok.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                   fp(ov), cap, blocks.ctypes.data_as(C.POINTER(C.c_uint64)))
+    assert got <= cap
+    return ok[:got].copy(), ov[:got].copy(), blocks[:n].copy()
+
+
+def fluence_projection(spec: ProblemSpec, mask: np.ndarray, beams) -> list:
+    """Structure mask (z, y, x bytes) projected onto each beam's fluence map (fy, fx floats)."""
+    p = spec.struct()
+    m = np.ascontiguousarray(mask, np.uint8)
+    assert m.shape == spec.density.shape
+    arr = (abi.Beam * len(beams))(*beams)
+    sizes = [(int(b.fmap_size[1]), int(b.fmap_size[0])) for b in beams]
+    out = np.zeros(sum(a * b for a, b in sizes), np.float32)
+    _chk(load().orc_fluence_projection(C.byref(p), m.ctypes.data, arr, len(beams), fp(out)), "fluence_projection")
+    maps, o = [], 0
+    for fy, fx in sizes:
+        maps.append(out[o:o + fy * fx].reshape(fy, fx).copy())
+        o += fy * fx
+    return maps
 
 
 def siddon_paths(spec: ProblemSpec, source, points: np.ndarray, max_segments: int = 1024):

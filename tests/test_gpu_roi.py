@@ -1,0 +1,62 @@
+"""GPU ROI reduction of beamlet columns (cccs_set_roi_masks) against the oracle's _reduce_by_roi
+applied to the same library's unreduced columns: keys, values and row-block sizes identical
+(integer and selection work: exact)."""
+import numpy as np
+import pytest
+
+from endtoend_amd import abi
+from endtoend_amd import workloads as W
+from endtoend_amd.engine import Engine
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rois(w):
+    nz, ny, nx = w.spec.density.shape
+    x, y, z = w.phantom.coords()
+    # PTV-like ellipsoid in a crop box, an OAR box overlapping it, and a thin full-grid slab
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    ell = (X / 3.0) ** 2 + ((Y + 1.0) / 4.0) ** 2 + (Z / 2.5) ** 2 <= 1.0
+    zs, ys, xs = np.nonzero(ell)
+    c0 = (int(xs.min()), int(ys.min()), int(zs.min()))
+    cs = (int(xs.max()) - c0[0] + 1, int(ys.max()) - c0[1] + 1, int(zs.max()) - c0[2] + 1)
+    ptv = ell[c0[2]:c0[2] + cs[2], c0[1]:c0[1] + cs[1], c0[0]:c0[0] + cs[0]].astype(np.uint8)
+    oar_c0, oar_cs = (nx // 2 - 2, 0, nz // 2 - 3), (6, ny // 2, 5)
+    oar = np.ones((oar_cs[2], oar_cs[1], oar_cs[0]), np.uint8)
+    oar[:, ::3, :] = 0
+    slab = np.zeros((nz, ny, nx), np.uint8)
+    slab[nz // 2 - 1:nz // 2 + 1] = 1
+    return [(ptv, c0, cs), (oar, oar_c0, oar_cs), (slab, (0, 0, 0), (nx, ny, nz))]
+
+
+@pytest.mark.parametrize("gantry,couch", [(0.0, 0.0), (215.0, -45.0)])
+def test_roi_reduced_columns_match_the_oracle(gantry, couch):
+    w = W.small(n=32, voxel=0.5, gantry=gantry, couch=couch, hetero=True, fmap=(8, 8), beamlet=(1.0, 1.0),
+                weights="rand")
+    w.spec.kernel_extent = 1.0
+    f = w.fmaps[0].copy()
+    f.ravel()[::3] = 0.0
+    rois = _rois(w)
+    nz, ny, nx = w.spec.density.shape
+    print(f"[case] roi g{gantry} c{couch}", flush=True)
+    with Engine(w.spec) as eng:
+        plain = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
+        eng.set_roi_masks(rois)
+        red = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
+        with pytest.raises(abi.CCCSError):
+            eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_PLAIN)
+        eng.set_roi_masks([])
+        again = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
+    assert len(plain) == len(red) > 10
+    kept = 0
+    for (b0, l0, k0, v0), (b1, l1, k1, v1, blocks) in zip(plain, red):
+        assert (b0, l0) == (b1, l1)
+        ok, ov, ob = O.reduce_by_roi(k0, v0, (nx, ny, nz), rois)
+        assert np.array_equal(blocks, ob)
+        assert np.array_equal(k1, ok)
+        assert np.array_equal(v1.view(np.uint32), ov.view(np.uint32))
+        kept += len(k1)
+    assert kept > 100
+    # turning the reduction off restores the plain columns
+    assert all(np.array_equal(a[2], b[2]) for a, b in zip(plain, again))
