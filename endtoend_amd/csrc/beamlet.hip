@@ -118,6 +118,7 @@ struct BlArgs {
     int fast;                   // unit-TERMA grid instead of a Siddon march per sample
     int tg[3];                  // unit-TERMA grid dims (CT dims + 2)
     uint32_t* counts;
+    const uint32_t* tile_batch; // batch of every gather tile (nullptr: a single batch)
 };
 
 __device__ __forceinline__ int find_cone(const BlConeDev* __restrict__ c, int n, uint64_t line) {
@@ -582,13 +583,11 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
 __global__ void __launch_bounds__(256) bl_gather_kernel(BlArgs a, uint32_t ntiles) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
-    int lo = 0, hi = a.nbatch - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.batches[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
-    }
+    const int lo = a.tile_batch ? (int)a.tile_batch[tile] : 0;
     const BatchDev& B = a.batches[lo];
     const BlConeDev& c = a.cones[lo];
+    // 16x4x4 voxel tiles (pY fastest): compact in every direction, so neighbouring lanes read
+    // neighbouring REV lines
     const uint32_t t = tile - B.tile0;
     const int ty = (int)(t % (uint32_t)B.tiles[0]);
     const int tz = (int)((t / (uint32_t)B.tiles[0]) % (uint32_t)B.tiles[1]);
@@ -626,6 +625,15 @@ __global__ void __launch_bounds__(256) bl_gather_kernel(BlArgs a, uint32_t ntile
     const v3 ri = mk3(fmaf(-c.minc.x, a.rrv.x, rk.x), fmaf(-c.minc.y, a.rrv.y, rk.y), fmaf(-c.minc.z, a.rrv.z, rk.z));
     const float dc = rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
     if (dc > 0.f) a.pdose[B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX)] += dc;
+}
+
+// The batch of every gather tile of a job (one fill per batch range).
+__global__ void __launch_bounds__(256) bl_tile_batch_kernel(const BatchDev* __restrict__ batches,
+                                                            uint32_t* __restrict__ tb) {
+    const int b = blockIdx.y;
+    const BatchDev& B = batches[b];
+    const uint32_t n = (uint32_t)B.tiles[0] * (uint32_t)B.tiles[1] * (uint32_t)B.tiles[2];
+    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) tb[B.tile0 + t] = (uint32_t)b;
 }
 
 __global__ void __launch_bounds__(256) bl_fill_kernel(float* __restrict__ p, uint64_t n, float v) {
@@ -1353,6 +1361,11 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     a.rev = ctx->rev.as<float>();
     a.pdose = ctx->bl_pdose.as<float>();
     a.nbatch = nb;
+    CCCS_TRY(ctx->bl_tiles.ensure(sizeof(uint32_t) * std::max<uint32_t>(1, J.tiles)));
+    hipLaunchKernelGGL(bl_tile_batch_kernel, dim3(64, (unsigned)nb), dim3(256), 0, ctx->stream, a.batches,
+                       ctx->bl_tiles.as<uint32_t>());
+    CCCS_HIP(hipGetLastError());
+    a.tile_batch = ctx->bl_tiles.as<uint32_t>();
     const int half = (int)p.ntheta / 2;
     for (int attempt = 0; attempt < 3; attempt++) {
         subcap = std::min<uint64_t>(subcap, 0xffffffffULL / kSubQueues);

@@ -309,7 +309,7 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     c->bl_tab.release(); c->bl_pdose.release(); c->bl_tgrid.release(); c->bl_arena.release();
     c->bl_cnt.release(); c->bl_keys.release(); c->bl_vals.release(); c->bl_fmap.release();
-    c->bl_h.release(); c->bl_queue.release(); c->bl_qcnt.release();
+    c->bl_h.release(); c->bl_queue.release(); c->bl_qcnt.release(); c->bl_tiles.release();
     c->h_bl.release(); c->h_bl_out.release();
     c->roi_tab.release(); c->roi_rank.release(); c->roi_cnt.release(); c->roi_keys.release(); c->roi_vals.release();
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);

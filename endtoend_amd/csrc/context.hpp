@@ -121,7 +121,7 @@ struct cccs_ctx {
     cccs::HostBuf h_small, h_cones, h_stage;  // h_stage: pinned double buffer of download_staged
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // beamlet path: job tables, packed grids, unit-TERMA grids, column arena, sparse output
-    cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt;
+    cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt, bl_tiles;
     cccs::HostBuf h_bl, h_bl_out;
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
