@@ -138,6 +138,7 @@ EXPORTS = [
     ("cccs_ctx_memory", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("cccs_ctx_set_schedule", C.c_int, [_P, C.c_char_p, C.c_int]),
     ("cccs_fluence_projection", C.c_int, [_P, C.c_void_p, C.POINTER(Beam), C.c_int, _fp]),
+    ("cccs_fluence_convolve", C.c_int, [_P, _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _fp]),
     ("cccs_set_roi_masks", C.c_int, [_P, C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("cccs_fmap_aperture_fill", C.c_int, [_fp, C.c_int, C.c_int, C.c_float]),
     ("cccs_fmap_is_aperture_ready", C.c_int, [_fp, C.c_int, C.c_int, C.POINTER(C.c_int)]),

@@ -788,6 +788,43 @@ __global__ void __launch_bounds__(64) projection_kernel(const float* __restrict_
     out[t] = hit ? 1.f : 0.f;
 }
 
+// ---- fluence-map convolution (the FCBB lateral step; no reference source) --------------------
+// out[y][x] = sum_j sum_i K[j][i] * F[y - (j - kh/2)][x - (i - kw/2)], zero outside the map, the
+// terms added j outer, i inner.  One lane per output of a 16x16 tile; the tile plus its halo and
+// the kernel sit in LDS.
+constexpr int kConvTile = 16, kConvMaxK = 31;
+
+__global__ void __launch_bounds__(256) fluence_conv_kernel(const float* __restrict__ f, int w, int h,
+                                                           const float* __restrict__ k, int kw, int kh,
+                                                           float* __restrict__ out) {
+    __shared__ float s_k[kConvMaxK * kConvMaxK];
+    __shared__ float s_f[(kConvTile + kConvMaxK - 1) * (kConvTile + kConvMaxK - 1)];
+    const int map = blockIdx.z;
+    const float* F = f + (size_t)map * w * h;
+    const int x0 = blockIdx.x * kConvTile, y0 = blockIdx.y * kConvTile;
+    const int rx = kw / 2, ry = kh / 2;
+    const int sw = kConvTile + kw - 1, sh = kConvTile + kh - 1;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < kw * kh; q += 256) s_k[q] = k[q];
+    // s_f[v][u] = F[y0 + ry - v][x0 + rx - u]: the flipped window, so output (tx, ty) reads
+    // s_f[ty + j][tx + i] for kernel tap (i, j)
+    for (int q = tid; q < sw * sh; q += 256) {
+        const int u = q % sw, v = q / sw;
+        const int xx = x0 + (kConvTile - 1) - u + rx, yy = y0 + (kConvTile - 1) - v + ry;
+        s_f[q] = (xx >= 0 && yy >= 0 && xx < w && yy < h) ? F[(size_t)yy * w + xx] : 0.f;
+    }
+    __syncthreads();
+    const int tx = tid % kConvTile, ty = tid / kConvTile;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) return;
+    // output (x, y) needs F[y - j + ry][x - i + rx] = s_f[(kConvTile-1-ty) + j][(kConvTile-1-tx) + i]
+    const int bu = kConvTile - 1 - tx, bv = kConvTile - 1 - ty;
+    float acc = 0.f;
+    for (int j = 0; j < kh; j++)
+        for (int i = 0; i < kw; i++) acc += s_k[j * kw + i] * s_f[(bv + j) * sw + bu + i];
+    out[(size_t)map * w * h + (size_t)y * w + x] = acc;
+}
+
 __global__ void mask_to_float_kernel(const uint8_t* __restrict__ m, float* __restrict__ f, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         f[i] = (float)m[i];
@@ -1239,6 +1276,40 @@ extern "C" int cccs_fluence_projection(cccs_ctx* ctx, const uint8_t* mask, const
     CCCS_HIP(hipGetLastError());
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
     CCCS_HIP(hipMemcpy(fluence, ob.p, sizeof(float) * total, hipMemcpyDeviceToHost));
+    return CCCS_OK;
+}
+
+extern "C" int cccs_fluence_convolve(cccs_ctx* ctx, const float* fluence, int nmaps, int width, int height,
+                                     const float* kernel, int kw, int kh, float* out) {
+    clear_error();
+    if (!ctx || (nmaps > 0 && (!fluence || !kernel || !out))) return set_error(CCCS_EINVAL, "null argument");
+    if (nmaps < 0 || width < 1 || height < 1 || width > 4096 || height > 4096) return set_error(CCCS_EINVAL, "map size");
+    if (kw < 1 || kh < 1 || kw > kConvMaxK || kh > kConvMaxK || !(kw & 1) || !(kh & 1))
+        return set_error(CCCS_EINVAL, "kernel dims must be odd and at most 31");
+    if (nmaps == 0) return CCCS_OK;
+    if (nmaps > 65535) return set_error(CCCS_EINVAL, "at most 65535 maps per call");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)nmaps * width * height;
+    DevBuf fb, kb, ob;
+    struct Release {
+        DevBuf* b[3];
+        ~Release() {
+            for (DevBuf* x : b) x->release();
+        }
+    } release{{&fb, &kb, &ob}};
+    CCCS_TRY(fb.ensure(sizeof(float) * n));
+    CCCS_TRY(kb.ensure(sizeof(float) * (size_t)kw * kh));
+    CCCS_TRY(ob.ensure(sizeof(float) * n));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(fb.p, fluence, sizeof(float) * n, hipMemcpyHostToDevice));
+    CCCS_HIP(hipMemcpy(kb.p, kernel, sizeof(float) * (size_t)kw * kh, hipMemcpyHostToDevice));
+    const dim3 grid((unsigned)((width + kConvTile - 1) / kConvTile), (unsigned)((height + kConvTile - 1) / kConvTile),
+                    (unsigned)nmaps);
+    hipLaunchKernelGGL(fluence_conv_kernel, grid, dim3(256), 0, ctx->stream, (const float*)fb.as<float>(), width, height,
+                       (const float*)kb.as<float>(), kw, kh, ob.as<float>());
+    CCCS_HIP(hipGetLastError());
+    CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipMemcpy(out, ob.p, sizeof(float) * n, hipMemcpyDeviceToHost));
     return CCCS_OK;
 }
 

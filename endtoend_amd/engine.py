@@ -203,6 +203,20 @@ class Engine:
             o += fy * fx
         return maps
 
+    def fluence_convolve(self, fmaps: np.ndarray, kernel: np.ndarray) -> np.ndarray:
+        """Fluence maps (n, h, w) or (h, w) convolved with an odd (kh, kw) kernel, same size, zero
+        outside (cccs_fluence_convolve)."""
+        f = np.ascontiguousarray(fmaps, np.float32)
+        one = f.ndim == 2
+        if one:
+            f = f[None]
+        k = np.ascontiguousarray(kernel, np.float32)
+        out = np.zeros_like(f)
+        abi.check(self.lib.cccs_fluence_convolve(self.ctx, abi.fptr(f), f.shape[0], f.shape[2], f.shape[1],
+                                                 abi.fptr(k), k.shape[1], k.shape[0], abi.fptr(out)),
+                  "cccs_fluence_convolve")
+        return out[0] if one else out
+
     def set_roi_masks(self, rois) -> None:
         """ROI reduction of later beamlet columns (cccs_set_roi_masks).  rois: list of
         (mask over the crop box as (z, y, x) bytes, crop_start (x, y, z), crop_size (x, y, z));

@@ -258,6 +258,12 @@ int cccs_server_destroy(cccs_server* server);
  * back to back (fmap_size[1] rows of fmap_size[0] each). */
 int cccs_fluence_projection(cccs_ctx* ctx, const uint8_t* mask, const cccs_beam* beams, int nbeams,
                             float* fluence);
+/* Fluence-map convolution, the lateral step of fluence-convolution broad-beam (FCBB) dose named by
+ * the reference's README (README.md:15-20; no source in the reference): for each of `nmaps` maps
+ * of height rows x width, out[y][x] = sum_j sum_i kernel[j][i] * map[y - (j - kh/2)][x - (i - kw/2)]
+ * (kernel rows j of kw entries, odd kw, kh <= 31), zero outside the map, the same size out. */
+int cccs_fluence_convolve(cccs_ctx* ctx, const float* fluence, int nmaps, int width, int height,
+                          const float* kernel, int kw, int kh, float* out);
 /* Aperture-ready fill of a fluence map, row by row (fmap_post_apertureready,
  * dosecalc-preprocess/fmapProcessing.cpp:7-46, with its segment bookkeeping: a run of non-zero
  * beamlets is recorded by its last beamlet, so the fill between two runs covers the gap and all

@@ -1194,6 +1194,26 @@ int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beamp, const float
     return CCCS_OK;
 }
 
+// FCBB fluence-map convolution (README.md:15-20; no source in the reference): zero-padded, same
+// size, out[y][x] = sum_j sum_i K[j][i] * F[y - (j - kh/2)][x - (i - kw/2)], j outer, i inner.
+int orc_fluence_convolve(const float* f, int nmaps, int w, int h, const float* k, int kw, int kh, float* out) {
+    if (!f || !k || !out || nmaps < 0 || w < 1 || h < 1 || kw < 1 || kh < 1 || !(kw & 1) || !(kh & 1)) return CCCS_EINVAL;
+    const int rx = kw / 2, ry = kh / 2;
+    for (int m = 0; m < nmaps; m++)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                float acc = 0.f;
+                for (int j = 0; j < kh; j++)
+                    for (int i = 0; i < kw; i++) {
+                        const int yy = y - (j - ry), xx = x - (i - rx);
+                        const float v = (xx >= 0 && yy >= 0 && xx < w && yy < h) ? f[((size_t)m * h + yy) * w + xx] : 0.f;
+                        acc += k[j * kw + i] * v;
+                    }
+                out[((size_t)m * h + y) * w + x] = acc;
+            }
+    return CCCS_OK;
+}
+
 // _reduce_by_roi (DoseCalcIO/sparse_io.cpp:62-158): the column scattered into a dense volume, then
 // for each ROI the crop box walked ii (x) outer, jj (y), kk (z) inner; every mask voxel advances the
 // ROI's element counter and a non-zero dose there is emitted as (mark + element, dose).

@@ -100,6 +100,9 @@ int orc_beamlet_anchor(const cccs_problem* p, const cccs_beam* beam, int idx, fl
 int orc_fluence_projection(const cccs_problem* p, const uint8_t* mask, const cccs_beam* beams, int nbeams,
                            float* out);
 
+/* A0 (FCBB): fluence-map convolution, zero outside, same size, j outer / i inner summation. */
+int orc_fluence_convolve(const float* f, int nmaps, int w, int h, const float* k, int kw, int kh, float* out);
+
 /* ROI reduction of one sparse column (_reduce_by_roi, DoseCalcIO/sparse_io.cpp:62-158).  ROI r:
  * byte mask over its crop box (x fastest).  Returns the reduced count (writes at most cap). */
 int64_t orc_reduce_by_roi(const uint64_t* keys, const float* vals, int64_t nnz, const uint32_t* size, int nroi,

@@ -62,6 +62,7 @@ def load() -> C.CDLL:
         "orc_beamlet_dense": [P, B, _fp, C.c_int, _fp, C.c_int, C.POINTER(C.c_int), C.POINTER(Stats)],
         "orc_beamlet_anchor": [P, B, C.c_int, _fp, _fp, _fp, _fp],
         "orc_fluence_projection": [P, C.c_void_p, B, C.c_int, _fp],
+        "orc_fluence_convolve": [_fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _fp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -127,6 +128,18 @@ def terma(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, beamletnum: int =
     _chk(load().orc_terma(C.byref(p), C.byref(beam), fp(np.ascontiguousarray(fmap, np.float32)), beamletnum, fp(out),
                           C.byref(st)), "terma")
     return out, st.as_dict()
+
+
+def fluence_convolve(fmaps: np.ndarray, kernel: np.ndarray) -> np.ndarray:
+    f = np.ascontiguousarray(fmaps, np.float32)
+    one = f.ndim == 2
+    if one:
+        f = f[None]
+    k = np.ascontiguousarray(kernel, np.float32)
+    out = np.zeros_like(f)
+    _chk(load().orc_fluence_convolve(fp(f), f.shape[0], f.shape[2], f.shape[1], fp(k), k.shape[1], k.shape[0],
+                                     fp(out)), "fluence_convolve")
+    return out[0] if one else out
 
 
 def reduce_by_roi(keys: np.ndarray, vals: np.ndarray, size, rois):
