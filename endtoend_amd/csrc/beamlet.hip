@@ -29,7 +29,6 @@ constexpr int kWall = 2;        // pillar wall, REV samples (PILLAR_GRID, io_dat
 constexpr int kBuffer = 2;      // pillar rim excluded from unpacking (same struct)
 constexpr int kMaxBatches = 100;  // batch-count retry limit of the reference (host.cu:692-834)
 constexpr uint32_t kSubQueues = 64;  // TERMA sub-queues per cone direction
-constexpr uint32_t kLineWaves = 8192;  // waves of the per-line kernels (each takes a stride of lines)
 constexpr uint32_t kPillarBits = 4096;  // batches up to this many pillars track footprint pillars per line
 
 // One Rodrigues rotation with host-evaluated sine/cosine.
@@ -1388,7 +1387,8 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             CCCS_TRY(launch_line_scan(ctx, a.lines, a.nlines));
             uint32_t maxn = 1;
             for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
-            const unsigned lw = (unsigned)std::min<uint64_t>(a.nlines, kLineWaves);
+            const unsigned lw = (unsigned)std::min<uint64_t>(a.nlines, (uint64_t)ctx->sched.bl_sample_waves);
+            const unsigned ww = (unsigned)std::min<uint64_t>(a.nlines, (uint64_t)ctx->sched.bl_walk_waves);
             a.counts = prof_counter(ctx, lw);
             int tok = prof_begin(ctx, K_BL_SAMPLE);
             hipLaunchKernelGGL(bl_sample_kernel, dim3(lw), dim3(64), 0, ctx->stream, a, ctx->bl_h.as<float>(), q,
@@ -1404,10 +1404,10 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                 prof_end(ctx, ts, 0.0, 0.0, 0);
                 CCCS_HIP(hipGetLastError());
             }
-            a.counts = prof_counter(ctx, lw);
+            a.counts = prof_counter(ctx, ww);
             const size_t lds = (sizeof(double) + sizeof(float2) + 3 * sizeof(short)) * (size_t)maxn;
             int tw = prof_begin(ctx, K_BL_WALK);
-            hipLaunchKernelGGL(bl_walk_kernel, dim3(lw), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>(),
+            hipLaunchKernelGGL(bl_walk_kernel, dim3(ww), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>(),
                                (int)maxn);
             prof_end(ctx, tw, 0.0, 0.0, a.nlines);
             CCCS_HIP(hipGetLastError());

@@ -336,6 +336,11 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
         c->sched.cone_ramp_min = value;
         return CCCS_OK;
     }
+    if (std::strcmp(key, "bl_sample_waves") == 0 || std::strcmp(key, "bl_walk_waves") == 0) {
+        if (value < 64 || value > (1 << 22)) return set_error(CCCS_EINVAL, "wave count must be in [64, 2^22]");
+        (key[3] == 's' ? c->sched.bl_sample_waves : c->sched.bl_walk_waves) = value;
+        return CCCS_OK;
+    }
     for (int i = 0; i < kConeClasses; i++) {
         const std::string k = "cone_class_threads" + std::to_string(i);
         if (k == key) {

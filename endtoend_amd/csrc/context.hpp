@@ -102,6 +102,11 @@ struct Schedule {
     int cone_class_len[kConeClasses] = {64, 160, 320, kMaxLine};
     // threads per line (64, 128 or 256) of each class: long lines spread over more waves
     int cone_class_threads[kConeClasses] = {64, 128, 128, 128};
+    // waves of the beamlet path's persistent per-line kernels (each takes a stride of lines); the
+    // sample stage's order of lines also orders the TERMA queue, and more waves there give the
+    // Siddon stage more coherent neighbouring requests
+    int bl_sample_waves = 262144;
+    int bl_walk_waves = 32768;
 };
 }  // namespace cccs
 

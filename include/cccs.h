@@ -144,6 +144,9 @@ int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scr
  *                    upper stored-line lengths of the first three cone-kernel launch classes
  *                    (defaults 64, 160, 320; the last class takes the longer lines).  Each class
  *                    launch sizes its LDS for its longest line.
+ *   "bl_sample_waves", "bl_walk_waves"
+ *                    waves of the beamlet path's persistent per-line kernels (defaults 262144 and
+ *                    32768; each wave takes a stride of lines).
  *   "cone_class_threads0".."cone_class_threads3"
  *                    threads per cone line in each class: 64, 128 or 256 (defaults 64, 128, 128,
  *                    128). */
