@@ -103,8 +103,14 @@ def cpu_baseline(w, eng, nbeamlets: int):
             "sample": f"{n} beamlets of beam 0 of the batched workload, dense columns, {dt:.1f} s"}, err
 
 
-def plain_beam_path(steps: int = 2):
+BEAM_KNOBS = ("cone_ramp_min", "cone_class_len0", "cone_class_len1", "cone_class_len2", "cone_class_threads1",
+              "cone_class_threads2", "cone_class_threads3")
+BEAMLET_KNOBS = ("bl_sample_waves", "bl_walk_waves")
+
+
+def plain_beam_path(steps: int = 2, autotune: bool = False):
     """The plain full-beam path on the 20-beam thorax plan: beamlet-doses/s and per-beam latency."""
+    from endtoend_amd import abi
     from endtoend_amd import workloads as W
     from endtoend_amd.engine import Engine
     w = W.cfg2()
@@ -112,6 +118,10 @@ def plain_beam_path(steps: int = 2):
     fl = np.concatenate([f.ravel() for f in w.fmaps]).astype(np.float32)
     offs = np.cumsum([0] + [f.size for f in w.fmaps])[:-1]
     eng = Engine(w.spec)
+    # the cone-kernel schedule: the library's defaults (chosen by its autotuner on this kind of
+    # plan), or a fresh autotune on two of the plan's beams
+    tune = (eng.autotune(w.beams[:2], w.fmaps[:2], kind=abi.CCCS_TUNE_BEAM)["knobs"] if autotune else
+            {k: eng.get_schedule(k) for k in BEAM_KNOBS})
     d_fl = eng.dev_alloc(fl.nbytes)
     eng.dev_upload(d_fl, fl)
     d_dose = eng.dev_alloc(nv * 4)
@@ -135,7 +145,8 @@ def plain_beam_path(steps: int = 2):
     eng.close()
     return {"workload": "200x200x160 thorax @ 2 mm, 20 coplanar beams, 40x40 x 5 mm U[0,1] fluence masked "
                         "to the PTV, full-beam dose", "value": w.active_beamlets / dt, "unit": "beamlet-doses/s",
-            "ms_per_plan": 1e3 * dt, "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)}}
+            "ms_per_plan": 1e3 * dt, "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99)},
+            "schedule": tune}
 
 
 def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
@@ -237,6 +248,8 @@ def main():
     ap.add_argument("--plain-beam-only", action="store_true",
                     help="only the plain full-beam path on the thorax plan (profiling runs)")
     ap.add_argument("--serving-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the schedule autotuner on a sample of each workload before timing it")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -248,7 +261,7 @@ def main():
         copy_calibration(args.copy_mib)
         return
     if args.plain_beam_only:
-        print(json.dumps(plain_beam_path(steps=args.steps)))
+        print(json.dumps(plain_beam_path(steps=args.steps, autotune=args.autotune)))
         return
     if args.serving_only:
         print(json.dumps(serving()))
@@ -262,6 +275,12 @@ def main():
     eng = Engine(w.spec)
     nbl = w.active_beamlets
     FA = abi.CCCS_BEAMLET_FAITHFUL
+    # launch schedule: the library's defaults (chosen by its autotuner on this workload), or with
+    # --autotune a fresh tuning on four of the workload's beams before the warmup
+    if args.autotune:
+        schedule = eng.autotune(w.beams[:4], w.fmaps[:4], kind=abi.CCCS_TUNE_BEAMLET)["knobs"]
+    else:
+        schedule = {k: eng.get_schedule(k) for k in BEAMLET_KNOBS}
 
     def step():
         eng.beamlet_dose(w.beams, w.fmaps, threshold=THRESH, mode=FA, keep=None)
@@ -300,7 +319,8 @@ def main():
                                "non-coplanar beams, 40x40 x 5 mm fluence open over the PTV projection, 64 cones, "
                                "kernel_extent 1 cm, sparse columns at 1e-4 of column max, faithful per-sample Siddon "
                                "TERMA, columns built in HBM",
-                   "path": "batched", "beamlet_doses_per_step": nbl, "beams_per_step": len(w.beams), "seed": w.seed},
+                   "path": "batched", "beamlet_doses_per_step": nbl, "beams_per_step": len(w.beams), "seed": w.seed,
+                   "schedule": schedule},
         "roofline": roofline(prof),
         "kernels": _kernels(prof),
     }
@@ -350,7 +370,7 @@ def main():
         out["max_err_pct_dmax"] = err
         out["cpu_baseline"] = cpu
         eng.close()
-        out["plain_beam_path"] = plain_beam_path()
+        out["plain_beam_path"] = plain_beam_path(autotune=args.autotune)
         out["serving"] = serving()
     else:
         eng.close()

@@ -104,6 +104,21 @@ class SparseColumn(C.Structure):
     ]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [
+        ("nkeys", C.c_int),
+        ("key", (C.c_char * 32) * 16),
+        ("value", C.c_int * 16),
+        ("default_ms", C.c_double),
+        ("best_ms", C.c_double),
+        ("trials", C.c_int),
+    ]
+
+
+CCCS_TUNE_BEAM = 0
+CCCS_TUNE_BEAMLET = 1
+
+
 class KernelStats(C.Structure):
     _fields_ = [
         ("name", C.c_char * 32),
@@ -137,6 +152,8 @@ EXPORTS = [
     ("cccs_ctx_destroy", C.c_int, [_P]),
     ("cccs_ctx_memory", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("cccs_ctx_set_schedule", C.c_int, [_P, C.c_char_p, C.c_int]),
+    ("cccs_ctx_get_schedule", C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int)]),
+    ("cccs_autotune", C.c_int, [_P, C.c_int, C.POINTER(Beam), C.c_int, C.POINTER(_fp), C.POINTER(TuneReport)]),
     ("cccs_fluence_projection", C.c_int, [_P, C.c_void_p, C.POINTER(Beam), C.c_int, _fp]),
     ("cccs_fluence_convolve", C.c_int, [_P, _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _fp]),
     ("cccs_set_roi_masks", C.c_int, [_P, C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),

@@ -360,6 +360,23 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
     return set_error(CCCS_EINVAL, std::string("unknown schedule key: ") + key);
 }
 
+extern "C" int cccs_ctx_get_schedule(const cccs_ctx* c, const char* key, int* value) {
+    clear_error();
+    if (!c || !key || !value) return set_error(CCCS_EINVAL, "null argument");
+    const Schedule& S = c->sched;
+    if (std::strcmp(key, "cone_ramp_min") == 0) { *value = S.cone_ramp_min; return CCCS_OK; }
+    if (std::strcmp(key, "bl_sample_waves") == 0) { *value = S.bl_sample_waves; return CCCS_OK; }
+    if (std::strcmp(key, "bl_walk_waves") == 0) { *value = S.bl_walk_waves; return CCCS_OK; }
+    for (int i = 0; i < kConeClasses; i++) {
+        if (("cone_class_threads" + std::to_string(i)) == key) { *value = S.cone_class_threads[i]; return CCCS_OK; }
+        if (i < kConeClasses - 1 && ("cone_class_len" + std::to_string(i)) == key) {
+            *value = S.cone_class_len[i];
+            return CCCS_OK;
+        }
+    }
+    return set_error(CCCS_EINVAL, std::string("unknown schedule key: ") + key);
+}
+
 // ---- memory helpers ----------------------------------------------------------------------------
 extern "C" int cccs_dev_alloc(cccs_ctx* ctx, size_t bytes, void** ptr) {
     clear_error();

@@ -99,9 +99,9 @@ struct Schedule {
     // segment-sum walk (O(nradii) per destination) instead of the sequential walk (O(line))
     int cone_ramp_min = 64;
     // upper stored-line lengths of the cone-kernel launch classes (the last class takes the rest)
-    int cone_class_len[kConeClasses] = {64, 160, 320, kMaxLine};
+    int cone_class_len[kConeClasses] = {96, 160, 320, kMaxLine};
     // threads per line (64, 128 or 256) of each class: long lines spread over more waves
-    int cone_class_threads[kConeClasses] = {64, 128, 128, 128};
+    int cone_class_threads[kConeClasses] = {64, 64, 128, 128};
     // waves of the beamlet path's persistent per-line kernels (each takes a stride of lines); the
     // sample stage's order of lines also orders the TERMA queue, and more waves there give the
     // Siddon stage more coherent neighbouring requests

@@ -231,6 +231,22 @@ class Engine:
         sz = (C.c_uint32 * max(3, 3 * n))(*[int(v) for _, _, s1 in rois for v in s1])
         abi.check(self.lib.cccs_set_roi_masks(self.ctx, n, C.cast(ptrs, C.c_void_p), st, sz), "cccs_set_roi_masks")
 
+    def get_schedule(self, key: str) -> int:
+        v = C.c_int(0)
+        abi.check(self.lib.cccs_ctx_get_schedule(self.ctx, key.encode(), C.byref(v)), "cccs_ctx_get_schedule")
+        return v.value
+
+    def autotune(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray], kind: int = abi.CCCS_TUNE_BEAMLET) -> dict:
+        """Coordinate-descent autotuning of the launch-schedule knobs on this workload
+        (cccs_autotune); the chosen values stay on the context."""
+        fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]
+        ptrs = (abi._fp * len(fm))(*[abi.fptr(f) for f in fm])
+        rep = abi.TuneReport()
+        abi.check(self.lib.cccs_autotune(self.ctx, kind, beams_array(beams), len(beams), ptrs, C.byref(rep)),
+                  "cccs_autotune")
+        return {"knobs": {rep.key[i].value.decode(): int(rep.value[i]) for i in range(rep.nkeys)},
+                "default_ms": rep.default_ms, "best_ms": rep.best_ms, "trials": rep.trials}
+
     def set_schedule(self, key: str, value: int):
         """Launch-schedule knob (cccs_ctx_set_schedule); e.g. "cone_ramp_min"."""
         abi.check(self.lib.cccs_ctx_set_schedule(self.ctx, key.encode(), int(value)), "cccs_ctx_set_schedule")

@@ -142,15 +142,34 @@ int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scr
  *                    O(line) per sample (default 64; a value above the longest line disables it).
  *   "cone_class_len0".."cone_class_len2"
  *                    upper stored-line lengths of the first three cone-kernel launch classes
- *                    (defaults 64, 160, 320; the last class takes the longer lines).  Each class
+ *                    (defaults 96, 160, 320; the last class takes the longer lines).  Each class
  *                    launch sizes its LDS for its longest line.
  *   "bl_sample_waves", "bl_walk_waves"
  *                    waves of the beamlet path's persistent per-line kernels (defaults 262144 and
  *                    32768; each wave takes a stride of lines).
  *   "cone_class_threads0".."cone_class_threads3"
- *                    threads per cone line in each class: 64, 128 or 256 (defaults 64, 128, 128,
+ *                    threads per cone line in each class: 64, 128 or 256 (defaults 64, 64, 128,
  *                    128). */
 int cccs_ctx_set_schedule(cccs_ctx* ctx, const char* key, int value);
+int cccs_ctx_get_schedule(const cccs_ctx* ctx, const char* key, int* value);
+
+/* Autotuner (no counterpart in the reference): times the given workload under candidate values of
+ * the schedule knobs, one knob at a time over two passes (coordinate descent), and leaves the
+ * fastest setting on the context.  CCCS_TUNE_BEAM tunes the full-beam path's cone-kernel knobs on
+ * device-resident beam doses of `beams`; CCCS_TUNE_BEAMLET tunes the beamlet path's wave counts on
+ * device-resident faithful columns.  Results do not depend on the knobs beyond FP32 rounding. */
+#define CCCS_TUNE_BEAM    0
+#define CCCS_TUNE_BEAMLET 1
+typedef struct cccs_tune_report {
+    int    nkeys;
+    char   key[16][32];
+    int    value[16];          /* chosen values                                                */
+    double default_ms;         /* one call of the workload with the knobs found on entry        */
+    double best_ms;            /* one call with the chosen knobs                                */
+    int    trials;             /* timed calls made                                              */
+} cccs_tune_report;
+int cccs_autotune(cccs_ctx* ctx, int kind, const cccs_beam* beams, int nbeams, const float* const* fluence,
+                  cccs_tune_report* report);
 
 /* ---- full-beam dose: replaces radconvolveTexture (dosecalc-beam) + cuda_sum_device_float_dose.
  * Computes the dose of `nbeams` beams and adds their sum into `dose` (Nz*Ny*Nx, host memory).
