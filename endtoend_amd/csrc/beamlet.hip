@@ -1366,56 +1366,107 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     CCCS_HIP(hipGetLastError());
     a.tile_batch = ctx->bl_tiles.as<uint32_t>();
     const int half = (int)p.ntheta / 2;
+    // Two cone directions in flight (bl_streams = 2): direction r runs on stream r % 2 with its own
+    // line table, REV store, half steps, TERMA queue and scan partials, so direction r+1's sample,
+    // Siddon and walk overlap direction r's.  The gathers read-modify-write the shared packed
+    // grids, so gather r waits for gather r-1 (an event).  A set is reused by direction r+2 on the
+    // same stream, after direction r's gather in stream order.
+    const int nset = ctx->sched.bl_streams == 2 ? 2 : 1;
+    hipStream_t S[2] = {ctx->stream, ctx->stream2};
+    DevBuf* set_lines[2] = {&ctx->lines, &ctx->lines2};
+    DevBuf* set_rev[2] = {&ctx->rev, &ctx->rev2};
+    DevBuf* set_h[2] = {&ctx->bl_h, &ctx->bl_h2};
+    DevBuf* set_q[2] = {&ctx->bl_queue, &ctx->bl_queue2};
+    DevBuf* set_scan[2] = {&ctx->scan, &ctx->scan2};
+    if (nset == 2) {
+        CCCS_TRY(ctx->lines2.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, J.max_lines)));
+        CCCS_TRY(ctx->rev2.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+        CCCS_TRY(ctx->bl_h2.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+    }
     for (int attempt = 0; attempt < 3; attempt++) {
         subcap = std::min<uint64_t>(subcap, 0xffffffffULL / kSubQueues);
-        CCCS_TRY(ctx->bl_queue.ensure(sizeof(TermaReq) * subcap * kSubQueues));
-        TermaReq* q = ctx->bl_queue.as<TermaReq>();
+        for (int k = 0; k < nset; k++) CCCS_TRY(set_q[k]->ensure(sizeof(TermaReq) * subcap * kSubQueues));
         uint32_t* qcnt = ctx->bl_qcnt.as<uint32_t>();
         hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for(J.pdose, 256)), dim3(256), 0, ctx->stream, a.pdose, J.pdose, 0.f);
         hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for((uint64_t)kSubQueues * nrays, 256)), dim3(256), 0, ctx->stream,
                            reinterpret_cast<float*>(qcnt), (uint64_t)kSubQueues * nrays, 0.f);
         CCCS_HIP(hipGetLastError());
-        for (int r = 0; r < nrays; r++) {
+        if (nset == 2) {  // the second stream starts after the tables, fills and earlier work
+            CCCS_HIP(hipEventRecord(ctx->ray_ev[2], S[0]));
+            CCCS_HIP(hipStreamWaitEvent(S[1], ctx->ray_ev[2], 0));
+        }
+        int last = -1;  // set of the last gather enqueued
+        int rc = CCCS_OK;
+        for (int r = 0; r < nrays && rc == CCCS_OK; r++) {
             a.cones = reinterpret_cast<const BlConeDev*>(d + o_c) + (size_t)r * nb;
             a.nlines = J.ray_lines[r];
             a.rowf = r % half;
             a.rowb = (int)p.ntheta - 1 - (r % half);
             a.counts = nullptr;
             if (a.nlines == 0) continue;
-            hipLaunchKernelGGL(bl_interval_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a);
-            CCCS_HIP(hipGetLastError());
-            CCCS_TRY(launch_line_scan(ctx, a.lines, a.nlines));
-            uint32_t maxn = 1;
-            for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
-            const unsigned lw = (unsigned)std::min<uint64_t>(a.nlines, (uint64_t)ctx->sched.bl_sample_waves);
-            const unsigned ww = (unsigned)std::min<uint64_t>(a.nlines, (uint64_t)ctx->sched.bl_walk_waves);
-            a.counts = prof_counter(ctx, lw);
-            int tok = prof_begin(ctx, K_BL_SAMPLE);
-            hipLaunchKernelGGL(bl_sample_kernel, dim3(lw), dim3(64), 0, ctx->stream, a, ctx->bl_h.as<float>(), q,
-                               (uint32_t)subcap, qcnt + (size_t)kSubQueues * r);
-            prof_end(ctx, tok, 0.0, 0.0, a.nlines);
-            CCCS_HIP(hipGetLastError());
-            if (!fast) {
-                const unsigned sb = kSubQueues * 64;
-                uint32_t* cnt = prof_counter(ctx, sb);
-                int ts = prof_begin(ctx, K_BL_SIDDON);
-                hipLaunchKernelGGL(bl_siddon_kernel, dim3(sb), dim3(256), 0, ctx->stream, a, q, (uint32_t)subcap,
-                                   (const uint32_t*)(qcnt + (size_t)kSubQueues * r), cnt);
-                prof_end(ctx, ts, 0.0, 0.0, 0);
-                CCCS_HIP(hipGetLastError());
-            }
-            a.counts = prof_counter(ctx, ww);
-            const size_t lds = (sizeof(double) + sizeof(float2) + 3 * sizeof(short)) * (size_t)maxn;
-            int tw = prof_begin(ctx, K_BL_WALK);
-            hipLaunchKernelGGL(bl_walk_kernel, dim3(ww), dim3(64), lds, ctx->stream, a, (const float*)ctx->bl_h.as<float>(),
-                               (int)maxn);
-            prof_end(ctx, tw, 0.0, 0.0, a.nlines);
-            CCCS_HIP(hipGetLastError());
-            a.counts = nullptr;
-            int tk = prof_begin(ctx, K_BL_GATHER);
-            hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
-            prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
-            CCCS_HIP(hipGetLastError());
+            const int k = nset == 2 ? (r & 1) : 0;
+            // the helpers (line scan, profiling events) enqueue on ctx->stream and use ctx->scan
+            ctx->stream = S[k];
+            std::swap(ctx->scan, *set_scan[k]);
+            a.lines = set_lines[k]->as<LineMeta>();
+            a.rev = set_rev[k]->as<float>();
+            float* H = set_h[k]->as<float>();
+            TermaReq* q = set_q[k]->as<TermaReq>();
+            do {
+                hipLaunchKernelGGL(bl_interval_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0,
+                                   ctx->stream, a);
+                if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "interval kernel"); break; }
+                if ((rc = launch_line_scan(ctx, a.lines, a.nlines)) != CCCS_OK) break;
+                uint32_t maxn = 1;
+                for (int bi = 0; bi < nb; bi++) maxn = std::max(maxn, (uint32_t)J.cones[(size_t)r * nb + bi].nx);
+                const unsigned lw = (unsigned)std::min<uint64_t>(a.nlines, (uint64_t)ctx->sched.bl_sample_waves);
+                const unsigned ww = (unsigned)std::min<uint64_t>(a.nlines, (uint64_t)ctx->sched.bl_walk_waves);
+                a.counts = prof_counter(ctx, lw);
+                int tok = prof_begin(ctx, K_BL_SAMPLE);
+                hipLaunchKernelGGL(bl_sample_kernel, dim3(lw), dim3(64), 0, ctx->stream, a, H, q, (uint32_t)subcap,
+                                   qcnt + (size_t)kSubQueues * r);
+                prof_end(ctx, tok, 0.0, 0.0, a.nlines);
+                if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "sample kernel"); break; }
+                if (!fast) {
+                    const unsigned sb = kSubQueues * 64;
+                    uint32_t* cnt = prof_counter(ctx, sb);
+                    int ts = prof_begin(ctx, K_BL_SIDDON);
+                    hipLaunchKernelGGL(bl_siddon_kernel, dim3(sb), dim3(256), 0, ctx->stream, a, q, (uint32_t)subcap,
+                                       (const uint32_t*)(qcnt + (size_t)kSubQueues * r), cnt);
+                    prof_end(ctx, ts, 0.0, 0.0, 0);
+                    if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "siddon kernel"); break; }
+                }
+                a.counts = prof_counter(ctx, ww);
+                const size_t lds = (sizeof(double) + sizeof(float2) + 3 * sizeof(short)) * (size_t)maxn;
+                int tw = prof_begin(ctx, K_BL_WALK);
+                hipLaunchKernelGGL(bl_walk_kernel, dim3(ww), dim3(64), lds, ctx->stream, a, (const float*)H, (int)maxn);
+                prof_end(ctx, tw, 0.0, 0.0, a.nlines);
+                if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "walk kernel"); break; }
+                a.counts = nullptr;
+                if (nset == 2 && last >= 0 && last != k &&
+                    hipStreamWaitEvent(ctx->stream, ctx->ray_ev[last], 0) != hipSuccess) {
+                    rc = set_error(CCCS_EDEVICE, "stream wait");
+                    break;
+                }
+                int tk = prof_begin(ctx, K_BL_GATHER);
+                hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
+                prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
+                if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "gather kernel"); break; }
+                if (nset == 2 && hipEventRecord(ctx->ray_ev[k], ctx->stream) != hipSuccess) {
+                    rc = set_error(CCCS_EDEVICE, "event");
+                    break;
+                }
+                last = k;
+            } while (false);
+            std::swap(ctx->scan, *set_scan[k]);
+            ctx->stream = S[0];
+        }
+        // join: the first stream continues after every direction's work
+        if (nset == 2 && last == 1) CCCS_HIP(hipStreamWaitEvent(S[0], ctx->ray_ev[1], 0));
+        if (rc != CCCS_OK) {
+            (void)hipStreamSynchronize(S[0]);
+            (void)hipStreamSynchronize(S[1]);
+            return rc;
         }
         if (fast) return CCCS_OK;
         uint32_t* hc = ctx->h_small.as<uint32_t>();

@@ -267,6 +267,10 @@ extern "C" int cccs_ctx_create(const cccs_problem* p, int device, cccs_ctx** out
     int st = CCCS_OK;
     do {
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { st = set_error(CCCS_EDEVICE, "stream"); break; }
+        if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) { st = set_error(CCCS_EDEVICE, "stream"); break; }
+        bool evok = true;
+        for (hipEvent_t& e : c->ray_ev) evok = evok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (!evok) { st = set_error(CCCS_EDEVICE, "events"); break; }
         size_t nv = (size_t)p->size[0] * p->size[1] * p->size[2];
         if (hipMalloc((void**)&c->d_dens, nv * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "density"); break; }
         if (hipMemcpy(c->d_dens, p->density, nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "density copy"); break; }
@@ -300,10 +304,12 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     if (!c) return CCCS_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->d_dens) (void)hipFree(c->d_dens);
     if (c->d_tables) (void)hipFree(c->d_tables);
     c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
     c->dose.release(); c->col.release(); c->lines.release(); c->scan.release(); c->dt.release();
+    c->lines2.release(); c->rev2.release(); c->bl_h2.release(); c->bl_queue2.release(); c->scan2.release();
     c->h_small.release(); c->h_cones.release(); c->h_stage.release();
     for (hipEvent_t& e : c->stage_ev)
         if (e) (void)hipEventDestroy(e);
@@ -314,6 +320,9 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     c->roi_tab.release(); c->roi_rank.release(); c->roi_cnt.release(); c->roi_keys.release(); c->roi_vals.release();
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (uint32_t* p : c->prof.chunks) (void)hipFree(p);
+    for (hipEvent_t e : c->ray_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CCCS_OK;
@@ -334,6 +343,11 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
     if (std::strcmp(key, "cone_ramp_min") == 0) {
         if (value < 1) return set_error(CCCS_EINVAL, "cone_ramp_min must be >= 1");
         c->sched.cone_ramp_min = value;
+        return CCCS_OK;
+    }
+    if (std::strcmp(key, "bl_streams") == 0) {
+        if (value != 1 && value != 2) return set_error(CCCS_EINVAL, "bl_streams must be 1 or 2");
+        c->sched.bl_streams = value;
         return CCCS_OK;
     }
     if (std::strcmp(key, "bl_sample_waves") == 0 || std::strcmp(key, "bl_walk_waves") == 0) {
@@ -367,6 +381,7 @@ extern "C" int cccs_ctx_get_schedule(const cccs_ctx* c, const char* key, int* va
     if (std::strcmp(key, "cone_ramp_min") == 0) { *value = S.cone_ramp_min; return CCCS_OK; }
     if (std::strcmp(key, "bl_sample_waves") == 0) { *value = S.bl_sample_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_walk_waves") == 0) { *value = S.bl_walk_waves; return CCCS_OK; }
+    if (std::strcmp(key, "bl_streams") == 0) { *value = S.bl_streams; return CCCS_OK; }
     for (int i = 0; i < kConeClasses; i++) {
         if (("cone_class_threads" + std::to_string(i)) == key) { *value = S.cone_class_threads[i]; return CCCS_OK; }
         if (i < kConeClasses - 1 && ("cone_class_len" + std::to_string(i)) == key) {
@@ -472,6 +487,7 @@ extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
     if (!ctx) return set_error(CCCS_EINVAL, "null argument");
     CCCS_HIP(hipSetDevice(ctx->device));
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
+    CCCS_HIP(hipStreamSynchronize(ctx->stream2));
     return CCCS_OK;
 }
 

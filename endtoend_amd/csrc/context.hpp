@@ -107,12 +107,19 @@ struct Schedule {
     // Siddon stage more coherent neighbouring requests
     int bl_sample_waves = 262144;
     int bl_walk_waves = 32768;
+    // cone directions of the beamlet path in flight at once (1 or 2): direction r+1's sample,
+    // Siddon and walk overlap direction r's on a second stream; the gathers stay in order
+    int bl_streams = 2;
 };
 }  // namespace cccs
 
 struct cccs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the beamlet path's two-ray pipeline, and its per-ray events (created with
+    // the context; default priority like the first)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ray_ev[3] = {nullptr, nullptr, nullptr};
     cccs_problem prob{};  // scalar copy (table pointers cleared)
     cccs::VolDesc vol{};
     cccs::SpecDev spec{};
@@ -123,6 +130,9 @@ struct cccs_ctx {
     float* d_tables = nullptr;  // kernel (ntheta*nradii) | radii | inverse widths | mean radii | inverse widths
     size_t resident = 0;
     cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan, dt;
+    // the second ray set of the beamlet pipeline: line table, REV store, half steps, TERMA queue,
+    // scan partials
+    cccs::DevBuf lines2, rev2, bl_h2, bl_queue2, scan2;
     cccs::HostBuf h_small, h_cones, h_stage;  // h_stage: pinned double buffer of download_staged
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // beamlet path: job tables, packed grids, unit-TERMA grids, column arena, sparse output
