@@ -1366,22 +1366,26 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     CCCS_HIP(hipGetLastError());
     a.tile_batch = ctx->bl_tiles.as<uint32_t>();
     const int half = (int)p.ntheta / 2;
-    // Two cone directions in flight (bl_streams = 2): direction r runs on stream r % 2 with its own
-    // line table, REV store, half steps, TERMA queue and scan partials, so direction r+1's sample,
-    // Siddon and walk overlap direction r's.  The gathers read-modify-write the shared packed
-    // grids, so gather r waits for gather r-1 (an event).  A set is reused by direction r+2 on the
+    // Several cone directions in flight (bl_streams = n): direction r runs on stream r % n with its
+    // own line table, REV store, half steps, TERMA queue and scan partials, so consecutive
+    // directions' sample, Siddon and walk overlap.  The gathers read-modify-write the shared packed
+    // grids, so gather r waits for gather r-1 (an event).  A set is reused by direction r+n on the
     // same stream, after direction r's gather in stream order.
-    const int nset = ctx->sched.bl_streams == 2 ? 2 : 1;
-    hipStream_t S[2] = {ctx->stream, ctx->stream2};
-    DevBuf* set_lines[2] = {&ctx->lines, &ctx->lines2};
-    DevBuf* set_rev[2] = {&ctx->rev, &ctx->rev2};
-    DevBuf* set_h[2] = {&ctx->bl_h, &ctx->bl_h2};
-    DevBuf* set_q[2] = {&ctx->bl_queue, &ctx->bl_queue2};
-    DevBuf* set_scan[2] = {&ctx->scan, &ctx->scan2};
-    if (nset == 2) {
-        CCCS_TRY(ctx->lines2.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, J.max_lines)));
-        CCCS_TRY(ctx->rev2.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
-        CCCS_TRY(ctx->bl_h2.ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+    const int nset = std::min(std::max(ctx->sched.bl_streams, 1), kRaySets);
+    hipStream_t S[kRaySets];
+    DevBuf *set_lines[kRaySets], *set_rev[kRaySets], *set_h[kRaySets], *set_q[kRaySets], *set_scan[kRaySets];
+    S[0] = ctx->stream;
+    set_lines[0] = &ctx->lines; set_rev[0] = &ctx->rev; set_h[0] = &ctx->bl_h; set_q[0] = &ctx->bl_queue;
+    set_scan[0] = &ctx->scan;
+    for (int k = 1; k < kRaySets; k++) {
+        S[k] = ctx->xstream[k - 1];
+        set_lines[k] = &ctx->xlines[k - 1]; set_rev[k] = &ctx->xrev[k - 1]; set_h[k] = &ctx->xh[k - 1];
+        set_q[k] = &ctx->xqueue[k - 1]; set_scan[k] = &ctx->xscan[k - 1];
+    }
+    for (int k = 1; k < nset; k++) {
+        CCCS_TRY(set_lines[k]->ensure(sizeof(LineMeta) * std::max<uint64_t>(1, J.max_lines)));
+        CCCS_TRY(set_rev[k]->ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
+        CCCS_TRY(set_h[k]->ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
     }
     for (int attempt = 0; attempt < 3; attempt++) {
         subcap = std::min<uint64_t>(subcap, 0xffffffffULL / kSubQueues);
@@ -1391,9 +1395,9 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
         hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for((uint64_t)kSubQueues * nrays, 256)), dim3(256), 0, ctx->stream,
                            reinterpret_cast<float*>(qcnt), (uint64_t)kSubQueues * nrays, 0.f);
         CCCS_HIP(hipGetLastError());
-        if (nset == 2) {  // the second stream starts after the tables, fills and earlier work
-            CCCS_HIP(hipEventRecord(ctx->ray_ev[2], S[0]));
-            CCCS_HIP(hipStreamWaitEvent(S[1], ctx->ray_ev[2], 0));
+        if (nset > 1) {  // the other streams start after the tables, fills and earlier work
+            CCCS_HIP(hipEventRecord(ctx->ray_ev[kRaySets], S[0]));
+            for (int k = 1; k < nset; k++) CCCS_HIP(hipStreamWaitEvent(S[k], ctx->ray_ev[kRaySets], 0));
         }
         int last = -1;  // set of the last gather enqueued
         int rc = CCCS_OK;
@@ -1404,7 +1408,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             a.rowb = (int)p.ntheta - 1 - (r % half);
             a.counts = nullptr;
             if (a.nlines == 0) continue;
-            const int k = nset == 2 ? (r & 1) : 0;
+            const int k = r % nset;
             // the helpers (line scan, profiling events) enqueue on ctx->stream and use ctx->scan
             ctx->stream = S[k];
             std::swap(ctx->scan, *set_scan[k]);
@@ -1443,7 +1447,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                 prof_end(ctx, tw, 0.0, 0.0, a.nlines);
                 if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "walk kernel"); break; }
                 a.counts = nullptr;
-                if (nset == 2 && last >= 0 && last != k &&
+                if (nset > 1 && last >= 0 && last != k &&
                     hipStreamWaitEvent(ctx->stream, ctx->ray_ev[last], 0) != hipSuccess) {
                     rc = set_error(CCCS_EDEVICE, "stream wait");
                     break;
@@ -1452,7 +1456,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                 hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
                 prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
                 if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "gather kernel"); break; }
-                if (nset == 2 && hipEventRecord(ctx->ray_ev[k], ctx->stream) != hipSuccess) {
+                if (nset > 1 && hipEventRecord(ctx->ray_ev[k], ctx->stream) != hipSuccess) {
                     rc = set_error(CCCS_EDEVICE, "event");
                     break;
                 }
@@ -1461,11 +1465,11 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             std::swap(ctx->scan, *set_scan[k]);
             ctx->stream = S[0];
         }
-        // join: the first stream continues after every direction's work
-        if (nset == 2 && last == 1) CCCS_HIP(hipStreamWaitEvent(S[0], ctx->ray_ev[1], 0));
+        // join: the first stream continues after every direction's work (the gathers are chained,
+        // and no stream has work after its last gather)
+        if (nset > 1 && last > 0) CCCS_HIP(hipStreamWaitEvent(S[0], ctx->ray_ev[last], 0));
         if (rc != CCCS_OK) {
-            (void)hipStreamSynchronize(S[0]);
-            (void)hipStreamSynchronize(S[1]);
+            for (int k = 0; k < nset; k++) (void)hipStreamSynchronize(S[k]);
             return rc;
         }
         if (fast) return CCCS_OK;
@@ -2039,6 +2043,7 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     }
     const size_t nvox = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
     const size_t budget = size_t(24) << 30;  // device bytes per job
+    const int nset = std::min(std::max(ctx->sched.bl_streams, 1), kRaySets);
     size_t next = 0;
     // device footprint of each batch, additive per cone direction: a job needs its batches' packed
     // grids and sparse-output bounds, plus one direction's REV store, half steps and line table
@@ -2059,7 +2064,9 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
                 mr = std::max(mr, rev[r] + b_rev[k][r]);
                 ml = std::max(ml, lines[r] + b_lines[k][r]);
             }
-            const uint64_t bytes = fixed + b_fixed[k] + sizeof(float) * 2 * mr + sizeof(LineMeta) * ml;
+            // every direction in flight holds its own REV store, half steps and line table
+            const uint64_t bytes =
+                fixed + b_fixed[k] + (uint64_t)nset * (sizeof(float) * 2 * mr + sizeof(LineMeta) * ml);
             if (k > next && bytes > budget) break;
             for (int r = 0; r < nrays; r++) {
                 rev[r] += b_rev[k][r];

@@ -267,7 +267,9 @@ extern "C" int cccs_ctx_create(const cccs_problem* p, int device, cccs_ctx** out
     int st = CCCS_OK;
     do {
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { st = set_error(CCCS_EDEVICE, "stream"); break; }
-        if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) { st = set_error(CCCS_EDEVICE, "stream"); break; }
+        bool sok = true;
+        for (hipStream_t& x : c->xstream) sok = sok && hipStreamCreateWithFlags(&x, hipStreamNonBlocking) == hipSuccess;
+        if (!sok) { st = set_error(CCCS_EDEVICE, "stream"); break; }
         bool evok = true;
         for (hipEvent_t& e : c->ray_ev) evok = evok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
         if (!evok) { st = set_error(CCCS_EDEVICE, "events"); break; }
@@ -304,12 +306,15 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     if (!c) return CCCS_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    for (hipStream_t x : c->xstream)
+        if (x) (void)hipStreamSynchronize(x);
     if (c->d_dens) (void)hipFree(c->d_dens);
     if (c->d_tables) (void)hipFree(c->d_tables);
     c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
     c->dose.release(); c->col.release(); c->lines.release(); c->scan.release(); c->dt.release();
-    c->lines2.release(); c->rev2.release(); c->bl_h2.release(); c->bl_queue2.release(); c->scan2.release();
+    for (int i = 0; i < kRaySets - 1; i++) {
+        c->xlines[i].release(); c->xrev[i].release(); c->xh[i].release(); c->xqueue[i].release(); c->xscan[i].release();
+    }
     c->h_small.release(); c->h_cones.release(); c->h_stage.release();
     for (hipEvent_t& e : c->stage_ev)
         if (e) (void)hipEventDestroy(e);
@@ -322,7 +327,8 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     for (uint32_t* p : c->prof.chunks) (void)hipFree(p);
     for (hipEvent_t e : c->ray_ev)
         if (e) (void)hipEventDestroy(e);
-    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    for (hipStream_t x : c->xstream)
+        if (x) (void)hipStreamDestroy(x);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CCCS_OK;
@@ -346,7 +352,7 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
         return CCCS_OK;
     }
     if (std::strcmp(key, "bl_streams") == 0) {
-        if (value != 1 && value != 2) return set_error(CCCS_EINVAL, "bl_streams must be 1 or 2");
+        if (value < 1 || value > kRaySets) return set_error(CCCS_EINVAL, "bl_streams must be in [1, 4]");
         c->sched.bl_streams = value;
         return CCCS_OK;
     }
@@ -487,7 +493,7 @@ extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
     if (!ctx) return set_error(CCCS_EINVAL, "null argument");
     CCCS_HIP(hipSetDevice(ctx->device));
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    CCCS_HIP(hipStreamSynchronize(ctx->stream2));
+    for (hipStream_t x : ctx->xstream) CCCS_HIP(hipStreamSynchronize(x));
     return CCCS_OK;
 }
 

@@ -144,9 +144,9 @@ int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scr
  *                    upper stored-line lengths of the first three cone-kernel launch classes
  *                    (defaults 96, 160, 320; the last class takes the longer lines).  Each class
  *                    launch sizes its LDS for its longest line.
- *   "bl_streams"     cone directions of the beamlet path in flight at once, 1 or 2 (default 2:
- *                    direction r+1's sample, Siddon and walk overlap direction r's on a second
- *                    stream; the gathers stay in direction order).
+ *   "bl_streams"     cone directions of the beamlet path in flight at once, 1 to 4 (default 4:
+ *                    consecutive directions' sample, Siddon and walk overlap on their own
+ *                    streams; the gathers stay in direction order).
  *   "bl_sample_waves", "bl_walk_waves"
  *                    waves of the beamlet path's persistent per-line kernels (defaults 262144 and
  *                    32768; each wave takes a stride of lines).

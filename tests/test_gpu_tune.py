@@ -43,7 +43,7 @@ def test_autotune_beamlet_path_keeps_the_columns():
 
 
 def test_two_ray_pipeline_gives_the_same_columns():
-    """bl_streams = 2 overlaps consecutive cone directions on two streams; the columns must be
+    """bl_streams = n overlaps consecutive cone directions on n streams; the columns must be
     identical to the one-stream pipeline, on a multi-beam job with every mode that uses it."""
     w = W.small(n=32, voxel=0.5, gantry=215.0, couch=-45.0, hetero=True, fmap=(8, 8), beamlet=(0.8, 0.8),
                 weights="rand", nphi=8, ntheta=8)
@@ -53,12 +53,13 @@ def test_two_ray_pipeline_gives_the_same_columns():
     print("[case] one vs two streams", flush=True)
     with Engine(w.spec) as eng:
         out = {}
-        for ns in (1, 2):
+        for ns in (1, 2, 3, 4):
             eng.set_schedule("bl_streams", ns)
             out[ns] = [eng.beamlet_dose(beams, fmaps, threshold=1e-4, mode=m)
                        for m in (abi.CCCS_BEAMLET_FAITHFUL, abi.CCCS_BEAMLET_FAST)]
-    for c1, c2 in zip(out[1], out[2]):
-        assert len(c1) == len(c2) > 10
-        for a, b in zip(c1, c2):
-            assert a[:2] == b[:2] and np.array_equal(a[2], b[2])
-            assert np.array_equal(a[3].view(np.uint32), b[3].view(np.uint32))
+    for ns in (2, 3, 4):
+        for c1, c2 in zip(out[1], out[ns]):
+            assert len(c1) == len(c2) > 10
+            for a, b in zip(c1, c2):
+                assert a[:2] == b[:2] and np.array_equal(a[2], b[2])
+                assert np.array_equal(a[3].view(np.uint32), b[3].view(np.uint32))
