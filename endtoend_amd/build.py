@@ -1,4 +1,5 @@
-"""Build the CCCS HIP library (endtoend_amd/lib/libcccs.so) for gfx950 with hipcc.
+"""Build the CCCS HIP library (endtoend_amd/lib/libcccs.so) for gfx950 with hipcc, and the
+host-only HDF5 file layer (endtoend_amd/lib/libcccs_h5.so) with g++ where HDF5 is installed.
 
 Every source is compiled with -ffp-contract=off (the Siddon segment lists must match the host
 oracle bit for bit) and with packed FP32 disabled.  Objects go to endtoend_amd/_build/, the
@@ -54,8 +55,48 @@ def _compile(src: str, force: bool) -> str:
     return obj
 
 
+H5_LIB = os.path.join(LIBDIR, "libcccs_h5.so")
+H5_SOURCE = os.path.join(CSRC, "h5io.cpp")
+
+
+def hdf5_root() -> str | None:
+    """Prefix holding include/hdf5.h and lib/libhdf5.so (CCCS_HDF5_ROOT, else /opt/conda or /usr)."""
+    for root in [os.environ.get("CCCS_HDF5_ROOT"), "/opt/conda", "/usr"]:
+        if root and os.path.exists(os.path.join(root, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(root, "lib", "libhdf5.so")):
+            return root
+    return None
+
+
+def build_h5(force: bool = True, verbose: bool = False) -> str | None:
+    """Compile the host-only HDF5 file layer (libcccs_h5.so) against the HDF5 C library.
+
+    libstdc++ is linked statically so that the HDF5 prefix's own C++ runtime is never picked up
+    through the run path.  Returns None (and builds nothing) when no HDF5 installation is found."""
+    root = hdf5_root()
+    if root is None:
+        if verbose:
+            print("libcccs_h5.so skipped: no HDF5 headers/library (set CCCS_HDF5_ROOT)")
+        return None
+    deps = [H5_SOURCE, os.path.join(ROOT, "include", "cccs.h"), os.path.join(ROOT, "include", "cccs_h5.h")]
+    if not force and not _stale(H5_LIB, deps):
+        return H5_LIB
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC", "-static-libstdc++", "-static-libgcc",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(root, "include"), H5_SOURCE,
+           "-L", os.path.join(root, "lib"), "-lhdf5", f"-Wl,-rpath,{os.path.join(root, 'lib')}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5io.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print(f"built {H5_LIB}")
+    return H5_LIB
+
+
 def build(force: bool = True, verbose: bool = False) -> str:
-    """Compile every source and link libcccs.so; returns the library path."""
+    """Compile every source and link libcccs.so (and libcccs_h5.so); returns the HIP library path."""
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -69,6 +110,7 @@ def build(force: bool = True, verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_h5(force=force, verbose=verbose)
     return LIB
 
 

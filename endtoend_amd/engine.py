@@ -381,6 +381,21 @@ class Engine:
                   "cccs_beamlet_dose")
         return cols
 
+    def beamlet_dose_h5(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray], coeff_file, uid_base: int = 0,
+                        labels=None, threshold: float = 1e-4, mode: int = abi.CCCS_BEAMLET_FAITHFUL) -> None:
+        """Columns of every active beamlet written straight into an open h5.CoeffFile (the native
+        HDF5 sink; no Python per column): beam i of this request becomes beam uid uid_base + i."""
+        coeff_file.add_beams(beams, fmaps, uid_base=uid_base, labels=labels)
+        fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]
+        ptrs = (abi._fp * len(fm))(*[abi.fptr(f) for f in fm])
+        rc = self.lib.cccs_beamlet_dose(self.ctx, beams_array(beams), len(beams), ptrs, threshold, mode,
+                                        coeff_file.sink, C.c_void_p(coeff_file.handle))
+        if rc != abi.CCCS_OK:
+            from . import h5
+            detail = h5.load().cccs_h5_last_error().decode(errors="replace")
+            msg = self.lib.cccs_last_error().decode(errors="replace")
+            raise abi.CCCSError(rc, "cccs_beamlet_dose", f"{msg}{'; ' + detail if detail else ''}")
+
 # ---- the reference's file formats (host functions of the library; no device needed) ----------
 
 def load_beam_list(path: str, requested: int = 0) -> list:

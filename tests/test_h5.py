@@ -1,0 +1,335 @@
+"""The HDF5 file layer (libcccs_h5.so), checked type for type against the reference's writers by an
+independent ctypes walk of the files (tests/h5_walk.py):
+
+* Dose.h5: Volume<float>::writeToFile + _write_file_version (DoseCalcIO/volume.h:118-150,
+  fmaps_io.cpp:168-184);
+* Dose_Coefficients.h5: SparsifyManager::init, write_beam_metadata, write_beamlet
+  (dosecalc-beamlet/sparsify_manager.cpp:11-13,44-120,176-221) with _write_patient_metadata,
+  _write_beam_metadata, _write_beamlet_metadata, _write_beamlet_to_hdf5 (sparse_io.cpp:178-326) and
+  BEAM::getFileCompoundType / _writeToHDF5 (beam.cpp:134-163,248-298); the coefficient matrix is
+  rebuilt the way read_dose_data.py:34-133 does and compared with the columns written;
+* fmaps.h5: Fmaps.generate (extern/python/fmaps.py:85-123), read back as read_fmaps does
+  (fmaps_io.cpp:10-165);
+* roi_list.h5: ROIMaskList / DenseROIMask / ArrayProps (roi.cpp:16-185, io_data_structs.h:80-134).
+"""
+import numpy as np
+import pytest
+
+from endtoend_amd import abi
+from endtoend_amd import build as B
+from endtoend_amd import engine as E
+from endtoend_amd import workloads as W
+
+pytestmark = pytest.mark.skipif(B.hdf5_root() is None, reason="no HDF5 C library in this image")
+
+if B.hdf5_root() is not None:
+    from endtoend_amd import h5 as H
+    from tests import h5_walk as HW
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    B.build_h5(force=False)
+
+
+def _spec():
+    ph = W.water_cube(24, 0.5)
+    return W.problem(ph, nphi=4, ntheta=4)
+
+
+def _columns(rng, nvox, beam_index, beamlets, empty=()):
+    cols = []
+    for b in beamlets:
+        n = 0 if b in empty else int(rng.integers(1, 200))
+        keys = np.sort(rng.choice(nvox, n, replace=False)).astype(np.uint64)
+        vals = rng.random(n, dtype=np.float32) + np.float32(1e-3)
+        cols.append((beam_index, b, keys, vals, float(rng.random()), float(rng.random())))
+    return cols
+
+
+# ---- Dose.h5 -------------------------------------------------------------------------------------------
+def test_volume_layout_and_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    vol = rng.random((5, 7, 9), dtype=np.float32)
+    p = str(tmp_path / "Dose.h5")
+    H.write_volume(p, vol, start=(-1.5, 2.0, 0.25), voxel=(0.2, 0.25, 0.3))
+    root = HW.walk(p)
+    assert set(root.groups) == {"filetype"} and set(root.datasets) == {"volume"}
+    ft = root.groups["filetype"].attrs
+    assert (ft["ftmagic"].value, ft["ftversionmajor"].value, ft["ftversionminor"].value) == (0x2A, 1, 0)
+    assert all(a.dtype == "|u1" and a.scalar for a in ft.values())
+    d = root.datasets["volume"]
+    assert d.dtype == "<f4" and d.layout == "contiguous" and d.data.shape == (5, 7, 9)
+    assert np.array_equal(d.data, vol)
+    for name, want in (("dicom_start_cm", (-1.5, 2.0, 0.25)), ("voxel_size_cm", (0.2, 0.25, 0.3))):
+        a = d.attrs[name]
+        assert a.dtype == "<f4" and not a.scalar and a.shape == (3,)
+        assert np.array_equal(a.value, np.array(want, np.float32))
+    got, start, voxel = H.read_volume(p, "volume")
+    assert np.array_equal(got, vol) and np.array_equal(start, np.float32([-1.5, 2.0, 0.25]))
+    assert np.array_equal(voxel, np.float32([0.2, 0.25, 0.3]))
+    # the reference reader's default dataset name is "data", not the writer's "volume"
+    with pytest.raises(abi.CCCSError):
+        H.read_volume(p)
+    with pytest.raises(abi.CCCSError):
+        H.read_volume(str(tmp_path / "missing.h5"), "volume")
+
+
+# ---- Dose_Coefficients.h5 -------------------------------------------------------------------------------
+BEAM_MEMBERS = [("uid", "<u2", 0), ("gantry_rot_rad", "<f4", 4), ("couch_rot_rad", "<f4", 8),
+                ("coll_rot_rad", "<f4", 12), ("src_coords_cm", "<f4", 16), ("direction", "<f4", 28),
+                ("iso_coords_cm", "<f4", 40), ("fmap_dims", "<u4", 52), ("beamlet_size_cm", "<f4", 60)]
+
+
+def _check_beam_group(g, beam, fmap, uid, labels):
+    a = g.attrs
+    assert a["beam_uid"].dtype == "<u2" and int(a["beam_uid"].value) == uid
+    assert a["N_beamlets"].dtype == "<u2" and int(a["N_beamlets"].value) == int((fmap > 0).sum())
+    for k, v in zip(("isocenter_type", "isocenter_location", "orientation_type"), labels):
+        assert a[k].value == v and a[k].dtype == f"S{len(v) + 1}z" and a[k].scalar
+    fw = a["fmap_weights"]
+    assert fw.dtype == "<f4" and fw.shape == (beam.fmap_size[0], beam.fmap_size[1])
+    assert np.array_equal(fw.value.ravel(), fmap.ravel())  # the map's own order, x fastest
+    spec = a["beam_specs"]
+    assert spec.scalar and spec.dtype == "compound"
+    rec = spec.value
+    dt = rec.dtype
+    assert dt.itemsize == 68 and list(dt.names) == [m for m, _, _ in BEAM_MEMBERS]
+    for name, base, off in BEAM_MEMBERS:
+        assert dt.fields[name][1] == off
+        sub = dt.fields[name][0]
+        assert (sub.subdtype[0] if sub.subdtype else sub).str == base
+    assert int(rec["uid"]) == uid
+    assert np.float32(rec["gantry_rot_rad"]) == np.float32(beam.azimuth)
+    assert np.float32(rec["couch_rot_rad"]) == np.float32(beam.zenith)
+    assert np.float32(rec["coll_rot_rad"]) == np.float32(beam.coll)
+    assert np.array_equal(rec["src_coords_cm"], np.float32(beam.source[:]))
+    assert np.array_equal(rec["direction"], np.float32(beam.direction[:]))
+    assert np.array_equal(rec["iso_coords_cm"], np.float32(beam.isocenter[:]))
+    assert list(rec["fmap_dims"]) == list(beam.fmap_size[:])
+    assert np.array_equal(rec["beamlet_size_cm"], np.float32(beam.beamlet_size[:]))
+
+
+def _coeff_case(rng):
+    beams = [E.make_beam(g, c, 0.0, fmap_size=fs, beamlet_size=(0.5, 0.5))
+             for g, c, fs in ((0.0, 0.0, (4, 3)), (90.0, 10.0, (5, 5)), (200.0, -30.0, (6, 2)))]
+    fmaps = [rng.random((fs[1], fs[0]), dtype=np.float32) for fs in ((4, 3), (5, 5), (6, 2))]
+    fmaps[0].ravel()[[1, 5]] = 0.0
+    fmaps[1][:] = 0.0  # no active beamlet: no groups at all
+    return beams, fmaps
+
+
+def test_coefficient_file_layout_and_matrix(tmp_path):
+    rng = np.random.default_rng(7)
+    spec = _spec()
+    p = spec.struct()
+    nvox = int(np.prod(spec.size))
+    beams, fmaps = _coeff_case(rng)
+    path = str(tmp_path / "Dose_Coefficients.h5")
+    cols = (_columns(rng, nvox, 0, [0, 2, 3, 4, 6, 11], empty=(3,)) + _columns(rng, nvox, 2, [0, 1, 7, 8, 11]))
+    labels = [("ptv", "in", "auto"), ("man", "out", "man"), (None, None, None)]
+    with H.CoeffFile(path, p, nbeams=3, sparsity_thresh=1e-4, beam_spectrum="spec_6mv",
+                     target_structure="PTV", version="0.9.1") as f:
+        f.add_beams(beams, fmaps, uid_base=0, labels=labels)
+        for c in cols:
+            f.add_column(*c)
+        written = [c for c in cols if len(c[2])]
+        assert f.counts() == (len(written), sum(len(c[2]) for c in written))
+    root = HW.walk(path)
+    assert set(root.groups) == {"filetype", "calc_specs", "beams"}
+    ft = root.groups["filetype"].attrs
+    assert (ft["ftmagic"].value, ft["ftversionmajor"].value, ft["ftversionminor"].value) == (0x2B, 1, 2)
+    cs = root.groups["calc_specs"].attrs
+    expect = {"dosecalc_version": "S6z", "N_beams": "<u2", "dicom_start_cm": "<f4[3]", "full_dicom_size": "<u2[3]",
+              "voxel_size_cm": "<f4[3]", "convlat_cm": "<f4", "convstep_cm": "<f4", "calc_bbox_start": "<u2[3]",
+              "calc_bbox_size": "<u2[3]", "penumbra_cm": "<f4", "sparsity_thresh": "<f4", "nphi": "<u2",
+              "ntheta": "<u2", "nradii": "<u2", "kernel_extent_cm": "<f4", "beam_spectrum": "S9z",
+              "target_structure": "S4z"}
+    assert {k: a.dtype for k, a in cs.items()} == expect
+    assert all(a.scalar for a in cs.values())
+    assert cs["dosecalc_version"].value == "0.9.1" and int(cs["N_beams"].value) == 3
+    assert list(cs["full_dicom_size"].value) == list(p.size) and list(cs["calc_bbox_size"].value) == list(p.bbox_size)
+    assert list(cs["calc_bbox_start"].value) == list(p.bbox_start)
+    assert np.array_equal(cs["dicom_start_cm"].value, np.float32(p.start[:]))
+    assert np.array_equal(cs["voxel_size_cm"].value, np.float32(p.voxel[:]))
+    assert (cs["convlat_cm"].value, cs["convstep_cm"].value) == (np.float32(p.rev_latspacing), np.float32(p.rev_longspacing))
+    assert (int(cs["nphi"].value), int(cs["ntheta"].value), int(cs["nradii"].value)) == (p.nphi, p.ntheta, p.nradii)
+    assert cs["penumbra_cm"].value == np.float32(p.penumbra) and cs["kernel_extent_cm"].value == np.float32(p.kernel_extent)
+    assert cs["sparsity_thresh"].value == np.float32(1e-4)
+    bg = root.groups["beams"]
+    assert set(bg.groups) == {"metadata", "data"} and not bg.attrs
+    assert set(bg.groups["metadata"].groups) == {"beam_00000", "beam_00002"}
+    assert set(bg.groups["data"].groups) == {"beam_00000", "beam_00002"}
+    _check_beam_group(bg.groups["metadata"].groups["beam_00000"], beams[0], fmaps[0], 0, labels[0])
+    _check_beam_group(bg.groups["metadata"].groups["beam_00002"], beams[2], fmaps[2], 2, ("unspec", "unspec", "unknown"))
+    # beamlet groups: only non-empty columns
+    for bi, beamlets in ((0, [0, 2, 4, 6, 11]), (2, [0, 1, 7, 8, 11])):
+        g = bg.groups["data"].groups[f"beam_{bi:05d}"]
+        assert sorted(g.groups) == [f"beamlet_{b:05d}" for b in beamlets]
+    for bi, bl, keys, vals, pnz, pdr in cols:
+        if not len(keys):
+            continue
+        g = bg.groups["data"].groups[f"beam_{bi:05d}"].groups[f"beamlet_{bl:05d}"]
+        assert {k: (a.dtype, a.scalar) for k, a in g.attrs.items()} == {
+            "beamlet_uid": ("<u2", True), "N_coeffs": ("<u8", True), "perc_nonzero": ("<f4", True),
+            "perc_dropped": ("<f4", True)}
+        assert int(g.attrs["beamlet_uid"].value) == bl and int(g.attrs["N_coeffs"].value) == len(keys)
+        assert g.attrs["perc_nonzero"].value == np.float32(pnz) and g.attrs["perc_dropped"].value == np.float32(pdr)
+        assert set(g.datasets) == {"lindex", "coeffs"}
+        li, co = g.datasets["lindex"], g.datasets["coeffs"]
+        assert (li.dtype, co.dtype, li.layout, co.layout) == ("<u8", "<f4", "contiguous", "contiguous")
+        assert np.array_equal(li.data, keys) and np.array_equal(co.data.view(np.uint32), vals.view(np.uint32))
+    # the reference reader's matrix: columns in beam then beamlet order, rows = lindex
+    A, col_idx = HW.coefficient_matrix(root)
+    want = [(b, l) for b, l, k, *_ in sorted(cols, key=lambda c: (c[0], c[1])) if len(k)]
+    assert col_idx == want and A.shape == (nvox, len(want))
+    for j, (b, l) in enumerate(want):
+        _, _, k, v, _, _ = next(c for c in cols if c[:2] == (b, l))
+        dense = np.zeros(nvox, np.float32)
+        dense[k.astype(np.int64)] = v
+        assert np.array_equal(A[:, j], dense)
+
+
+def test_reduced_coefficient_file(tmp_path):
+    rng = np.random.default_rng(3)
+    spec = _spec()
+    beams, fmaps = _coeff_case(rng)
+    names, caps = ["PTV", "Cord", "Body_external"], [40, 25, 100]
+    path = str(tmp_path / "reduced.h5")
+    with H.CoeffFile(path, spec.struct(), nbeams=3, rois=(names, caps)) as f:
+        f.add_beams(beams, fmaps)
+        blocks = np.array([3, 0, 5], np.uint64)
+        keys = np.concatenate([np.arange(3), 40 + np.arange(0), 65 + np.array([1, 4, 9, 50, 99])]).astype(np.uint64)
+        f.add_column(0, 4, keys, np.arange(1, 9, dtype=np.float32), 1.0, 2.0, blocks=blocks, perc_reducedrop=37.5)
+    root = HW.walk(path)
+    cs = root.groups["calc_specs"].attrs
+    assert cs["row_block_capacities"].dtype == "<u8" and cs["row_block_capacities"].shape == (3,)
+    assert list(cs["row_block_capacities"].value) == caps
+    assert cs["roi_order"].dtype == "vlen-str" and cs["roi_order"].value == names
+    g = root.groups["beams"].groups["data"].groups["beam_00000"].groups["beamlet_00004"]
+    assert g.attrs["row_block_sizes"].dtype == "<u8" and list(g.attrs["row_block_sizes"].value) == [3, 0, 5]
+    assert g.attrs["perc_reducedrop"].dtype == "<f4" and g.attrs["perc_reducedrop"].value == np.float32(37.5)
+    A, col_idx = HW.coefficient_matrix(root)
+    assert A.shape == (sum(caps), 1) and col_idx == [(0, 4)]
+    assert np.array_equal(np.nonzero(A[:, 0])[0], keys.astype(np.int64))
+
+
+def test_compressed_chunked_datasets(tmp_path):
+    rng = np.random.default_rng(5)
+    spec = _spec()
+    beams, fmaps = _coeff_case(rng)
+    path = str(tmp_path / "deflate.h5")
+    keys = np.arange(0, 3000, 7, dtype=np.uint64)
+    vals = rng.random(keys.size, dtype=np.float32)
+    with H.CoeffFile(path, spec.struct(), nbeams=3, compress_lvl=12, chunksize=50) as f:
+        f.add_beams(beams, fmaps)
+        f.add_column(0, 0, keys, vals)
+        f.add_column(0, 2, keys[:20], vals[:20])
+    bg = HW.walk(path).groups["beams"].groups["data"].groups["beam_00000"]
+    for name, n in (("beamlet_00000", keys.size), ("beamlet_00002", 20)):
+        for ds in bg.groups[name].datasets.values():
+            assert ds.layout == "chunked" and ds.chunk == (min(n, 50),) and ds.nfilters == 1
+        assert np.array_equal(bg.groups[name].datasets["lindex"].data, keys[:n])
+        assert np.array_equal(bg.groups[name].datasets["coeffs"].data, vals[:n])
+
+
+def test_coefficient_file_rejects_bad_input(tmp_path):
+    rng = np.random.default_rng(9)
+    spec = _spec()
+    beams, fmaps = _coeff_case(rng)
+    with H.CoeffFile(str(tmp_path / "bad.h5"), spec.struct(), nbeams=3) as f:
+        with pytest.raises(abi.CCCSError):  # no beams declared yet
+            f.add_column(0, 0, np.arange(3, dtype=np.uint64), np.ones(3, np.float32))
+        f.add_beams(beams, fmaps)
+        with pytest.raises(abi.CCCSError):  # beam 1 has no active beamlet
+            f.add_column(1, 0, np.arange(3, dtype=np.uint64), np.ones(3, np.float32))
+        with pytest.raises(abi.CCCSError):  # beyond the request
+            f.add_column(3, 0, np.arange(3, dtype=np.uint64), np.ones(3, np.float32))
+        f.add_column(0, 0, np.arange(3, dtype=np.uint64), np.ones(3, np.float32))
+        with pytest.raises(abi.CCCSError):  # the same beamlet twice
+            f.add_column(0, 0, np.arange(3, dtype=np.uint64), np.ones(3, np.float32))
+        with pytest.raises(abi.CCCSError):  # beamlet uid beyond 16 bits
+            f.add_column(0, 70000, np.arange(3, dtype=np.uint64), np.ones(3, np.float32))
+        assert f.counts() == (1, 3)
+    with pytest.raises(abi.CCCSError):
+        H.CoeffFile(str(tmp_path / "bad2.h5"), spec.struct(), nbeams=70000)
+
+
+# ---- fmaps.h5 ------------------------------------------------------------------------------------------
+def test_fmaps_file_and_reader(tmp_path):
+    rng = np.random.default_rng(11)
+    beams, fmaps = _coeff_case(rng)
+    fmaps[2].ravel()[0] = -0.5  # count_nonzero counts it; the coefficient file's N_beamlets would not
+    path = str(tmp_path / "fmaps.h5")
+    H.write_fmaps(path, beams, fmaps)
+    root = HW.walk(path)
+    assert set(root.groups) == {"filetype", "beams"} and set(root.groups["beams"].groups) == {"metadata"}
+    ft = root.groups["filetype"].attrs
+    assert (ft["ftmagic"].value, ft["ftversionmajor"].value, ft["ftversionminor"].value) == (0x2C, 1, 1)
+    meta = root.groups["beams"].groups["metadata"].groups
+    assert sorted(meta) == ["beam_00000", "beam_00001", "beam_00002"]
+    for i in (0, 1):
+        _check_beam_group(meta[f"beam_{i:05d}"], beams[i], fmaps[i], i, ("man", "unspec", "auto"))
+    assert int(meta["beam_00002"].attrs["N_beamlets"].value) == int(np.count_nonzero(fmaps[2]))
+    got, uids, maps = H.read_fmaps(path)
+    assert uids == [0, 1, 2]
+    for b, g, m, f in zip(beams, got, maps, fmaps):
+        for k in ("azimuth", "zenith", "coll"):
+            assert np.float32(getattr(b, k)) == np.float32(getattr(g, k))
+        for k in ("source", "direction", "isocenter", "beamlet_size", "fmap_size"):
+            assert list(getattr(b, k)) == list(getattr(g, k))
+        sad = np.sqrt(np.float32(sum((np.float32(i) - np.float32(s)) ** 2 for i, s in zip(b.isocenter, b.source))))
+        assert abs(g.sad - b.sad) <= 1e-4 * b.sad and abs(g.sad - sad) <= 1e-5 * sad
+        assert np.array_equal(m, f)
+    # a coefficient file is read the same way (its beams sit under /beams/metadata)
+    cpath = str(tmp_path / "coeffs.h5")
+    spec = _spec()
+    with H.CoeffFile(cpath, spec.struct(), nbeams=3) as f:
+        f.add_beams(beams[::-1], fmaps[::-1], uid_base=10)
+    got, uids, maps = H.read_fmaps(cpath)
+    assert uids == [10, 12] and np.array_equal(maps[0], fmaps[2]) and np.array_equal(maps[1], fmaps[0])
+
+
+# ---- roi_list.h5 ---------------------------------------------------------------------------------------
+def test_roi_list_file_and_reader(tmp_path):
+    rng = np.random.default_rng(13)
+    size = (30, 20, 10)
+    names = ["PTV", "Body", "Cord"]  # index order differs from name order
+    starts = [(3, 4, 1), (0, 0, 0), (10, 2, 5)]
+    masks = [(rng.random((4, 5, 6)) > 0.4).astype(np.uint8), np.ones((10, 20, 30), np.uint8),
+             (rng.random((2, 3, 7)) > 0.5).astype(np.uint8)]
+    path = str(tmp_path / "roi_list.h5")
+    H.write_roi_list(path, names, masks, size, starts)
+    root = HW.walk(path)
+    assert sorted(root.groups) == sorted(names) and not root.datasets
+    for i, (n, s, m) in enumerate(zip(names, starts, masks)):
+        g = root.groups[n]
+        assert g.attrs["name"].value == n and g.attrs["name"].dtype == f"S{len(n) + 1}z"
+        assert g.attrs["index"].dtype == "<u2" and int(g.attrs["index"].value) == i + 1
+        props = g.groups["ArrayProps"].attrs
+        assert {k: (a.dtype, a.scalar) for k, a in props.items()} == {
+            "size": ("<u2[3]", True), "crop_size": ("<u2[3]", True), "crop_start": ("<u2[3]", True)}
+        assert list(props["size"].value) == list(size) and list(props["crop_start"].value) == list(s)
+        assert list(props["crop_size"].value) == [m.shape[2], m.shape[1], m.shape[0]]
+        d = g.datasets["mask"]
+        assert d.dtype == "|u1" and d.data.shape == (m.size,) and np.array_equal(d.data, m.ravel())
+    vsize, rois = H.read_roi_list(path)
+    assert vsize == size and [r[0] for r in rois] == names
+    for (n, s, m), want_s, want_m in zip(rois, starts, masks):
+        assert s == want_s and np.array_equal(m, want_m)
+    for bad in (["a", "a"], ["a/b", "c"], ["", "c"]):
+        with pytest.raises(abi.CCCSError):
+            H.write_roi_list(str(tmp_path / "bad.h5"), bad, masks[:2], size, starts[:2])
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes as C
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(root, "include", "cccs_h5.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(cccs_h5_[a-z0-9_]+)\s*\(", txt)))
+    assert len(names) >= 12
+    lib = C.CDLL(H.LIB_PATH)
+    assert not [n for n in names if not hasattr(lib, n)]
+    assert set(names) <= {n for n, _, _ in H.EXPORTS}
