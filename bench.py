@@ -255,7 +255,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-beams", type=int, default=2, help="beams in the plain-path sample")
     ap.add_argument("--latency-beams", type=int, default=20, help="single-beam requests timed for p50/p99")
-    ap.add_argument("--cpu-beamlets", type=int, default=12, help="beamlets in the oracle sample (0 = skip)")
+    ap.add_argument("--cpu-beamlets", type=int, default=200,
+                    help="beamlets in the oracle sample, about 10 s of host work (0 = skip)")
     ap.add_argument("--no-extras", action="store_true", help="headline only (profiling runs)")
     ap.add_argument("--plain-beam-only", action="store_true",
                     help="only the plain full-beam path on the thorax plan (profiling runs)")
