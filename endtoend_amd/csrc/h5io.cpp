@@ -390,15 +390,20 @@ int cccs_h5_coeff_add_beams(cccs_h5_coeff_file* f, const cccs_beam* beams, int n
 int cccs_h5_coeff_add_column(cccs_h5_coeff_file* f, const cccs_sparse_column* c) {
     g_err.clear();
     if (!f || !c) return fail(CCCS_EINVAL, "null argument");
-    if (c->beam_index < 0 || c->beam_index >= (int)f->request_uids.size())
-        return fail(CCCS_EINVAL, "column of an undeclared beam (call cccs_h5_coeff_add_beams first)");
+    if (c->beam_index < 0) return fail(CCCS_EINVAL, "negative beam index");
+    {
+        std::lock_guard<std::mutex> lk(g_lock);
+        if (c->beam_index >= (int)f->request_uids.size())
+            return fail(CCCS_EINVAL, "column of an undeclared beam (call cccs_h5_coeff_add_beams first)");
+    }
     if (c->beamlet_index < 0 || c->beamlet_index > 65535) return fail(CCCS_EINVAL, "beamlet uid must fit 16 bits");
     if (c->nnz && (!c->keys || !c->values)) return fail(CCCS_EINVAL, "null column data");
     if (c->nblocks < 0 || (c->nblocks > 0 && !c->row_block_sizes)) return fail(CCCS_EINVAL, "null row-block sizes");
     if (c->nnz == 0) return CCCS_OK;  // the reference writes no group for an empty column
-    const int uid = f->request_uids[c->beam_index];
     std::lock_guard<std::mutex> lk(g_lock);
     quiet();
+    if (c->beam_index >= (int)f->request_uids.size()) return fail(CCCS_EINVAL, "column of an undeclared beam");
+    const int uid = f->request_uids[c->beam_index];
     if (!f->beams_written.count(uid)) return fail(CCCS_EINVAL, "column of a beam with no active beamlet");
     char name[96];
     std::snprintf(name, sizeof(name), "/beams/data/beam_%05d/beamlet_%05d", uid, c->beamlet_index);

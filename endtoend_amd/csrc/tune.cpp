@@ -107,7 +107,9 @@ extern "C" int cccs_autotune(cccs_ctx* ctx, int kind, const cccs_beam* beams, in
                  {"cone_class_threads2", {64, 128, 256}},
                  {"cone_class_threads3", {64, 128, 256}}};
     } else {
-        knobs = {{"bl_sample_waves", {65536, 131072, 262144, 524288}}, {"bl_walk_waves", {16384, 32768, 65536}}};
+        knobs = {{"bl_sample_waves", {65536, 131072, 262144, 524288}},
+                 {"bl_walk_waves", {16384, 32768, 65536}},
+                 {"bl_streams", {1, 2, 3, 4}}};
     }
     Trial t{ctx, kind, beams, nbeams, fluence};
     int st = CCCS_OK;

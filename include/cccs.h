@@ -159,8 +159,8 @@ int cccs_ctx_get_schedule(const cccs_ctx* ctx, const char* key, int* value);
 /* Autotuner (no counterpart in the reference): times the given workload under candidate values of
  * the schedule knobs, one knob at a time over two passes (coordinate descent), and leaves the
  * fastest setting on the context.  CCCS_TUNE_BEAM tunes the full-beam path's cone-kernel knobs on
- * device-resident beam doses of `beams`; CCCS_TUNE_BEAMLET tunes the beamlet path's wave counts on
- * device-resident faithful columns.  Results do not depend on the knobs beyond FP32 rounding. */
+ * device-resident beam doses of `beams`; CCCS_TUNE_BEAMLET tunes the beamlet path's wave counts and
+ * cone directions in flight (bl_streams) on device-resident faithful columns.  Results do not depend on the knobs beyond FP32 rounding. */
 #define CCCS_TUNE_BEAM    0
 #define CCCS_TUNE_BEAMLET 1
 typedef struct cccs_tune_report {

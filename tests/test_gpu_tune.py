@@ -37,6 +37,7 @@ def test_autotune_beamlet_path_keeps_the_columns():
         after = eng.beamlet_dose(w.beams, w.fmaps, threshold=1e-4)
     print(rep, flush=True)
     assert rep["knobs"]["bl_walk_waves"] in (16384, 32768, 65536)
+    assert rep["knobs"]["bl_streams"] in (1, 2, 3, 4)
     assert len(before) == len(after) > 10
     for a, b in zip(before, after):
         assert a[:2] == b[:2] and np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
