@@ -671,8 +671,12 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
           bv.z < a.bev_hi.z))
         return;
     // Cones in groups of kPvcsGroup: the line-table loads of a group are issued together, then its
-    // REV loads, so each lane keeps several dependent gathers in flight.  The arithmetic per cone is
-    // rev_fetch's and the sum runs in cone order.
+    // REV loads (branch-free: taps outside a line's stored interval read sample 0 of the store and
+    // are replaced by zero), so each lane keeps several dependent gathers in flight.  The
+    // arithmetic per cone is rev_fetch's and the sum runs in cone order.  The voxel's dose is read
+    // first so that its latency overlaps the gathers (this lane alone writes the voxel).
+    float* dst = a.dose + ((size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz));
+    const float old = *dst;
     float sum = 0.f;
     for (int q0 = 0; q0 < a.ncones; q0 += kPvcsGroup) {
         LineMeta mt[kPvcsGroup][4];
@@ -717,9 +721,10 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
             for (int t = 0; t < 4; t++) {
                 const LineMeta m = mt[u][t];
                 const int d = di[u] - m.i0;
-                float lo = 0.f, hi = 0.f;
-                if (d >= 0 && d < m.len) lo = a.rev[m.off + d];
-                if (d + 1 >= 0 && d + 1 < m.len) hi = a.rev[m.off + d + 1];
+                const bool vlo = d >= 0 && d < m.len, vhi = d + 1 >= 0 && d + 1 < m.len;
+                const float x0 = a.rev[vlo ? m.off + (uint32_t)d : 0u];
+                const float x1 = a.rev[vhi ? m.off + (uint32_t)(d + 1) : 0u];
+                const float lo = vlo ? x0 : 0.f, hi = vhi ? x1 : 0.f;
                 tv[t] = lo + fa[u] * (hi - lo);
             }
             const float e0 = tv[0] + fb[u] * (tv[1] - tv[0]);
@@ -727,7 +732,7 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
             sum += e0 + fc[u] * (e1 - e0);
         }
     }
-    if (sum != 0.f) a.dose[(size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz)] += sum;
+    if (sum != 0.f) *dst = old + sum;
 }
 
 // Interleaves density and TERMA over the CT grid for the cone kernel's paired fetch; TERMA is 0

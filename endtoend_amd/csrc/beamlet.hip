@@ -622,8 +622,12 @@ __global__ void __launch_bounds__(256) bl_gather_kernel(BlArgs a, uint32_t ntile
     // the last bits of the fetch coordinate)
     const v3 rk = c.kfwd_m(mk3((float)pX, (float)pY, (float)pZ));
     const v3 ri = mk3(fmaf(-c.minc.x, a.rrv.x, rk.x), fmaf(-c.minc.y, a.rrv.y, rk.y), fmaf(-c.minc.z, a.rrv.z, rk.z));
+    // the voxel's packed dose is read before the fetch so that its latency overlaps the fetch's
+    // (this lane alone writes the voxel in this launch)
+    float* dst = a.pdose + B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX);
+    const float old = *dst;
     const float dc = rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
-    if (dc > 0.f) a.pdose[B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX)] += dc;
+    if (dc > 0.f) *dst = old + dc;
 }
 
 // The batch of every gather tile of a job (one fill per batch range).

@@ -52,24 +52,35 @@ static_assert(sizeof(LineMeta) == 8, "line table entries are 8 bytes");
 // Trilinear fetch from a compactly stored REV grid (nx samples per line, ny x nz lines) at texture
 // coordinates (u, v, w) = (sample, line y, line z), texel centres at +0.5, border 0: the same
 // operation sequence as tex3_border on the dense grid, so both give identical results.
+//
+// The loads are branch-free and issued in two rounds: the four line records together, then the
+// eight samples together, so a lane waits for two memory round trips instead of one per tap.  A
+// tap outside the grid reads the grid's first record and a sample outside its line's stored
+// interval reads sample 0 of the store (both valid addresses); either is then replaced by zero.
 __device__ __forceinline__ float rev_fetch(const LineMeta* __restrict__ meta, const float* __restrict__ rev,
                                            uint64_t line0, int nx, int ny, int nz, float u, float v, float w) {
+    if (ny <= 0 || nz <= 0) return 0.f;
     const float ub = u - 0.5f, vb = v - 0.5f, wb = w - 0.5f;
     if (!(ub > -1.f && vb > -1.f && wb > -1.f && ub < (float)nx && vb < (float)ny && wb < (float)nz)) return 0.f;
     const float fu = floorf(ub), fv = floorf(vb), fw = floorf(wb);
     const int i = (int)fu, j = (int)fv, k = (int)fw;
     const float a = ub - fu, b = vb - fv, c = wb - fw;
-    float t[4];
+    LineMeta m[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int jj = j + (q & 1), kk = k + (q >> 1);
-        float lo = 0.f, hi = 0.f;
-        if (jj >= 0 && jj < ny && kk >= 0 && kk < nz) {
-            const LineMeta m = meta[line0 + (uint64_t)jj + (uint64_t)ny * kk];
-            const int d = i - m.i0;
-            if (d >= 0 && d < m.len) lo = rev[m.off + d];
-            if (d + 1 >= 0 && d + 1 < m.len) hi = rev[m.off + d + 1];
-        }
+        const bool in = jj >= 0 && jj < ny && kk >= 0 && kk < nz;
+        m[q] = meta[in ? line0 + (uint64_t)jj + (uint64_t)ny * (uint64_t)kk : line0];
+        if (!in) m[q].len = 0;
+    }
+    float t[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int d = i - m[q].i0;
+        const bool vlo = d >= 0 && d < m[q].len, vhi = d + 1 >= 0 && d + 1 < m[q].len;
+        const float x0 = rev[vlo ? m[q].off + (uint32_t)d : 0u];
+        const float x1 = rev[vhi ? m[q].off + (uint32_t)(d + 1) : 0u];
+        const float lo = vlo ? x0 : 0.f, hi = vhi ? x1 : 0.f;
         t[q] = lo + a * (hi - lo);
     }
     const float e0 = t[0] + b * (t[1] - t[0]);
