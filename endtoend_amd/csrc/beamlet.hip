@@ -362,6 +362,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     ops += (uint32_t)m * kOpsCull / 64u + 1u;
     __syncthreads();
     // pass B.2: exact mapping, density and TERMA request of the listed samples, full waves
+    int nlo = m, nhi = -1;  // first and last sample that is not a wall
     for (int r = 0; r < nex; r += 64) {
         float h = -1.f, t = 0.f;
         bool need = false;
@@ -393,6 +394,10 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             }
             Hl[j] = h;
             T[j] = t;
+            if (h >= 0.f) {
+                nlo = min(nlo, j);
+                nhi = max(nhi, j);
+            }
         }
         const uint64_t mask = __ballot(need);
         if (mask) {
@@ -402,6 +407,20 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt);
             if (need && pos < subcap) qs[pos] = rq;
         }
+    }
+    // Walls before the first and after the last non-wall sample bound no segment and receive no
+    // dose: the line's stored interval shrinks to [nlo, nhi], so the walk and the gather skip them
+    // (the gather reads samples outside a line's interval as zero, the dose the walk gives a wall).
+    for (int o = 32; o > 0; o >>= 1) {
+        nlo = min(nlo, __shfl_xor(nlo, o, 64));
+        nhi = max(nhi, __shfl_xor(nhi, o, 64));
+    }
+    if (tid == 0 && (nlo > 0 || nhi < m - 1)) {
+        LineMeta t;
+        t.off = nhi < nlo ? meta.off : meta.off + (uint32_t)nlo;
+        t.i0 = nhi < nlo ? meta.i0 : (int16_t)(i0 + nlo);
+        t.len = nhi < nlo ? (int16_t)0 : (int16_t)(nhi - nlo + 1);
+        a.lines[line] = t;
     }
     __syncthreads();  // the next line reuses the LDS arrays
     }
