@@ -248,7 +248,7 @@ constexpr uint32_t kOpsCull = 12, kOpsLocate = 70, kOpsFluence = 150, kOpsDensit
 // samples are queued for their Siddon TERMA (fast mode evaluates TERMA here instead).
 __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restrict__ H, TermaReq* __restrict__ q,
                                                        uint32_t subcap, uint32_t* __restrict__ qcnt) {
-    __shared__ uint8_t s_fp[kMaxLine];            // 1: footprint sample
+    __shared__ uint8_t s_fp[kMaxLine];            // bit 0: footprint sample, bit 1: on the exact list
     __shared__ short s_list[kMaxLine];            // compacted work list of the current pass
     __shared__ uint32_t s_pil[kPillarBits / 32];  // pillars with footprint samples on this line
     const int tid = threadIdx.x;
@@ -329,8 +329,10 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     __syncthreads();
     // pass B.1: a pillar without footprint samples on this line gets no dose on it, so its samples
     // act as walls (a pillar crossing is exactly one segment of the walk).  Samples confidently
-    // in such pillars (or outside the grid's pillars) are written as walls here; the others go to
-    // the exact list.
+    // in such pillars (or outside the grid's pillars) are walls; the others go to the exact list
+    // (and are marked with bit 1 of s_fp).  Only the walls between the first and the last listed
+    // sample are written: the line is trimmed to its non-wall span below, so no stage reads the
+    // others.
     float* T = a.rev + meta.off;
     float* Hl = H + meta.off;
     int nex = 0;
@@ -350,17 +352,27 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
                               !((s_pil[pa >> 5] >> (pa & 31)) & 1u));
                 }
             }
-            if (!exact) {
-                Hl[j] = -1.f;
-                T[j] = 0.f;
-            }
+            if (exact) s_fp[j] |= 2;
         }
         const uint64_t mask = __ballot(exact);
         if (exact) s_list[nex + __popcll(mask & lt)] = (short)j;
         nex += __popcll(mask);
     }
     ops += (uint32_t)m * kOpsCull / 64u + 1u;
+    if (nex == 0) {  // every sample is a wall: no dose on this line
+        if (tid == 0) a.lines[line].len = 0;
+        __syncthreads();
+        continue;
+    }
     __syncthreads();
+    {
+        const int elo = s_list[0], ehi = s_list[nex - 1];
+        for (int j = elo + tid; j <= ehi; j += 64)
+            if (!(s_fp[j] & 2)) {
+                Hl[j] = -1.f;
+                T[j] = 0.f;
+            }
+    }
     // pass B.2: exact mapping, density and TERMA request of the listed samples, full waves
     int nlo = m, nhi = -1;  // first and last sample that is not a wall
     for (int r = 0; r < nex; r += 64) {
@@ -378,7 +390,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
                 const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
                 h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
                 ops += kOpsDensity;
-                if (s_fp[j]) {
+                if (s_fp[j] & 1) {
                     const v3 gcs = bl_terma_point(a, B, bl, inp);
                     if (a.fast) {
                         const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
