@@ -608,57 +608,75 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
     if (a.counts) wave_count(iters, a.counts, blockIdx.x);
 }
 
+// Only REV lines with footprint samples in a pillar carry dose in it, and those lines pass within
+// sqrt(2)*fw of that pillar's beamlet axis (footprint samples satisfy |x-fc|,|z-fc| <= fw).  The
+// four lines a voxel reads lie within sqrt(2) of the line through it, so if that line passes
+// farther than sqrt(2)*(fw + 1) + 0.5 from the axes of the voxel's pillar and of the neighbouring
+// pillars it touches, the fetch is zero and is skipped.  Depends on pX and pZ only.
+__device__ __forceinline__ bool bl_gather_hit(const BatchDev& B, const BlConeDev& c, int pX, int pZ) {
+    if (!(B.fw < 1e29f)) return true;
+    const float wx = c.kinv_m.m[1], wz = c.kinv_m.m[7];
+    const float wn = sqrtf(wx * wx + wz * wz);
+    const float bound = 1.41421356f * (B.fw + 1.f) + 0.5f;
+    const int pix = pX / B.pd[0], piy = pZ / B.pd[2];
+    const int rx = pX - pix * B.pd[0], rz = pZ - piy * B.pd[2];
+    const int sx = rx < 2 ? -1 : (rx >= B.pd[0] - 2 ? 1 : 0);
+    const int sz = rz < 2 ? -1 : (rz >= B.pd[2] - 2 ? 1 : 0);
+    bool hit = false;
+    for (int ox = 0; ox <= (sx != 0 ? 1 : 0) && !hit; ox++)
+        for (int oz = 0; oz <= (sz != 0 ? 1 : 0) && !hit; oz++) {
+            const int qx = pix + ox * sx, qz = piy + oz * sz;
+            if (qx < 0 || qz < 0 || qx >= B.npx || qz >= B.npy || qx + qz * B.npx >= B.n) continue;
+            const float dx = (float)(pX - qx * B.pd[0]) - B.fc[0], dz = (float)(pZ - qz * B.pd[2]) - B.fc[1];
+            const float dist = wn > 1e-6f ? fabsf(dx * wz - dz * wx) / wn : sqrtf(dx * dx + dz * dz);
+            hit = dist <= bound;
+        }
+    return hit;
+}
+
 // PackedREVtoBEVdose (nvbbRayConvolve_device.cu:427-452) for one cone direction over every batch's
-// packed grid: 16x4x4 voxel tiles, dose += cone dose where it is positive.
-__global__ void __launch_bounds__(256) bl_gather_kernel(BlArgs a, uint32_t ntiles) {
+// packed grid: dose += cone dose where it is positive.  One wave per 16x4x4 voxel tile (pY
+// fastest: compact in every direction, so neighbouring lanes read neighbouring REV lines); each
+// lane owns one (pY, pZ) and the tile's four pX voxels.  The four voxels' loads are all issued
+// before any of their stores, so a lane has four fetches in flight, and the tile's scalar set-up
+// is paid once per wave.
+constexpr int kGatherX = 4;  // pX voxels per lane (the tile's depth along pX)
+__global__ void __launch_bounds__(64) bl_gather_kernel(BlArgs a, uint32_t ntiles) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
     const int lo = a.tile_batch ? (int)a.tile_batch[tile] : 0;
     const BatchDev& B = a.batches[lo];
     const BlConeDev& c = a.cones[lo];
-    // 16x4x4 voxel tiles (pY fastest): compact in every direction, so neighbouring lanes read
-    // neighbouring REV lines
     const uint32_t t = tile - B.tile0;
     const int ty = (int)(t % (uint32_t)B.tiles[0]);
     const int tz = (int)((t / (uint32_t)B.tiles[0]) % (uint32_t)B.tiles[1]);
     const int tx = (int)(t / ((uint32_t)B.tiles[0] * (uint32_t)B.tiles[1]));
     const int tid = threadIdx.x;
-    const int pY = ty * 16 + (tid & 15), pZ = tz * 4 + ((tid >> 4) & 3), pX = tx * 4 + (tid >> 6);
-    if (pX >= B.gd[0] || pY >= B.gd[1] || pZ >= B.gd[2]) return;
-    // Only REV lines with footprint samples in a pillar carry dose in it, and those lines pass
-    // within sqrt(2)*fw of that pillar's beamlet axis (footprint samples satisfy |x-fc|,|z-fc| <=
-    // fw).  The four lines this voxel reads lie within sqrt(2) of the line through it, so if that
-    // line passes farther than sqrt(2)*(fw + 1) + 0.5 from the axes of its pillar and of the
-    // neighbouring pillars it touches, the fetch is zero and is skipped.
-    if (B.fw < 1e29f) {
-        const float wx = c.kinv_m.m[1], wz = c.kinv_m.m[7];
-        const float wn = sqrtf(wx * wx + wz * wz);
-        const float bound = 1.41421356f * (B.fw + 1.f) + 0.5f;
-        const int pix = pX / B.pd[0], piy = pZ / B.pd[2];
-        const int rx = pX - pix * B.pd[0], rz = pZ - piy * B.pd[2];
-        const int sx = rx < 2 ? -1 : (rx >= B.pd[0] - 2 ? 1 : 0);
-        const int sz = rz < 2 ? -1 : (rz >= B.pd[2] - 2 ? 1 : 0);
-        bool hit = false;
-        for (int ox = 0; ox <= (sx != 0 ? 1 : 0) && !hit; ox++)
-            for (int oz = 0; oz <= (sz != 0 ? 1 : 0) && !hit; oz++) {
-                const int qx = pix + ox * sx, qz = piy + oz * sz;
-                if (qx < 0 || qz < 0 || qx >= B.npx || qz >= B.npy || qx + qz * B.npx >= B.n) continue;
-                const float dx = (float)(pX - qx * B.pd[0]) - B.fc[0], dz = (float)(pZ - qz * B.pd[2]) - B.fc[1];
-                const float dist = wn > 1e-6f ? fabsf(dx * wz - dz * wx) / wn : sqrtf(dx * dx + dz * dz);
-                hit = dist <= bound;
-            }
-        if (!hit) return;
+    const int pY = ty * 16 + (tid & 15), pZ = tz * 4 + (tid >> 4);
+    if (pY >= B.gd[1] || pZ >= B.gd[2]) return;
+    float old[kGatherX], dc[kGatherX];
+    float* dst[kGatherX];
+#pragma unroll
+    for (int s = 0; s < kGatherX; s++) {
+        const int pX = tx * kGatherX + s;
+        dc[s] = 0.f;
+        old[s] = 0.f;
+        dst[s] = nullptr;
+        if (pX >= B.gd[0] || !bl_gather_hit(B, c, pX, pZ)) continue;
+        // continuous resampling: the matrix form of rotateKernel (the Rodrigues form differs only
+        // in the last bits of the fetch coordinate)
+        const v3 rk = c.kfwd_m(mk3((float)pX, (float)pY, (float)pZ));
+        const v3 ri = mk3(fmaf(-c.minc.x, a.rrv.x, rk.x), fmaf(-c.minc.y, a.rrv.y, rk.y),
+                          fmaf(-c.minc.z, a.rrv.z, rk.z));
+        // the voxel's packed dose is read before the fetch so that its latency overlaps the
+        // fetch's (this lane alone writes the voxel in this launch)
+        dst[s] = a.pdose + B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX);
+        old[s] = *dst[s];
+        dc[s] = rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
     }
-    // continuous resampling: the matrix form of rotateKernel (the Rodrigues form differs only in
-    // the last bits of the fetch coordinate)
-    const v3 rk = c.kfwd_m(mk3((float)pX, (float)pY, (float)pZ));
-    const v3 ri = mk3(fmaf(-c.minc.x, a.rrv.x, rk.x), fmaf(-c.minc.y, a.rrv.y, rk.y), fmaf(-c.minc.z, a.rrv.z, rk.z));
-    // the voxel's packed dose is read before the fetch so that its latency overlaps the fetch's
-    // (this lane alone writes the voxel in this launch)
-    float* dst = a.pdose + B.pdose + (size_t)pY + (size_t)B.gd[1] * ((size_t)pZ + (size_t)B.gd[2] * pX);
-    const float old = *dst;
-    const float dc = rev_fetch(a.lines, a.rev, c.line0, c.nx, c.ny, c.nz, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
-    if (dc > 0.f) *dst = old + dc;
+#pragma unroll
+    for (int s = 0; s < kGatherX; s++)
+        if (dc[s] > 0.f) *dst[s] = old[s] + dc[s];
 }
 
 // The batch of every gather tile of a job (one fill per batch range).
@@ -1254,7 +1272,7 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
         J.pdose += (uint64_t)B.gd[0] * B.gd[1] * B.gd[2];
         B.tiles[0] = (B.gd[1] + 15) / 16;
         B.tiles[1] = (B.gd[2] + 3) / 4;
-        B.tiles[2] = (B.gd[0] + 3) / 4;
+        B.tiles[2] = (B.gd[0] + kGatherX - 1) / kGatherX;
         B.tile0 = J.tiles;
         // footprint prefilter: the beamlet axis runs along the pillar's y axis through its x/z
         // centre; a sample farther from it than half the widest divergent beamlet (at the far end
@@ -1488,7 +1506,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                     break;
                 }
                 int tk = prof_begin(ctx, K_BL_GATHER);
-                hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
+                hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(64), 0, ctx->stream, a, J.tiles);
                 prof_end(ctx, tk, 8.0 * (double)J.pdose, 0.0, J.pdose);
                 if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "gather kernel"); break; }
                 if (nset > 1 && hipEventRecord(ctx->ray_ev[k], ctx->stream) != hipSuccess) {
@@ -1637,7 +1655,7 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
         hipLaunchKernelGGL(plain_meta_kernel, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, ctx->stream, a.lines,
                            a.nlines, c.nx);
         int t2 = prof_begin(ctx, K_BL_GATHER);
-        hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(256), 0, ctx->stream, a, J.tiles);
+        hipLaunchKernelGGL(bl_gather_kernel, dim3(J.tiles), dim3(64), 0, ctx->stream, a, J.tiles);
         prof_end(ctx, t2, 8.0 * (double)J.pdose, 0.0, J.pdose);
         CCCS_HIP(hipGetLastError());
     }
