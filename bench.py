@@ -105,7 +105,7 @@ def cpu_baseline(w, eng, nbeamlets: int):
 
 BEAM_KNOBS = ("cone_ramp_min", "cone_class_len0", "cone_class_len1", "cone_class_len2", "cone_class_threads1",
               "cone_class_threads2", "cone_class_threads3")
-BEAMLET_KNOBS = ("bl_sample_waves", "bl_walk_waves", "bl_streams")
+BEAMLET_KNOBS = ("bl_sample_waves", "bl_walk_waves", "bl_walk_split", "bl_streams")
 
 
 def plain_beam_path(steps: int = 2, autotune: bool = False):

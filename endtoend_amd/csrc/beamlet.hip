@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(256) bl_siddon_kernel(BlArgs a, const TermaReq
 //
 // Dynamic LDS: for up to maxn samples {half step, TERMA} (8 B), the radiological centre (FP64),
 // the segment bounds and the footprint list (3 x 2 B).
-__global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __restrict__ H, int maxn) {
+__global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __restrict__ H, int maxn, int len_lo) {
     extern __shared__ double s_dyn[];
     __shared__ float4 sTab[2][kMaxRadii];
     const int tid = threadIdx.x;
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
     uint32_t iters = 0;
     for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
         const LineMeta meta = a.lines[line];
-        if (meta.len <= 0) continue;
+        if (meta.len < len_lo || meta.len > maxn) continue;  // empty, or another launch's line
         const int m = meta.len;
         float* T = a.rev + meta.off;
         const float* Hl = H + meta.off;
@@ -1493,10 +1493,22 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                     prof_end(ctx, ts, 0.0, 0.0, 0);
                     if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "siddon kernel"); break; }
                 }
-                a.counts = prof_counter(ctx, ww);
-                const size_t lds = (sizeof(double) + sizeof(float2) + 3 * sizeof(short)) * (size_t)maxn;
+                // length classes (bl_walk_split, twice that, ...; the last ends at the longest
+                // line): each launch sizes its LDS for its own longest line, so the short lines,
+                // the bulk after trimming, run with more waves per CU
+                const uint32_t split = (uint32_t)ctx->sched.bl_walk_split;
+                uint32_t bounds[12];
+                int nclass = 0;
+                for (uint32_t hi = split; split > 0 && hi < maxn && nclass < 11; hi *= 2) bounds[nclass++] = hi;
+                bounds[nclass++] = maxn;
+                const size_t per = sizeof(double) + sizeof(float2) + 3 * sizeof(short);
+                uint32_t* wc = prof_counter(ctx, (size_t)nclass * ww);
                 int tw = prof_begin(ctx, K_BL_WALK);
-                hipLaunchKernelGGL(bl_walk_kernel, dim3(ww), dim3(64), lds, ctx->stream, a, (const float*)H, (int)maxn);
+                for (int ci = 0; ci < nclass; ci++) {
+                    a.counts = wc ? wc + (size_t)ci * ww : nullptr;
+                    hipLaunchKernelGGL(bl_walk_kernel, dim3(ww), dim3(64), per * bounds[ci], ctx->stream, a,
+                                       (const float*)H, (int)bounds[ci], ci == 0 ? 1 : (int)bounds[ci - 1] + 1);
+                }
                 prof_end(ctx, tw, 0.0, 0.0, a.nlines);
                 if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "walk kernel"); break; }
                 a.counts = nullptr;

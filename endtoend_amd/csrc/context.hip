@@ -356,6 +356,11 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
         c->sched.bl_streams = value;
         return CCCS_OK;
     }
+    if (std::strcmp(key, "bl_walk_split") == 0) {
+        if (value < 0 || value > kMaxLine) return set_error(CCCS_EINVAL, "bl_walk_split must be in [0, 1024]");
+        c->sched.bl_walk_split = value;
+        return CCCS_OK;
+    }
     if (std::strcmp(key, "bl_sample_waves") == 0 || std::strcmp(key, "bl_walk_waves") == 0) {
         if (value < 64 || value > (1 << 22)) return set_error(CCCS_EINVAL, "wave count must be in [64, 2^22]");
         (key[3] == 's' ? c->sched.bl_sample_waves : c->sched.bl_walk_waves) = value;
@@ -388,6 +393,7 @@ extern "C" int cccs_ctx_get_schedule(const cccs_ctx* c, const char* key, int* va
     if (std::strcmp(key, "bl_sample_waves") == 0) { *value = S.bl_sample_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_walk_waves") == 0) { *value = S.bl_walk_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_streams") == 0) { *value = S.bl_streams; return CCCS_OK; }
+    if (std::strcmp(key, "bl_walk_split") == 0) { *value = S.bl_walk_split; return CCCS_OK; }
     for (int i = 0; i < kConeClasses; i++) {
         if (("cone_class_threads" + std::to_string(i)) == key) { *value = S.cone_class_threads[i]; return CCCS_OK; }
         if (i < kConeClasses - 1 && ("cone_class_len" + std::to_string(i)) == key) {

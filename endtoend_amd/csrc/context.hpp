@@ -108,6 +108,10 @@ struct Schedule {
     // Siddon stage more coherent neighbouring requests
     int bl_sample_waves = 262144;
     int bl_walk_waves = 32768;
+    // the beamlet walk sizes its LDS for the longest line of its launch; it runs one launch per
+    // length class: lines up to this many stored samples, up to twice that, ..., and the rest
+    // (0: one launch)
+    int bl_walk_split = 256;
     // cone directions of the beamlet path in flight at once (1 .. kRaySets): direction r runs on
     // stream r mod bl_streams, so consecutive directions' sample, Siddon and walk overlap; the
     // gathers stay in direction order (measured on the 50-beam job: 1 -> 10.8k, 2 -> 12.3k,

@@ -109,6 +109,7 @@ extern "C" int cccs_autotune(cccs_ctx* ctx, int kind, const cccs_beam* beams, in
     } else {
         knobs = {{"bl_sample_waves", {65536, 131072, 262144, 524288}},
                  {"bl_walk_waves", {16384, 32768, 65536}},
+                 {"bl_walk_split", {0, 128, 256, 384}},
                  {"bl_streams", {1, 2, 3, 4}}};
     }
     Trial t{ctx, kind, beams, nbeams, fluence};

@@ -150,6 +150,9 @@ int cccs_ctx_memory(const cccs_ctx* ctx, uint64_t* resident_bytes, uint64_t* scr
  *   "bl_sample_waves", "bl_walk_waves"
  *                    waves of the beamlet path's persistent per-line kernels (defaults 262144 and
  *                    32768; each wave takes a stride of lines).
+ *   "bl_walk_split"  the beamlet walk runs one launch per line-length class (up to this many
+ *                    stored samples, up to twice that, ..., the rest), each with its LDS sized
+ *                    for its own longest line (default 256; 0 = one launch).
  *   "cone_class_threads0".."cone_class_threads3"
  *                    threads per cone line in each class: 64, 128 or 256 (defaults 64, 64, 128,
  *                    128). */

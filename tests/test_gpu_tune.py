@@ -38,6 +38,7 @@ def test_autotune_beamlet_path_keeps_the_columns():
     print(rep, flush=True)
     assert rep["knobs"]["bl_walk_waves"] in (16384, 32768, 65536)
     assert rep["knobs"]["bl_streams"] in (1, 2, 3, 4)
+    assert rep["knobs"]["bl_walk_split"] in (0, 128, 256, 384)
     assert len(before) == len(after) > 10
     for a, b in zip(before, after):
         assert a[:2] == b[:2] and np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
