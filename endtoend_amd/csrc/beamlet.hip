@@ -29,6 +29,7 @@ constexpr int kWall = 2;        // pillar wall, REV samples (PILLAR_GRID, io_dat
 constexpr int kBuffer = 2;      // pillar rim excluded from unpacking (same struct)
 constexpr int kMaxBatches = 100;  // batch-count retry limit of the reference (host.cu:692-834)
 constexpr uint32_t kSubQueues = 64;  // TERMA sub-queues per cone direction
+constexpr uint32_t kCntStride = 32;  // a sub-queue counter per 128-byte line: no two share one
 constexpr uint32_t kPillarBits = 4096;  // batches up to this many pillars track footprint pillars per line
 
 // One Rodrigues rotation with host-evaluated sine/cosine.
@@ -79,8 +80,10 @@ struct BatchDev {
 };
 
 // A footprint sample whose TERMA needs a Siddon march.
+constexpr uint32_t kNoSlot = 0xffffffffu;  // a reserved queue entry that carries no request
+
 struct TermaReq {
-    uint32_t slot;     // position in the REV store of the cone direction
+    uint32_t slot;     // position in the REV store of the cone direction, or kNoSlot
     uint32_t beamlet;  // job beamlet index
     float x, y, z;     // TERMA point [cm]
 };
@@ -301,6 +304,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     __syncthreads();
     // pass A.2: exact pillar mapping and fluence-cell test of the candidates, full waves
     bool any = false;
+    uint32_t nfoot = 0;  // footprint samples of the line: exactly its TERMA requests below
     for (int r = 0; r < ncand; r += 64) {
         bool fp = false;
         if (r + tid < ncand) {
@@ -319,13 +323,19 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
                 }
             }
         }
-        any = any || __ballot(fp) != 0ull;
+        const uint64_t fm = __ballot(fp);
+        any = any || fm != 0ull;
+        nfoot += (uint32_t)__popcll(fm);
     }
     if (!any) {
         if (tid == 0) a.lines[line].len = 0;
         __syncthreads();
         continue;
     }
+    // one reservation per line in the sub-queue (its latency overlaps pass B.1): every footprint
+    // sample, and no other, queues one TERMA request in pass B.2
+    uint32_t qres = 0;
+    if (tid == 0 && !a.fast) qres = atomicAdd(qcnt + sq * kCntStride, nfoot);
     __syncthreads();
     // pass B.1: a pillar without footprint samples on this line gets no dose on it, so its samples
     // act as walls (a pillar crossing is exactly one segment of the walk).  Samples confidently
@@ -359,8 +369,12 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         nex += __popcll(mask);
     }
     ops += (uint32_t)m * kOpsCull / 64u + 1u;
-    if (nex == 0) {  // every sample is a wall: no dose on this line
+    const uint32_t qbase = __shfl(qres, 0, 64);
+    if (nex == 0) {  // every sample is a wall: no dose on this line, and the reservation stays empty
         if (tid == 0) a.lines[line].len = 0;
+        if (!a.fast)
+            for (uint32_t k = (uint32_t)tid; k < nfoot; k += 64)
+                if (qbase + k < subcap) qs[qbase + k].slot = kNoSlot;
         __syncthreads();
         continue;
     }
@@ -375,6 +389,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     }
     // pass B.2: exact mapping, density and TERMA request of the listed samples, full waves
     int nlo = m, nhi = -1;  // first and last sample that is not a wall
+    uint32_t qdone = 0;
     for (int r = 0; r < nex; r += 64) {
         float h = -1.f, t = 0.f;
         bool need = false;
@@ -413,13 +428,18 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         }
         const uint64_t mask = __ballot(need);
         if (mask) {
-            uint32_t basepos = 0;
-            if (tid == 0) basepos = atomicAdd(qcnt + sq, (uint32_t)__popcll(mask));
-            basepos = __shfl(basepos, 0, 64);
-            const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt);
+            const uint32_t pos = qbase + qdone + (uint32_t)__popcll(mask & lt);
             if (need && pos < subcap) qs[pos] = rq;
+            qdone += (uint32_t)__popcll(mask);
         }
     }
+    // The reservation counted footprint samples of pass A.2; pass B.2 requests TERMA for those of
+    // them it maps to the same pillar again, so any shortfall is closed with empty entries.
+    if (!a.fast)
+        for (uint32_t k = qdone + (uint32_t)tid; k < nfoot; k += 64) {
+            const uint32_t pos = qbase + k;
+            if (pos < subcap) qs[pos].slot = kNoSlot;
+        }
     // Walls before the first and after the last non-wall sample bound no segment and receive no
     // dose: the line's stored interval shrinks to [nlo, nhi], so the walk and the gather skip them
     // (the gather reads samples outside a line's interval as zero, the dose the walk gives a wall).
@@ -445,12 +465,13 @@ __global__ void __launch_bounds__(256) bl_siddon_kernel(BlArgs a, const TermaReq
                                                         const uint32_t* __restrict__ qcnt, uint32_t* counts) {
     // workgroup b serves sub-queue b % kSubQueues
     const uint32_t sq = blockIdx.x % kSubQueues, per = gridDim.x / kSubQueues;
-    const uint32_t n = min(qcnt[sq], subcap);
+    const uint32_t n = min(qcnt[sq * kCntStride], subcap);
     const TermaReq* qs = q + (size_t)sq * subcap;
     const v3 ivox = mk3(1.f / a.vd.voxel.x, 1.f / a.vd.voxel.y, 1.f / a.vd.voxel.z);
     uint32_t steps = 0;
     for (uint32_t i = (blockIdx.x / kSubQueues) * 256 + threadIdx.x; i < n; i += per * 256) {
         const TermaReq r = qs[i];
+        if (r.slot == kNoSlot) continue;
         const BeamletDev& bl = a.beamlets[r.beamlet];
         const BatchDev& B = a.batches[bl.batch];
         int it = 0;
@@ -1404,8 +1425,8 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     // TERMA queue: footprint samples are a few percent of the stored samples; the counts are
     // checked after the pass and the pass reruns with larger queues if one overflowed
     uint64_t subcap = std::max<uint64_t>(uint64_t(1) << 14, J.max_rev / 8 / kSubQueues);
-    CCCS_TRY(ctx->bl_qcnt.ensure(sizeof(uint32_t) * kSubQueues * (size_t)std::max(1, nrays)));
-    CCCS_TRY(ctx->h_small.ensure(sizeof(uint32_t) * kSubQueues * kMaxRays));
+    CCCS_TRY(ctx->bl_qcnt.ensure(sizeof(uint32_t) * kSubQueues * kCntStride * (size_t)std::max(1, nrays)));
+    CCCS_TRY(ctx->h_small.ensure(sizeof(uint32_t) * kSubQueues * kCntStride * kMaxRays));
     BlArgs a = base_args(ctx, fast);
     a.batches = reinterpret_cast<const BatchDev*>(d);
     a.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l);
@@ -1445,8 +1466,9 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
         for (int k = 0; k < nset; k++) CCCS_TRY(set_q[k]->ensure(sizeof(TermaReq) * subcap * kSubQueues));
         uint32_t* qcnt = ctx->bl_qcnt.as<uint32_t>();
         hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for(J.pdose, 256)), dim3(256), 0, ctx->stream, a.pdose, J.pdose, 0.f);
-        hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for((uint64_t)kSubQueues * nrays, 256)), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<float*>(qcnt), (uint64_t)kSubQueues * nrays, 0.f);
+        hipLaunchKernelGGL(bl_fill_kernel, dim3(grid_for((uint64_t)kSubQueues * kCntStride * nrays, 256)), dim3(256), 0,
+                           ctx->stream,
+                           reinterpret_cast<float*>(qcnt), (uint64_t)kSubQueues * kCntStride * nrays, 0.f);
         CCCS_HIP(hipGetLastError());
         if (nset > 1) {  // the other streams start after the tables, fills and earlier work
             CCCS_HIP(hipEventRecord(ctx->ray_ev[kRaySets], S[0]));
@@ -1481,7 +1503,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                 a.counts = prof_counter(ctx, lw);
                 int tok = prof_begin(ctx, K_BL_SAMPLE);
                 hipLaunchKernelGGL(bl_sample_kernel, dim3(lw), dim3(64), 0, ctx->stream, a, H, q, (uint32_t)subcap,
-                                   qcnt + (size_t)kSubQueues * r);
+                                   qcnt + (size_t)kSubQueues * kCntStride * r);
                 prof_end(ctx, tok, 0.0, 0.0, a.nlines);
                 if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "sample kernel"); break; }
                 if (!fast) {
@@ -1489,7 +1511,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                     uint32_t* cnt = prof_counter(ctx, sb);
                     int ts = prof_begin(ctx, K_BL_SIDDON);
                     hipLaunchKernelGGL(bl_siddon_kernel, dim3(sb), dim3(256), 0, ctx->stream, a, q, (uint32_t)subcap,
-                                       (const uint32_t*)(qcnt + (size_t)kSubQueues * r), cnt);
+                                       (const uint32_t*)(qcnt + (size_t)kSubQueues * kCntStride * r), cnt);
                     prof_end(ctx, ts, 0.0, 0.0, 0);
                     if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "siddon kernel"); break; }
                 }
@@ -1539,10 +1561,11 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
         }
         if (fast) return CCCS_OK;
         uint32_t* hc = ctx->h_small.as<uint32_t>();
-        CCCS_HIP(hipMemcpyAsync(hc, qcnt, sizeof(uint32_t) * kSubQueues * nrays, hipMemcpyDeviceToHost, ctx->stream));
+        CCCS_HIP(hipMemcpyAsync(hc, qcnt, sizeof(uint32_t) * kSubQueues * kCntStride * nrays, hipMemcpyDeviceToHost,
+                                ctx->stream));
         CCCS_HIP(hipStreamSynchronize(ctx->stream));
         uint64_t need = 0;
-        for (int r = 0; r < kSubQueues * nrays; r++) need = std::max<uint64_t>(need, hc[r]);
+        for (int r = 0; r < kSubQueues * nrays; r++) need = std::max<uint64_t>(need, hc[(size_t)r * kCntStride]);
         if (need <= subcap) return CCCS_OK;
         subcap = need + need / 4;
     }
