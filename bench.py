@@ -149,6 +149,69 @@ def plain_beam_path(steps: int = 2, autotune: bool = False):
             "schedule": tune}
 
 
+def full_4pi(stride: int = 12, rate: float = 20.0):
+    """Full 4pi precompute on a bounded sample: every stride-th beam of the 1162-beam 4pi list on
+    the batched workload's phantom, 64x64 maps of 2.5 mm beamlets over the PTV projection, 96 cones.
+    First the batched rate with the columns left in HBM; then the same beams as beamlet requests
+    through the serving loop (columns to the host, native counting sink) while single-beam dose
+    requests arrive at `rate`/s on the same resident state (Poisson; 4pi-list beams, 20^2-64^2 x
+    5 mm U[0,1] maps): the precompute's rate under that load and the requests' latency."""
+    import threading
+
+    from endtoend_amd import abi
+    from endtoend_amd import workloads as W
+    from endtoend_amd.engine import Engine, Server
+    w = W.cfg5(stride=stride)
+    nbl = w.active_beamlets
+    FA = abi.CCCS_BEAMLET_FAITHFUL
+    with Engine(w.spec) as eng:
+        eng.beamlet_dose(w.beams[:2], w.fmaps[:2], threshold=THRESH, mode=FA, keep=None)  # warm
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.beamlet_dose(w.beams, w.fmaps, threshold=THRESH, mode=FA, keep=None)
+        eng.sync()
+        batched = nbl / (time.perf_counter() - t0)
+    rng = np.random.default_rng(W.SEED_ARRIVALS)
+    reqs = []
+    for i in rng.integers(0, len(w.beams), 400):
+        b = w.beams[int(i)].copy()
+        n = int(rng.integers(20, 65))
+        b.fmap_size[:] = (n, n)
+        b.beamlet_size[:] = (0.5, 0.5)
+        reqs.append((b, rng.uniform(0.0, 1.0, (n, n)).astype(np.float32)))
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    with Server(w.spec, slots=2) as srv:
+        srv.wait(srv.submit(*reqs[0]))                                       # warm both kinds
+        srv.wait(srv.submit(w.beams[0], w.fmaps[0], abi.CCCS_REQ_BEAMLET, THRESH), abi.CCCS_REQ_BEAMLET,
+                 counts_only=True)
+        lat, done = [], threading.Event()
+
+        def requests():
+            ids = []
+            for k in range(1, len(reqs)):
+                if done.wait(gaps[k]):
+                    break
+                ids.append(srv.submit(*reqs[k]))
+            lat.extend(srv.wait(r)[1] for r in ids)
+
+        th = threading.Thread(target=requests)
+        t0 = time.perf_counter()
+        th.start()
+        pre = [srv.submit(b, f, abi.CCCS_REQ_BEAMLET, THRESH) for b, f in zip(w.beams, w.fmaps)]
+        cols = sum(srv.wait(r, abi.CCCS_REQ_BEAMLET, counts_only=True)[0]["columns"] for r in pre)
+        span = time.perf_counter() - t0
+        done.set()
+        th.join()
+    return {"workload": f"{len(w.beams)} beams (every {stride}th of the 1162-beam 4pi list), 128^3 water @ 2.5 mm + "
+                        "ellipsoidal PTV, 64x64 x 2.5 mm fluence open over the PTV projection, 96 cones, sparse "
+                        "columns at 1e-4 of column max, faithful TERMA",
+            "beamlets": nbl, "batched": {"value": batched, "unit": "beamlet-doses/s"},
+            "with_requests": {"precompute_value": cols / span, "unit": "beamlet-doses/s", "columns": cols,
+                              "request_rate": rate, "requests": len(lat),
+                              "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99),
+                                            "max": max(lat) if lat else None}}}
+
+
 def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
     """Annealing-style request stream through the serving loop: Poisson arrivals of single-beam
     dose requests on the 256x256x180 thorax phantom (beams from the 1162-beam 4pi list, 20^2-64^2
@@ -385,6 +448,7 @@ def main():
         eng.close()
         out["plain_beam_path"] = plain_beam_path(autotune=args.autotune)
         out["serving"] = serving()
+        out["full_4pi"] = full_4pi()
     else:
         eng.close()
         out["cpu_baseline"] = None

@@ -7,6 +7,11 @@
 //   just one, and runs them as one multi-beam job (shared launches).  Requests that arrive while
 //   a batch runs join the next batch.  A full-beam request fills the device alone and shares no
 //   launches, so a slot takes one at a time; concurrent slots overlap them.
+// * Latency first: queued full-beam requests are taken before queued beamlet requests, whatever
+//   their arrival order, and with two or more slots at most all but one slot run beamlet batches
+//   at a time, so a full-beam request never waits for a long beamlet batch (a precompute that
+//   shares the resident state with an optimiser's requests).  One beamlet batch already keeps
+//   the device busy with its several cone directions in flight.
 // * Admission by HBM footprint: a request is admitted only while the device bytes in flight plus
 //   its estimate stay within the budget.  A request larger than the whole budget runs alone.
 // * Latency: each request's submit-to-completion time.  Statistics give p50/p99.
@@ -59,7 +64,8 @@ struct cccs_server {
     std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Request>> queue;
+    std::deque<std::shared_ptr<Request>> q_dose, q_bl;  // queued full-beam and beamlet requests
+    int bl_running = 0;                                // slots running a beamlet batch
     std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
     uint64_t next_id = 1;
     uint64_t budget = 0, in_flight = 0;
@@ -187,35 +193,43 @@ void worker(cccs_server* s, int slot) {
         uint64_t bytes = 0;
         {
             std::unique_lock<std::mutex> lk(s->mu);
+            // the queue this slot may serve now: full-beam requests first; beamlet requests only
+            // while another slot is left for full-beam requests
+            auto pick = [&]() -> std::deque<std::shared_ptr<Request>>* {
+                if (!s->q_dose.empty()) return &s->q_dose;
+                if (!s->q_bl.empty() && (s->slots.size() < 2 || s->bl_running < (int)s->slots.size() - 1))
+                    return &s->q_bl;
+                return nullptr;
+            };
+            std::deque<std::shared_ptr<Request>>* q = nullptr;
             s->cv_work.wait(lk, [&] {
-                if (s->stop) return true;
-                if (s->queue.empty()) return false;
-                const auto& h = s->queue.front();
-                return s->in_flight == 0 || s->in_flight + h->bytes <= s->budget;
+                q = pick();
+                if (q) return s->in_flight == 0 || s->in_flight + q->front()->bytes <= s->budget;
+                return s->stop && s->q_dose.empty() && s->q_bl.empty();
             });
-            if (s->stop && s->queue.empty()) {
+            if (!q) {  // stopping, nothing queued
                 if (sb.fl) cccs_dev_free(ctx, sb.fl);
                 if (sb.dose) cccs_dev_free(ctx, sb.dose);
                 return;
             }
-            const int kind = s->queue.front()->kind;
             // a full-beam request fills the device by itself and shares no launches with another:
             // it runs alone on its slot so that it completes as soon as its own work does
-            const int cap = kind == CCCS_REQ_BEAM_DOSE ? 1 : s->max_batch;
-            while (!s->queue.empty() && (int)batch.size() < cap) {
-                const auto& h = s->queue.front();
-                if (h->kind != kind) break;
+            const int cap = q == &s->q_dose ? 1 : s->max_batch;
+            while (!q->empty() && (int)batch.size() < cap) {
+                const auto& h = q->front();
                 if (!batch.empty() && s->in_flight + bytes + h->bytes > s->budget) break;
                 bytes += h->bytes;
                 batch.push_back(h);
-                s->queue.pop_front();
+                q->pop_front();
             }
             s->in_flight += bytes;
+            if (q == &s->q_bl) s->bl_running++;
         }
         run_batch(s, ctx, sb, batch);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->in_flight -= bytes;
+            if (batch.front()->kind == CCCS_REQ_BEAMLET) s->bl_running--;
             const auto now = Clock::now();
             for (auto& r : batch) {
                 r->finished = now;
@@ -279,10 +293,10 @@ extern "C" int cccs_server_submit(cccs_server* s, int kind, const cccs_beam* bea
         r->id = s->next_id++;
         r->submitted = Clock::now();
         s->requests[r->id] = r;
-        s->queue.push_back(r);
+        (kind == CCCS_REQ_BEAM_DOSE ? s->q_dose : s->q_bl).push_back(r);
         *id = r->id;
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();  // the slot rules decide which waiting slot may take it
     return CCCS_OK;
 }
 
@@ -338,7 +352,7 @@ extern "C" int cccs_server_stats(cccs_server* s, cccs_server_info* out) {
     out->max_ms = v.empty() ? 0.0 : v.back();
     out->batches = s->batches;
     out->mean_batch = s->batches ? (double)s->batched / (double)s->batches : 0.0;
-    out->queued = s->queue.size();
+    out->queued = s->q_dose.size() + s->q_bl.size();
     return CCCS_OK;
 }
 

@@ -146,6 +146,18 @@ def make_beam(gantry_deg: float, couch_deg: float = 0.0, coll_deg: float = 0.0, 
     return b
 
 
+def retarget_beam(beam: abi.Beam, iso, sad: float, fmap_size, beamlet_size) -> abi.Beam:
+    """A copy of `beam` (same gantry, couch and collimator angles) with a new isocenter, SAD and
+    fluence-map geometry, reconfigured (BEAM::reconfigure, beam.cpp:300-331)."""
+    b = beam.copy()
+    b.isocenter[:] = [float(v) for v in iso]
+    b.sad = sad
+    b.fmap_size[:] = [int(v) for v in fmap_size]
+    b.beamlet_size[:] = [float(v) for v in beamlet_size]
+    abi.check(abi.load().cccs_beam_reconfigure(C.byref(b), 0), "cccs_beam_reconfigure")
+    return b
+
+
 def beams_array(beams: Sequence[abi.Beam]):
     arr = (abi.Beam * len(beams))()
     for i, b in enumerate(beams):
@@ -506,9 +518,18 @@ class Server:
                   "cccs_server_submit")
         return rid.value
 
-    def wait(self, rid: int, kind: int = abi.CCCS_REQ_BEAM_DOSE):
-        """(result, latency_ms, batch_size): a dose volume, or a list of (beamlet, keys, values)."""
+    def wait(self, rid: int, kind: int = abi.CCCS_REQ_BEAM_DOSE, counts_only: bool = False):
+        """(result, latency_ms, batch_size): a dose volume, or a list of (beamlet, keys, values); with
+        counts_only a beamlet request's columns go to the library's native counting sink and the
+        result is their totals."""
         lat, bs = C.c_double(0), C.c_int(0)
+        if kind == abi.CCCS_REQ_BEAMLET and counts_only:
+            tot = abi.SinkTotals()
+            sink = abi.SPARSE_SINK(C.cast(self.lib.cccs_sink_count, C.c_void_p).value)
+            abi.check(self.lib.cccs_server_wait(self.h, rid, None, sink, C.cast(C.pointer(tot), C.c_void_p),
+                                                C.byref(lat), C.byref(bs)), "cccs_server_wait")
+            return ({"columns": int(tot.columns), "nnz": int(tot.nnz), "value_sum": float(tot.value_sum)}, lat.value,
+                    bs.value)
         if kind == abi.CCCS_REQ_BEAM_DOSE:
             dose = np.zeros(self.spec.density.shape, np.float32)
             abi.check(self.lib.cccs_server_wait(self.h, rid, abi.fptr(dose), abi.SPARSE_SINK(), None, C.byref(lat),

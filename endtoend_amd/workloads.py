@@ -221,6 +221,26 @@ def cfg3(seed: int = SEED_ANGLES, nbeams: int = 50, kernel_extent: float = 1.0) 
     return Workload("cfg3-water128-ptv-50beams", spec, beams, fmaps, seed, ph)
 
 
+def cfg5(nbeams: int | None = None, stride: int = 1, indices=None, kernel_extent: float = 1.0) -> Workload:
+    """Full 4pi beamlet precompute: the cfg3 phantom, the 1162 beams of the 4pi candidate list (the
+    reference's CCCS/tests/input_files/beamlist_4pi.txt) aimed at the PTV centre, 64x64 maps of
+    0.25 cm beamlets open over the PTV projection, 96 cones (12 zenith x 8 azimuth).  `stride`
+    and `nbeams` take every stride-th beam of the list, up to nbeams of them, and `indices` names
+    list entries instead (bounded samples of the same workload)."""
+    ph = water_with_ptv()
+    spec = problem(ph, nphi=8, ntheta=12, kernel_extent=kernel_extent, max_rev_size=(800, 800, 800))
+    cands = engine.load_beam_list(os.path.join(DATA, "beamlist_4pi.txt"))
+    cands = [cands[i] for i in indices] if indices is not None else cands[::stride]
+    if nbeams is not None:
+        cands = cands[:nbeams]
+    beams, fmaps = [], []
+    for c in cands:
+        b = engine.retarget_beam(c, iso=ph.ptv[0], sad=100.0, fmap_size=(64, 64), beamlet_size=(0.25, 0.25))
+        beams.append(b)
+        fmaps.append(projection_mask(b, ph.ptv, (64, 64), (0.25, 0.25)))
+    return Workload(f"cfg5-water128-ptv-4pi-{len(beams)}beams", spec, beams, fmaps, None, ph)
+
+
 def small(n=32, voxel=0.5, gantry=0.0, couch=0.0, fmap=(8, 8), beamlet=(1.0, 1.0), nphi=4, ntheta=4,
           hetero=False, ss_factor=1, bbox=None, seed=7, weights="ones") -> Workload:
     """Small cases for parity tests: n^3 water (optionally with a lung block and a bone rod)."""

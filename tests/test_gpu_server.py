@@ -69,6 +69,31 @@ def test_queued_requests_are_batched():
 
 
 @pytest.mark.gpu
+def test_beam_requests_overtake_queued_beamlet_batches():
+    """Two slots, one beamlet request per batch: six beamlet requests are queued first, then a
+    full-beam request.  The beamlet batches run one at a time on one slot, the other slot stays
+    free for full-beam requests, so the full-beam request finishes before the last beamlet batch
+    and its dose equals the direct call's."""
+    import time
+    spec, reqs = _requests()
+    with Engine(spec) as eng:
+        ref = eng.beam_dose([reqs[0][0]], [reqs[0][1]])
+    with Server(spec, slots=2, max_batch=1) as srv:
+        srv.wait(srv.submit(*reqs[0]))                                    # warm both kinds
+        srv.wait(srv.submit(*reqs[0], abi.CCCS_REQ_BEAMLET), abi.CCCS_REQ_BEAMLET)
+        t0 = time.perf_counter()
+        bl = [(time.perf_counter(), srv.submit(*reqs[i % len(reqs)], abi.CCCS_REQ_BEAMLET)) for i in range(6)]
+        td = time.perf_counter()
+        rid = srv.submit(*reqs[0])
+        dose, lat, _ = srv.wait(rid)
+        fin = [ts + 1e-3 * srv.wait(r, abi.CCCS_REQ_BEAMLET)[1] - t0 for ts, r in bl]
+        st = srv.stats()
+    assert _same(dose, ref)
+    assert st["mean_batch"] == 1.0
+    assert td - t0 + 1e-3 * lat < max(fin)
+
+
+@pytest.mark.gpu
 def test_tight_hbm_budget_admits_one_request_at_a_time():
     spec, reqs = _requests()
     with Engine(spec) as eng:
