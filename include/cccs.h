@@ -248,7 +248,9 @@ int cccs_beamlet_footprint(const cccs_ctx* ctx, const cccs_beam* beam, const flo
  * beam's beamlet columns).  `slots` contexts (own stream each) run concurrently; a free slot
  * takes every queued beamlet request that fits and runs them as one multi-beam job (continuous
  * batching: requests that arrive while a batch runs join the next one), and full-beam requests
- * one at a time (each fills the device alone); a request is admitted only while the bytes
+ * one at a time (each fills the device alone).  Queued full-beam requests are taken first, and
+ * with two or more slots at most slots-1 run beamlet batches at once, so a full-beam request
+ * never waits behind a long beamlet batch.  A request is admitted only while the bytes
  * in flight plus its estimate stay within hbm_budget (a request above the whole budget runs
  * alone).  Results are the same as the direct calls (cccs_beam_dose / cccs_beamlet_dose) made
  * alone.  No counterpart in the reference (its dose calculation is one batch program). */
