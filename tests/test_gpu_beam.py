@@ -25,6 +25,7 @@ SMALL = {
                                   fmap=(10, 10), beamlet=(0.8, 0.8)),
     "hetero-g300-nt8-ss2": dict(n=24, voxel=0.5, gantry=300.0, hetero=True, nphi=8, ntheta=8, ss_factor=2),
     "bbox-subset-g120": dict(n=36, voxel=0.5, gantry=120.0, hetero=True, bbox=((4, 2, 6), (28, 30, 24))),
+    "hetero-g48-c12-96cones": dict(n=28, voxel=0.5, gantry=48.0, couch=12.0, hetero=True, nphi=8, ntheta=12),
 }
 
 

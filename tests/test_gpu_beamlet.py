@@ -39,6 +39,8 @@ CASES = {
                             (70, 70, 70), "all"),
     "bbox-subset-g120": (dict(n=32, voxel=0.5, gantry=120.0, hetero=True, fmap=(8, 8), beamlet=(1.0, 1.0),
                               bbox=((3, 2, 5), (26, 28, 22))), 1.0, None, "block"),
+    "hetero-g48-c12-96cones": (dict(n=28, voxel=0.5, gantry=48.0, couch=12.0, hetero=True, nphi=8, ntheta=12,
+                                    fmap=(6, 6), beamlet=(1.0, 1.0)), 1.5, None, "block"),
 }
 
 
