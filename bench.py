@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--plain-beam-only", action="store_true",
                     help="only the plain full-beam path on the thorax plan (profiling runs)")
     ap.add_argument("--serving-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--full-4pi-only", action="store_true",
+                    help="only the 4pi precompute sample with concurrent requests (profiling runs)")
     ap.add_argument("--autotune", action="store_true",
                     help="run the schedule autotuner on a sample of each workload before timing it")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -341,6 +343,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps(serving()))
+        return
+    if args.full_4pi_only:
+        print(json.dumps(full_4pi()))
         return
 
     from endtoend_amd import abi
