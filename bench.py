@@ -149,13 +149,14 @@ def plain_beam_path(steps: int = 2, autotune: bool = False):
             "schedule": tune}
 
 
-def full_4pi(stride: int = 12, rate: float = 20.0):
+def full_4pi(stride: int = 12, rate: float = 20.0, requests: bool = True):
     """Full 4pi precompute on a bounded sample: every stride-th beam of the 1162-beam 4pi list on
     the batched workload's phantom, 64x64 maps of 2.5 mm beamlets over the PTV projection, 96 cones.
     First the batched rate with the columns left in HBM; then the same beams as beamlet requests
     through the serving loop (columns to the host, native counting sink) while single-beam dose
     requests arrive at `rate`/s on the same resident state (Poisson; 4pi-list beams, 20^2-64^2 x
-    5 mm U[0,1] maps): the precompute's rate under that load and the requests' latency."""
+    5 mm U[0,1] maps): the precompute's rate under that load and the requests' latency (skipped
+    with requests=False)."""
     import threading
 
     from endtoend_amd import abi
@@ -170,7 +171,16 @@ def full_4pi(stride: int = 12, rate: float = 20.0):
         t0 = time.perf_counter()
         eng.beamlet_dose(w.beams, w.fmaps, threshold=THRESH, mode=FA, keep=None)
         eng.sync()
-        batched = nbl / (time.perf_counter() - t0)
+        span_batched = time.perf_counter() - t0
+        batched = nbl / span_batched
+    which = "all of the 1162-beam 4pi list" if stride == 1 else f"every {stride}th of the 1162-beam 4pi list"
+    desc = (f"{len(w.beams)} beams ({which}), 128^3 water @ 2.5 mm + "
+            "ellipsoidal PTV, 64x64 x 2.5 mm fluence open over the PTV projection, 96 cones, sparse "
+            "columns at 1e-4 of column max, faithful TERMA")
+    out = {"workload": desc, "beamlets": nbl,
+           "batched": {"value": batched, "unit": "beamlet-doses/s", "seconds": span_batched}}
+    if not requests:
+        return out
     rng = np.random.default_rng(W.SEED_ARRIVALS)
     reqs = []
     for i in rng.integers(0, len(w.beams), 400):
@@ -202,14 +212,11 @@ def full_4pi(stride: int = 12, rate: float = 20.0):
         span = time.perf_counter() - t0
         done.set()
         th.join()
-    return {"workload": f"{len(w.beams)} beams (every {stride}th of the 1162-beam 4pi list), 128^3 water @ 2.5 mm + "
-                        "ellipsoidal PTV, 64x64 x 2.5 mm fluence open over the PTV projection, 96 cones, sparse "
-                        "columns at 1e-4 of column max, faithful TERMA",
-            "beamlets": nbl, "batched": {"value": batched, "unit": "beamlet-doses/s"},
-            "with_requests": {"precompute_value": cols / span, "unit": "beamlet-doses/s", "columns": cols,
-                              "request_rate": rate, "requests": len(lat),
-                              "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99),
-                                            "max": max(lat) if lat else None}}}
+    out["with_requests"] = {"precompute_value": cols / span, "unit": "beamlet-doses/s", "columns": cols,
+                            "request_rate": rate, "requests": len(lat),
+                            "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99),
+                                           "max": max(lat) if lat else None}}
+    return out
 
 
 def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
@@ -326,6 +333,10 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="only the serving stream")
     ap.add_argument("--full-4pi-only", action="store_true",
                     help="only the 4pi precompute sample with concurrent requests (profiling runs)")
+    ap.add_argument("--full-4pi-stride", type=int, default=12,
+                    help="with --full-4pi-only: every n-th beam of the 1162-beam list (1 = the whole list)")
+    ap.add_argument("--full-4pi-no-requests", action="store_true",
+                    help="with --full-4pi-only: the batched precompute alone")
     ap.add_argument("--autotune", action="store_true",
                     help="run the schedule autotuner on a sample of each workload before timing it")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -345,7 +356,7 @@ def main():
         print(json.dumps(serving()))
         return
     if args.full_4pi_only:
-        print(json.dumps(full_4pi()))
+        print(json.dumps(full_4pi(stride=args.full_4pi_stride, requests=not args.full_4pi_no_requests)))
         return
 
     from endtoend_amd import abi
