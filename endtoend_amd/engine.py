@@ -358,10 +358,12 @@ class Engine:
         return out[:n.value]
 
     def beamlet_dose(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray], threshold: float = 1e-4,
-                     mode: int = abi.CCCS_BEAMLET_FAITHFUL, keep: bool = True):
-        """Sparse columns of every active beamlet: a list of (beam, beamlet, keys, values); with
-        keep=False only the totals (the library's native counting sink, no Python per column); with
-        keep=None the columns are built on the device and not copied out."""
+                     mode: int = abi.CCCS_BEAMLET_FAITHFUL, keep: bool = True, percs: bool = False):
+        """Sparse columns of every active beamlet: a list of (beam, beamlet, keys, values), plus the
+        row-block sizes of ROI-reduced columns, plus (perc_nonzero, perc_dropped, perc_reducedrop)
+        as the last element when percs=True; with keep=False only the totals (the library's native
+        counting sink, no Python per column); with keep=None the columns are built on the device and
+        not copied out."""
         fm = [np.ascontiguousarray(f, np.float32) for f in fmaps]
         ptrs = (abi._fp * len(fm))(*[abi.fptr(f) for f in fm])
         arr = beams_array(beams)
@@ -381,11 +383,12 @@ class Engine:
             c = colp.contents
             k = np.ctypeslib.as_array(c.keys, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.uint64)
             v = np.ctypeslib.as_array(c.values, shape=(c.nnz,)).copy() if c.nnz else np.zeros(0, np.float32)
+            col = (c.beam_index, c.beamlet_index, k, v)
             if c.nblocks:  # ROI-reduced column: its row-block sizes as a fifth element
-                blocks = np.ctypeslib.as_array(c.row_block_sizes, shape=(c.nblocks,)).copy()
-                cols.append((c.beam_index, c.beamlet_index, k, v, blocks))
-            else:
-                cols.append((c.beam_index, c.beamlet_index, k, v))
+                col += (np.ctypeslib.as_array(c.row_block_sizes, shape=(c.nblocks,)).copy(),)
+            if percs:
+                col += ((float(c.perc_nonzero), float(c.perc_dropped), float(c.perc_reducedrop)),)
+            cols.append(col)
             return 0
 
         cb = abi.SPARSE_SINK(sink_fn)

@@ -30,6 +30,14 @@
 #include <omp.h>
 #endif
 
+// Mutants (test infrastructure): ORC_MUTANT = 1, 2 or 3 builds the oracle with one deliberate
+// mistake -- the cone's kernel row taken from its phi index, the beamlet walk on the raw radial
+// boundaries, the beamlet unpack half a sample off along the beamlet axis -- so the tests can
+// show that their checks catch it (tests/test_oracle_composition.py).
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 namespace {
 
 // float PI as the reference's CUDART_PI_F (include/dosecalc_defs.h:8)
@@ -158,7 +166,11 @@ struct Prob {
     int nrays;
     float theta(int ray) const { return p->conv_theta_deg[ray] * kPI / 180.f; }   // brain_defs.h:67-73
     float phi(int ray) const { return p->conv_phi_deg[ray] * kPI / 180.f; }
-    int kern_index(int ray) const { return ray % (int)(p->ntheta / 2); }
+#if ORC_MUTANT == 1
+    int kern_index(int ray) const { return ray / (int)(p->ntheta / 2); }  // mutant: the phi index
+#else
+    int kern_index(int ray) const { return ray % (int)(p->ntheta / 2); }  // get_kernel_theta_index
+#endif
     explicit Prob(const cccs_problem* pp) : p(pp) {
         dens = Vol{pp->density, (int)pp->size[0], (int)pp->size[1], (int)pp->size[2]};
         start = from_f(pp->start);
@@ -704,6 +716,262 @@ struct PillarGrid {
     std::vector<f2> angles;
 };
 
+// Test hooks of the beamlet path (the oracle's own instrumentation, not reference behaviour).
+struct BeamletHooks {
+    // stage dump: REV density, TERMA and cone dose of batch 0 for this cone, then stop
+    int stage_ray = -1;
+    float *rd = nullptr, *rt = nullptr, *rdo = nullptr;
+    uint64_t cap = 0;
+    uint32_t* rev_size = nullptr;
+    // linear field: every packed node gets linear[0] + (linear[1], linear[2], linear[3]) . X(node), X the
+    // PVCS point the raytrace assigns to the node, instead of the cone dose; then the unpack runs
+    const float* linear = nullptr;
+};
+
+// PVCS point of pillar-local REV coordinate inp (the density point of cudaBeamletRaytrace,
+// dosecalc-beamlet/nvbbRayConvolve_device.cu:213-219)
+inline f3 pillar_point(f3 inp, f3 rvs, f2 a, const cccs_beam& b, f3 pst) {
+    return invRotateBeam(invRotateBeamlet(inp * rvs, a.x, a.y), b.azimuth, b.zenith, b.coll) + pst;
+}
+
+// dosecalc-beamlet radconvolveTexture (nvbbRayConvolve_host.cu:555-1349) for one beam.
+int beamlet_run(const cccs_problem* p, const cccs_beam& b, const float* fmap, float* out, int max_beamlets,
+                int* nactive, orc_stats* st, const BeamletHooks& hk) {
+    Prob P(p);
+    SpecT sp = spec_of(p);
+    f3 end_vol = volume_end(P);
+    f3 rvs = mk(p->rev_latspacing, p->rev_longspacing, p->rev_latspacing);
+    f3 rrv = rcp3(rvs);
+    f3 src = from_f(b.source), iso = from_f(b.isocenter);
+    // KERN_RADII = mean radii (dosecalc-beamlet/nvbbRayConvolve_host.cu:351-357)
+    std::vector<float> R(p->nradii);
+#if ORC_MUTANT == 2
+    for (uint32_t r = 0; r < p->nradii; r++) R[r] = p->radii[r];  // mutant: the beam path's raw radii
+#else
+    R[0] = (float)(0.5 * p->radii[0]);
+    for (uint32_t r = 1; r < p->nradii; r++) R[r] = (float)(0.5 * (p->radii[r] + p->radii[r - 1]));
+#endif
+    // active beamlets (:575-581)
+    PillarGrid H;
+    for (uint32_t i = 0; i < b.fmap_size[0] * b.fmap_size[1]; i++)
+        if (fmap[i] > 0) H.idx.push_back((int)i);
+    H.n = (int)H.idx.size();
+    if (nactive) *nactive = H.n;
+    if (H.n == 0) return hk.stage_ray >= 0 ? CCCS_EINVAL : CCCS_OK;
+    const bool stage = hk.stage_ray >= 0;
+    if (!stage && H.n > max_beamlets) return CCCS_EINVAL;
+    const size_t nbb = (size_t)p->bbox_size[0] * p->bbox_size[1] * p->bbox_size[2];
+    if (!stage) std::memset(out, 0, sizeof(float) * nbb * H.n);
+    // anchors, lengths, divergence sizes (:609-636)
+    std::vector<f3> bstart(H.n), bend(H.n);
+    std::vector<float> blen(H.n);
+    H.angles.resize(H.n);
+    float max_len = 0.f;
+    f2 max_bs{0.f, 0.f};
+    f2 bsz{b.beamlet_size[0], b.beamlet_size[1]};
+    for (int ii = 0; ii < H.n; ii++) {
+        f3 biso;
+        if (!beamlet_anchors(bstart[ii], bend[ii], H.angles[ii], biso, src, iso, (unsigned)H.idx[ii], bsz, b.fmap_size,
+                             P.voxel, P.start, p->bbox_start, p->bbox_size, b.azimuth, b.zenith, b.coll))
+            return CCCS_EGEOM;
+        blen[ii] = length(bend[ii] - bstart[ii]);
+        if (blen[ii] > max_len) max_len = blen[ii];
+        float ratio_num = length(bend[ii] - src), ratio_den = length(biso - src);
+        f2 dv{bsz.x * ratio_num / ratio_den, bsz.y * ratio_num / ratio_den};
+        max_bs.x = std::max(max_bs.x, dv.x);
+        max_bs.y = std::max(max_bs.y, dv.y);
+    }
+    // pillar size (:638-649)
+    float psize_long = max_len + 2.f * p->kernel_extent + H.wall * rvs.y;
+    f2 psize_trans{max_bs.x + 2.f * p->kernel_extent + H.wall * rvs.x, max_bs.y + 2.f * p->kernel_extent + H.wall * rvs.x};
+    f3 expand = rvs - mk(fmodf(psize_trans.x, rvs.x), fmodf(psize_long, rvs.y), fmodf(psize_trans.y, rvs.z));
+    H.psize = mk(psize_trans.x, psize_long, psize_trans.y) + expand + 2.f * float(H.buffer) * rvs;
+    f3 pdf = H.psize / rvs;
+    H.pd[0] = (int)pdf.x; H.pd[1] = (int)pdf.y; H.pd[2] = (int)pdf.z;
+    // pillar start coords (:652-661)
+    H.pstart.resize(H.n);
+    for (int ii = 0; ii < H.n; ii++) {
+        f3 go = mk(-0.5f * (H.psize.x + H.wall * rvs.x), -0.5f * (H.psize.y - blen[ii] + H.wall * rvs.y),
+                   -0.5f * (H.psize.z + H.wall * rvs.z));
+        go = invRotateBeam(invRotateBeamlet(go, H.angles[ii].x, H.angles[ii].y), b.azimuth, b.zenith, b.coll);
+        H.pstart[ii] = bstart[ii] + go;
+    }
+    const uint32_t* max_rev = p->max_rev_size;
+    uint32_t mr_default[3] = {500, 500, 500};  // argparser default (dosecalc-preprocess/argparser.h:29)
+    if (max_rev[0] == 0) max_rev = mr_default;
+    // batching with retries on ArraySizeError (:692-834, allocate_batch_sizes :69-80)
+    int nb = 1;
+    std::vector<int> bsizes;
+    for (;;) {
+        if (nb > 100) return CCCS_ENOMEM;
+        bsizes.assign(nb, H.n / nb);
+        int rem = H.n - (H.n / nb) * nb, mk_ = 0;
+        while (rem > 0) { bsizes[(mk_++) % nb]++; rem--; }
+        bool ok = true;
+        for (int bi = 0; bi < nb && ok; bi++) {
+            int n = bsizes[bi];
+            int npx = (int)ceilf(sqrtf((float)n));
+            int npy = (int)ceilf((float)n / npx);
+            f3 gsize = mk((float)(npx * H.pd[0] + H.wall), (float)(H.pd[1] + H.wall), (float)(npy * H.pd[2] + H.wall)) * rvs;
+            for (int r = 0; r < P.nrays && ok; r++) {
+                Rev rv;
+                if (!find_rev(rv, p, max_rev, gsize, P.theta(r), P.phi(r))) ok = false;
+                else if (rv.size[0] > 1024) return CCCS_EGEOM;
+            }
+        }
+        if (ok) break;
+        nb++;
+    }
+    int memidx = 0;
+    for (int bi = 0; bi < nb; bi++) {
+        const int n = bsizes[bi];
+        const int* bidx = H.idx.data() + memidx;
+        const f3* pst = H.pstart.data() + memidx;
+        const f2* ang = H.angles.data() + memidx;
+        int npx = (int)ceilf(sqrtf((float)n));
+        int npy = (int)ceilf((float)n / npx);
+        int gd[3] = {npx * H.pd[0] + H.wall, H.pd[1] + H.wall, npy * H.pd[2] + H.wall};
+        f3 gsize = mk((float)gd[0], (float)gd[1], (float)gd[2]) * rvs;
+        f3 gdf = mk((float)gd[0], (float)gd[1], (float)gd[2]);
+        f3 pdf3 = mk((float)H.pd[0], (float)H.pd[1], (float)H.pd[2]);
+        // packed BEV dose, layout [pX][pZ][pY] (PackedREVtoBEVdose, device.cu:422)
+        std::vector<float> pdose((size_t)gd[0] * gd[1] * gd[2], 0.f);
+        for (int ray = 0; ray < P.nrays && !hk.linear; ray++) {
+            if (stage && ray != hk.stage_ray) continue;
+            Rev rev;
+            find_rev(rev, p, max_rev, gsize, P.theta(ray), P.phi(ray));
+            const int nx = (int)rev.size[0], ny = (int)rev.size[1], nz = (int)rev.size[2];
+            float kt = P.theta(ray), kp = P.phi(ray);
+            size_t nrev = (size_t)nx * ny * nz;
+            if (stage) {
+                for (int k = 0; k < 3; k++) hk.rev_size[k] = rev.size[k];
+                if (nrev > hk.cap || !hk.rd || !hk.rt || !hk.rdo) return CCCS_EINVAL;
+            }
+            std::vector<float> rd(nrev), rt(nrev), rdo(nrev);
+            uint64_t valid = 0, steps = 0;
+            // cudaBeamletRaytrace (device.cu:19-242), ss_factor forced to 1 (host.cu:747)
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : valid, steps)
+            for (int tZ = 0; tZ < nz; tZ++)
+                for (int tY = 0; tY < ny; tY++)
+                    for (int tX = 0; tX < nx; tX++) {
+                        size_t o = (size_t)tX + (size_t)nx * ((size_t)tY + (size_t)ny * tZ);
+                        float vd = -1e-12f, vt = 0.f;
+                        f3 pb = invRotateKernel(fma3(rev.min_c, rrv, mk((float)tZ, (float)tX, (float)tY)), kt, kp) + 0.5f;
+                        bool act = !(pb.x < 0.f || pb.y < 0.f || pb.z < 0.f || pb.x >= gdf.x || pb.y >= gdf.y || pb.z >= gdf.z);
+                        int pidx = 0;
+                        f3 inp{0, 0, 0};
+                        if (act) {
+                            int pix = f2i_rd(pb.x / pdf3.x), piy = f2i_rd(pb.z / pdf3.z);
+                            pidx = pix + piy * npx;
+                            if (pidx >= n) act = false;
+                        }
+                        if (act) {
+                            inp = mk(fmodf(pb.x, pdf3.x), fmodf(pb.y, pdf3.y), fmodf(pb.z, pdf3.z));
+                            if (inp.x < H.wall || inp.y < H.wall || inp.z < H.wall) act = false;
+                        }
+                        if (act) {
+                            inp = inp - 0.5f;
+                            f2 a = ang[pidx];
+                            f3 gcs = invRotateBeam(invRotateBeamlet(inp, a.x, a.y), b.azimuth, b.zenith, b.coll);
+                            gcs = fma3(gcs, rvs, pst[pidx]);
+                            vt = siddon_terma(gcs, fmap, P.start, end_vol, P.voxel, p->beamhard_correct, b, sp, P.dens,
+                                              bidx[pidx], &steps, nullptr);
+                            f3 gi = (pillar_point(inp, rvs, a, b, pst[pidx]) - P.start) / P.voxel;
+                            vd = tex3d(P.dens, gi.x + 0.5f, gi.y + 0.5f, gi.z + 0.5f);
+                            valid++;
+                        }
+                        rd[o] = vd;
+                        rt[o] = vt;
+                    }
+            // PackRowConvolve (device.cu:253-418)
+            uint64_t iters = 0;
+            int Pk = P.kern_index(ray);
+#pragma omp parallel for collapse(2) schedule(dynamic, 8) reduction(+ : iters)
+            for (int tZ = 0; tZ < nz; tZ++)
+                for (int tY = 0; tY < ny; tY++) {
+                    size_t base = (size_t)nx * ((size_t)tY + (size_t)ny * tZ);
+                    for (int tX = 0; tX < nx; tX++)
+                        rdo[base + tX] = pack_row_convolve_one(rd.data() + base, rt.data() + base, nx, tX, p->kernel,
+                                                               (int)p->ntheta, (int)p->nradii, R.data(), (float)Pk,
+                                                               p->rev_longspacing, (int)p->nphi, &iters);
+                }
+            if (stage) {
+                std::memcpy(hk.rd, rd.data(), nrev * sizeof(float));
+                std::memcpy(hk.rt, rt.data(), nrev * sizeof(float));
+                std::memcpy(hk.rdo, rdo.data(), nrev * sizeof(float));
+                return CCCS_OK;
+            }
+            // PackedREVtoBEVdose (device.cu:427-452)
+            Vol rv{rdo.data(), nx, ny, nz};
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int pX = 0; pX < gd[0]; pX++)
+                for (int pZ = 0; pZ < gd[2]; pZ++)
+                    for (int pY = 0; pY < gd[1]; pY++) {
+                        f3 ri = fma3(-rev.min_c, rrv, rotateKernel(mk((float)pX, (float)pY, (float)pZ), kt, kp));
+                        float dc = tex3d(rv, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
+                        if (dc <= 0.f) continue;
+                        pdose[(size_t)pY + (size_t)gd[1] * ((size_t)pZ + (size_t)gd[2] * pX)] += dc;
+                    }
+            if (st) {
+                st->rev_samples += nrev;
+                st->rev_valid += valid;
+                st->siddon_steps += steps;
+                st->cone_iters += iters;
+                st->cone_iters_all += iters;
+            }
+        }
+        if (stage) return CCCS_EINVAL;  // unreachable: batch 0 holds every ray
+        if (hk.linear) {
+            // node (pX, pY, pZ) of pillar (pix, piy) is pillar-local REV coordinate (pX - pix*pd, pY, pZ - piy*pd)
+            const float* c = hk.linear;
+            for (int pX = 0; pX < gd[0]; pX++)
+                for (int pZ = 0; pZ < gd[2]; pZ++) {
+                    int pix = pX / H.pd[0], piy = pZ / H.pd[2], pi = pix + piy * npx;
+                    if (pix >= npx || pi >= n) continue;
+                    for (int pY = 0; pY < gd[1]; pY++) {
+                        f3 x = pillar_point(mk((float)(pX - pix * H.pd[0]), (float)pY, (float)(pZ - piy * H.pd[2])), rvs,
+                                            ang[pi], b, pst[pi]);
+                        pdose[(size_t)pY + (size_t)gd[1] * ((size_t)pZ + (size_t)gd[2] * pX)] =
+                            c[0] + c[1] * x.x + c[2] * x.y + c[3] * x.z;
+                    }
+                }
+        }
+        // UnpackBEVDosePillar (device.cu:459-521), one dense calc_bbox column per beamlet
+        Vol pv{pdose.data(), gd[1], gd[2], gd[0]};
+        const int bx = (int)p->bbox_size[0], by = (int)p->bbox_size[1], bz = (int)p->bbox_size[2];
+        f3 fbs = from_u3(p->bbox_start);
+        const int lo = H.wall + H.buffer;
+        for (int pi = 0; pi < n; pi++) {
+            float* col = out + (size_t)(memidx + pi) * nbb;
+            int pix = pi % npx, piy = pi / npx;
+            f2 a = ang[pi];
+            f3 ps = pst[pi];
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int tZ = 0; tZ < bz; tZ++)
+                for (int tY = 0; tY < by; tY++)
+                    for (int tX = 0; tX < bx; tX++) {
+                        f3 gc = (mk((float)tX, (float)tY, (float)tZ) + fbs) * P.voxel + P.start;
+#if ORC_MUTANT == 3
+                        // mutant: the half-texel shift dropped along the beamlet axis
+                        f3 pidx = rotateBeamlet(rotateBeam(gc - ps, b.azimuth, b.zenith, b.coll), a.x, a.y) / rvs +
+                                  mk(0.5f, 0.f, 0.5f);
+#else
+                        f3 pidx = rotateBeamlet(rotateBeam(gc - ps, b.azimuth, b.zenith, b.coll), a.x, a.y) / rvs + 0.5f;
+#endif
+                        if (pidx.x < lo || pidx.y < lo || pidx.z < lo || pidx.x >= (H.pd[0] - H.buffer) ||
+                            pidx.y >= (H.pd[1] - H.buffer) || pidx.z >= (H.pd[2] - H.buffer))
+                            continue;
+                        pidx = mk(pidx.x + pix * H.pd[0], pidx.y, pidx.z + piy * H.pd[2]);
+                        float dc = tex3d(pv, pidx.y, pidx.z, pidx.x);
+                        if (dc <= 0.f) continue;
+                        col[(size_t)tX + (size_t)bx * ((size_t)tY + (size_t)by * tZ)] = dc;
+                    }
+        }
+        memidx += n;
+    }
+    return CCCS_OK;
+}
+
 }  // namespace
 
 // ================================ extern "C" API =============================================
@@ -994,204 +1262,31 @@ int orc_beamlet_anchor(const cccs_problem* p, const cccs_beam* beamp, int idx, f
 int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beamp, const float* fmap, int mode, float* out,
                       int max_beamlets, int* nactive, orc_stats* st) {
     if (!problem_ok(p) || !beamp || !fmap || mode != CCCS_BEAMLET_FAITHFUL) return CCCS_EINVAL;
-    const cccs_beam& b = *beamp;
-    Prob P(p);
-    SpecT sp = spec_of(p);
-    f3 end_vol = volume_end(P);
-    f3 rvs = mk(p->rev_latspacing, p->rev_longspacing, p->rev_latspacing);
-    f3 rrv = rcp3(rvs);
-    f3 src = from_f(b.source), iso = from_f(b.isocenter);
-    // KERN_RADII = mean radii (dosecalc-beamlet/nvbbRayConvolve_host.cu:351-357)
-    std::vector<float> R(p->nradii);
-    R[0] = (float)(0.5 * p->radii[0]);
-    for (uint32_t r = 1; r < p->nradii; r++) R[r] = (float)(0.5 * (p->radii[r] + p->radii[r - 1]));
-    // active beamlets (:575-581)
-    PillarGrid H;
-    for (uint32_t i = 0; i < b.fmap_size[0] * b.fmap_size[1]; i++)
-        if (fmap[i] > 0) H.idx.push_back((int)i);
-    H.n = (int)H.idx.size();
-    if (nactive) *nactive = H.n;
-    if (H.n == 0) return CCCS_OK;
-    if (H.n > max_beamlets) return CCCS_EINVAL;
-    const size_t nbb = (size_t)p->bbox_size[0] * p->bbox_size[1] * p->bbox_size[2];
-    std::memset(out, 0, sizeof(float) * nbb * H.n);
-    // anchors, lengths, divergence sizes (:609-636)
-    std::vector<f3> bstart(H.n), bend(H.n);
-    std::vector<float> blen(H.n);
-    H.angles.resize(H.n);
-    float max_len = 0.f;
-    f2 max_bs{0.f, 0.f};
-    f2 bsz{b.beamlet_size[0], b.beamlet_size[1]};
-    for (int ii = 0; ii < H.n; ii++) {
-        f3 biso;
-        if (!beamlet_anchors(bstart[ii], bend[ii], H.angles[ii], biso, src, iso, (unsigned)H.idx[ii], bsz, b.fmap_size,
-                             P.voxel, P.start, p->bbox_start, p->bbox_size, b.azimuth, b.zenith, b.coll))
-            return CCCS_EGEOM;
-        blen[ii] = length(bend[ii] - bstart[ii]);
-        if (blen[ii] > max_len) max_len = blen[ii];
-        float ratio_num = length(bend[ii] - src), ratio_den = length(biso - src);
-        f2 dv{bsz.x * ratio_num / ratio_den, bsz.y * ratio_num / ratio_den};
-        max_bs.x = std::max(max_bs.x, dv.x);
-        max_bs.y = std::max(max_bs.y, dv.y);
-    }
-    // pillar size (:638-649)
-    float psize_long = max_len + 2.f * p->kernel_extent + H.wall * rvs.y;
-    f2 psize_trans{max_bs.x + 2.f * p->kernel_extent + H.wall * rvs.x, max_bs.y + 2.f * p->kernel_extent + H.wall * rvs.x};
-    f3 expand = rvs - mk(fmodf(psize_trans.x, rvs.x), fmodf(psize_long, rvs.y), fmodf(psize_trans.y, rvs.z));
-    H.psize = mk(psize_trans.x, psize_long, psize_trans.y) + expand + 2.f * float(H.buffer) * rvs;
-    f3 pdf = H.psize / rvs;
-    H.pd[0] = (int)pdf.x; H.pd[1] = (int)pdf.y; H.pd[2] = (int)pdf.z;
-    // pillar start coords (:652-661)
-    H.pstart.resize(H.n);
-    for (int ii = 0; ii < H.n; ii++) {
-        f3 go = mk(-0.5f * (H.psize.x + H.wall * rvs.x), -0.5f * (H.psize.y - blen[ii] + H.wall * rvs.y),
-                   -0.5f * (H.psize.z + H.wall * rvs.z));
-        go = invRotateBeam(invRotateBeamlet(go, H.angles[ii].x, H.angles[ii].y), b.azimuth, b.zenith, b.coll);
-        H.pstart[ii] = bstart[ii] + go;
-    }
-    const uint32_t* max_rev = p->max_rev_size;
-    uint32_t mr_default[3] = {500, 500, 500};  // argparser default (dosecalc-preprocess/argparser.h:29)
-    if (max_rev[0] == 0) max_rev = mr_default;
-    // batching with retries on ArraySizeError (:692-834, allocate_batch_sizes :69-80)
-    int nb = 1;
-    std::vector<int> bsizes;
-    std::vector<Rev> revs;
-    for (;;) {
-        if (nb > 100) return CCCS_ENOMEM;
-        bsizes.assign(nb, H.n / nb);
-        int rem = H.n - (H.n / nb) * nb, mk_ = 0;
-        while (rem > 0) { bsizes[(mk_++) % nb]++; rem--; }
-        bool ok = true;
-        for (int bi = 0; bi < nb && ok; bi++) {
-            int n = bsizes[bi];
-            int npx = (int)ceilf(sqrtf((float)n));
-            int npy = (int)ceilf((float)n / npx);
-            f3 gsize = mk((float)(npx * H.pd[0] + H.wall), (float)(H.pd[1] + H.wall), (float)(npy * H.pd[2] + H.wall)) * rvs;
-            for (int r = 0; r < P.nrays && ok; r++) {
-                Rev rv;
-                if (!find_rev(rv, p, max_rev, gsize, P.theta(r), P.phi(r))) ok = false;
-                else if (rv.size[0] > 1024) return CCCS_EGEOM;
-            }
-        }
-        if (ok) break;
-        nb++;
-    }
-    int memidx = 0;
-    for (int bi = 0; bi < nb; bi++) {
-        const int n = bsizes[bi];
-        const int* bidx = H.idx.data() + memidx;
-        const f3* pst = H.pstart.data() + memidx;
-        const f2* ang = H.angles.data() + memidx;
-        int npx = (int)ceilf(sqrtf((float)n));
-        int npy = (int)ceilf((float)n / npx);
-        int gd[3] = {npx * H.pd[0] + H.wall, H.pd[1] + H.wall, npy * H.pd[2] + H.wall};
-        f3 gsize = mk((float)gd[0], (float)gd[1], (float)gd[2]) * rvs;
-        f3 gdf = mk((float)gd[0], (float)gd[1], (float)gd[2]);
-        f3 pdf3 = mk((float)H.pd[0], (float)H.pd[1], (float)H.pd[2]);
-        // packed BEV dose, layout [pX][pZ][pY] (PackedREVtoBEVdose, device.cu:422)
-        std::vector<float> pdose((size_t)gd[0] * gd[1] * gd[2], 0.f);
-        for (int ray = 0; ray < P.nrays; ray++) {
-            Rev rev;
-            find_rev(rev, p, max_rev, gsize, P.theta(ray), P.phi(ray));
-            const int nx = (int)rev.size[0], ny = (int)rev.size[1], nz = (int)rev.size[2];
-            float kt = P.theta(ray), kp = P.phi(ray);
-            size_t nrev = (size_t)nx * ny * nz;
-            std::vector<float> rd(nrev), rt(nrev), rdo(nrev);
-            uint64_t valid = 0, steps = 0;
-            // cudaBeamletRaytrace (device.cu:19-242), ss_factor forced to 1 (host.cu:747)
-#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : valid, steps)
-            for (int tZ = 0; tZ < nz; tZ++)
-                for (int tY = 0; tY < ny; tY++)
-                    for (int tX = 0; tX < nx; tX++) {
-                        size_t o = (size_t)tX + (size_t)nx * ((size_t)tY + (size_t)ny * tZ);
-                        float vd = -1e-12f, vt = 0.f;
-                        f3 pb = invRotateKernel(fma3(rev.min_c, rrv, mk((float)tZ, (float)tX, (float)tY)), kt, kp) + 0.5f;
-                        bool act = !(pb.x < 0.f || pb.y < 0.f || pb.z < 0.f || pb.x >= gdf.x || pb.y >= gdf.y || pb.z >= gdf.z);
-                        int pidx = 0;
-                        f3 inp{0, 0, 0};
-                        if (act) {
-                            int pix = f2i_rd(pb.x / pdf3.x), piy = f2i_rd(pb.z / pdf3.z);
-                            pidx = pix + piy * npx;
-                            if (pidx >= n) act = false;
-                        }
-                        if (act) {
-                            inp = mk(fmodf(pb.x, pdf3.x), fmodf(pb.y, pdf3.y), fmodf(pb.z, pdf3.z));
-                            if (inp.x < H.wall || inp.y < H.wall || inp.z < H.wall) act = false;
-                        }
-                        if (act) {
-                            inp = inp - 0.5f;
-                            f2 a = ang[pidx];
-                            f3 gcs = invRotateBeam(invRotateBeamlet(inp, a.x, a.y), b.azimuth, b.zenith, b.coll);
-                            gcs = fma3(gcs, rvs, pst[pidx]);
-                            vt = siddon_terma(gcs, fmap, P.start, end_vol, P.voxel, p->beamhard_correct, b, sp, P.dens,
-                                              bidx[pidx], &steps, nullptr);
-                            f3 g2 = invRotateBeam(invRotateBeamlet(inp * rvs, a.x, a.y), b.azimuth, b.zenith, b.coll) + pst[pidx];
-                            f3 gi = (g2 - P.start) / P.voxel;
-                            vd = tex3d(P.dens, gi.x + 0.5f, gi.y + 0.5f, gi.z + 0.5f);
-                            valid++;
-                        }
-                        rd[o] = vd;
-                        rt[o] = vt;
-                    }
-            // PackRowConvolve (device.cu:253-418)
-            uint64_t iters = 0;
-            int Pk = P.kern_index(ray);
-#pragma omp parallel for collapse(2) schedule(dynamic, 8) reduction(+ : iters)
-            for (int tZ = 0; tZ < nz; tZ++)
-                for (int tY = 0; tY < ny; tY++) {
-                    size_t base = (size_t)nx * ((size_t)tY + (size_t)ny * tZ);
-                    for (int tX = 0; tX < nx; tX++)
-                        rdo[base + tX] = pack_row_convolve_one(rd.data() + base, rt.data() + base, nx, tX, p->kernel,
-                                                               (int)p->ntheta, (int)p->nradii, R.data(), (float)Pk,
-                                                               p->rev_longspacing, (int)p->nphi, &iters);
-                }
-            // PackedREVtoBEVdose (device.cu:427-452)
-            Vol rv{rdo.data(), nx, ny, nz};
-#pragma omp parallel for collapse(2) schedule(static)
-            for (int pX = 0; pX < gd[0]; pX++)
-                for (int pZ = 0; pZ < gd[2]; pZ++)
-                    for (int pY = 0; pY < gd[1]; pY++) {
-                        f3 ri = fma3(-rev.min_c, rrv, rotateKernel(mk((float)pX, (float)pY, (float)pZ), kt, kp));
-                        float dc = tex3d(rv, ri.y + 0.5f, ri.z + 0.5f, ri.x + 0.5f);
-                        if (dc <= 0.f) continue;
-                        pdose[(size_t)pY + (size_t)gd[1] * ((size_t)pZ + (size_t)gd[2] * pX)] += dc;
-                    }
-            if (st) {
-                st->rev_samples += nrev;
-                st->rev_valid += valid;
-                st->siddon_steps += steps;
-                st->cone_iters += iters;
-                st->cone_iters_all += iters;
-            }
-        }
-        // UnpackBEVDosePillar (device.cu:459-521), one dense calc_bbox column per beamlet
-        Vol pv{pdose.data(), gd[1], gd[2], gd[0]};
-        const int bx = (int)p->bbox_size[0], by = (int)p->bbox_size[1], bz = (int)p->bbox_size[2];
-        f3 fbs = from_u3(p->bbox_start);
-        const int lo = H.wall + H.buffer;
-        for (int pi = 0; pi < n; pi++) {
-            float* col = out + (size_t)(memidx + pi) * nbb;
-            int pix = pi % npx, piy = pi / npx;
-            f2 a = ang[pi];
-            f3 ps = pst[pi];
-#pragma omp parallel for collapse(2) schedule(static)
-            for (int tZ = 0; tZ < bz; tZ++)
-                for (int tY = 0; tY < by; tY++)
-                    for (int tX = 0; tX < bx; tX++) {
-                        f3 gc = (mk((float)tX, (float)tY, (float)tZ) + fbs) * P.voxel + P.start;
-                        f3 pidx = rotateBeamlet(rotateBeam(gc - ps, b.azimuth, b.zenith, b.coll), a.x, a.y) / rvs + 0.5f;
-                        if (pidx.x < lo || pidx.y < lo || pidx.z < lo || pidx.x >= (H.pd[0] - H.buffer) ||
-                            pidx.y >= (H.pd[1] - H.buffer) || pidx.z >= (H.pd[2] - H.buffer))
-                            continue;
-                        pidx = mk(pidx.x + pix * H.pd[0], pidx.y, pidx.z + piy * H.pd[2]);
-                        float dc = tex3d(pv, pidx.y, pidx.z, pidx.x);
-                        if (dc <= 0.f) continue;
-                        col[(size_t)tX + (size_t)bx * ((size_t)tY + (size_t)by * tZ)] = dc;
-                    }
-        }
-        memidx += n;
-    }
-    return CCCS_OK;
+    BeamletHooks h;
+    return beamlet_run(p, *beamp, fmap, out, max_beamlets, nactive, st, h);
+}
+
+int orc_beamlet_cone_stage(const cccs_problem* p, const cccs_beam* beamp, const float* fmap, int ray, float* rev_dens,
+                           float* rev_terma, float* rev_dose, uint64_t cap, uint32_t* rev_size) {
+    if (!problem_ok(p) || !beamp || !fmap || !rev_size) return CCCS_EINVAL;
+    if (ray < 0 || ray >= (int)(p->nphi * (p->ntheta / 2))) return CCCS_EINVAL;
+    BeamletHooks h;
+    h.stage_ray = ray;
+    h.rd = rev_dens;
+    h.rt = rev_terma;
+    h.rdo = rev_dose;
+    h.cap = cap;
+    h.rev_size = rev_size;
+    int nact = 0;
+    return beamlet_run(p, *beamp, fmap, nullptr, 0, &nact, nullptr, h);
+}
+
+int orc_beamlet_unpack_linear(const cccs_problem* p, const cccs_beam* beamp, const float* fmap, const float* coef,
+                              float* out, int max_beamlets, int* nactive) {
+    if (!problem_ok(p) || !beamp || !fmap || !coef || !out) return CCCS_EINVAL;
+    BeamletHooks h;
+    h.linear = coef;
+    return beamlet_run(p, *beamp, fmap, out, max_beamlets, nactive, nullptr, h);
 }
 
 // FCBB fluence-map convolution (README.md:15-20; no source in the reference): zero-padded, same

@@ -87,6 +87,22 @@ int orc_pack_row_convolve(const float* dens, const float* terma, int n, const fl
 int orc_beamlet_dense(const cccs_problem* p, const cccs_beam* beam, const float* fmap, int mode,
                       float* out, int max_beamlets, int* nactive, orc_stats* stats);
 
+/* Stage dump of the beamlet path for one cone (oracle instrumentation): batch 0's REV density
+ * (cudaBeamletRaytrace, dosecalc-beamlet/nvbbRayConvolve_device.cu:19-242), REV TERMA and REV cone
+ * dose (PackRowConvolve, :253-418), line fastest.  rev_size receives the REV grid dims even when
+ * cap (floats per buffer) is too small, in which case CCCS_EINVAL is returned. */
+int orc_beamlet_cone_stage(const cccs_problem* p, const cccs_beam* beam, const float* fmap, int ray,
+                           float* rev_dens, float* rev_terma, float* rev_dose, uint64_t cap,
+                           uint32_t* rev_size);
+
+/* Unpack check of the beamlet path (oracle instrumentation): each packed node is filled with
+ * coef[0] + (coef[1], coef[2], coef[3]) . X, X the PVCS point the raytrace assigns to the node, and
+ * UnpackBEVDosePillar (device.cu:459-521) resamples the grids into dense columns as
+ * orc_beamlet_dense does.  An unpack that inverts the raytrace's map returns the linear field at
+ * every voxel it writes. */
+int orc_beamlet_unpack_linear(const cccs_problem* p, const cccs_beam* beam, const float* fmap,
+                              const float* coef, float* out, int max_beamlets, int* nactive);
+
 /* A15: _sparsify (sparse_io.cpp:22-58) of one dense calc_bbox column. Returns nnz, writes at
  * most cap entries. */
 /* A10 alone: calcBeamletAnchors (dosecalc-beamlet/nvbbRayConvolve_host.cu:180-280) for fluence

@@ -5,7 +5,9 @@ checker -- never as the thing measured or shipped.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import fcntl
 import os
 import subprocess
 
@@ -27,23 +29,45 @@ class Stats(C.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
-_lib = None
+_libs: dict = {}
+_active = LIB_PATH
 
 
-def build() -> str:
-    r = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+def build(target: str = "all") -> str:
+    # one make at a time (parallel test workers may all ask for the mutants)
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    with open(os.path.join(HERE, "_build", ".lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        r = subprocess.run(["make", "-C", HERE, target], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
     return LIB_PATH
 
 
+def mutant_path(k: int) -> str:
+    """Oracle built with deliberate mistake k (ORC_MUTANT in cccs_oracle.cpp)."""
+    return os.path.join(HERE, "_build", f"libcccs_oracle_mut{k}.so")
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Route every wrapper in this module to the oracle library at `path` (a mutant) meanwhile."""
+    global _active
+    prev = _active
+    _active = path
+    try:
+        yield
+    finally:
+        _active = prev
+
+
 def load() -> C.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+    lib = _libs.get(_active)
+    if lib is not None:
+        return lib
+    if not os.path.exists(_active):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(_active)
     P, B, G = C.POINTER(abi.Problem), C.POINTER(abi.Beam), C.POINTER(abi.BeamGeometry)
     i32p = C.POINTER(C.c_int32)
     sig = {
@@ -75,7 +99,11 @@ def load() -> C.CDLL:
                                       C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint64), _fp, C.c_int64, C.POINTER(C.c_uint64)]
     lib.orc_reduce_by_roi.restype = C.c_int64
-    _lib = lib
+    lib.orc_beamlet_cone_stage.argtypes = [P, B, _fp, C.c_int, _fp, _fp, _fp, C.c_uint64, C.POINTER(C.c_uint32)]
+    lib.orc_beamlet_cone_stage.restype = C.c_int
+    lib.orc_beamlet_unpack_linear.argtypes = [P, B, _fp, _fp, _fp, C.c_int, C.POINTER(C.c_int)]
+    lib.orc_beamlet_unpack_linear.restype = C.c_int
+    _libs[_active] = lib
     return lib
 
 
@@ -276,3 +304,33 @@ def sparsify(dense: np.ndarray, size, crop_start, crop_size, thresh: float):
     n = load().orc_sparsify(fp(d), u3(*size), u3(*crop_start), u3(*crop_size), thresh,
                             keys.ctypes.data_as(C.POINTER(C.c_uint64)), fp(vals), cap, C.byref(pn), C.byref(pd))
     return keys[:n], vals[:n], pn.value, pd.value
+
+
+def beamlet_cone_stage(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, ray: int):
+    """REV density, TERMA and cone dose of batch 0 of the beamlet path for cone `ray`, each shaped
+    (nz, ny, nx) with the line (x) fastest."""
+    p = spec.struct()
+    f = np.ascontiguousarray(fmap, np.float32)
+    size = (C.c_uint32 * 3)()
+    rc = load().orc_beamlet_cone_stage(C.byref(p), C.byref(beam), fp(f), ray, None, None, None, 0, size)
+    if rc != abi.CCCS_EINVAL or size[0] == 0:
+        _chk(rc if rc != 0 else -1, "beamlet_cone_stage(size)")
+    n = int(size[0]) * int(size[1]) * int(size[2])
+    d, t, o = (np.zeros(n, np.float32) for _ in range(3))
+    _chk(load().orc_beamlet_cone_stage(C.byref(p), C.byref(beam), fp(f), ray, fp(d), fp(t), fp(o), n, size),
+         "beamlet_cone_stage")
+    shape = (int(size[2]), int(size[1]), int(size[0]))
+    return d.reshape(shape), t.reshape(shape), o.reshape(shape)
+
+
+def beamlet_unpack_linear(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, coef):
+    """Dense columns unpacked from packed grids that hold coef[0] + coef[1:] . X(node)."""
+    p = spec.struct()
+    f = np.ascontiguousarray(fmap, np.float32)
+    cap = max(1, int((f > 0).sum()))
+    out = np.zeros((cap,) + spec.bbox_shape, np.float32)
+    n = C.c_int(0)
+    c = np.ascontiguousarray(coef, np.float32)
+    _chk(load().orc_beamlet_unpack_linear(C.byref(p), C.byref(beam), fp(f), fp(c), fp(out), cap, C.byref(n)),
+         "beamlet_unpack_linear")
+    return out[:n.value]

@@ -104,15 +104,17 @@ def test_sparse_columns(case, mode):
     name, w, f, ref, eng = case
     thr = 1e-4
     dense = eng.beamlet_dense(w.beams[0], f, mode)
-    cols = eng.beamlet_dose([w.beams[0]], [f], threshold=thr, mode=mode)
+    cols = eng.beamlet_dose([w.beams[0]], [f], threshold=thr, mode=mode, percs=True)
     act = np.flatnonzero(f.ravel() > 0)
     assert [c[1] for c in cols] == act.tolist()
     assert all(c[0] == 0 for c in cols)
     size = w.spec.size
-    for (bi, bl, k, v), d, r in zip(cols, dense, ref):
-        ek, ev, _, _ = O.sparsify(d, size, w.spec.bbox_start, w.spec.bbox_size, thr)
+    for (bi, bl, k, v, pc), d, r in zip(cols, dense, ref):
+        ek, ev, pn, pd = O.sparsify(d, size, w.spec.bbox_start, w.spec.bbox_size, thr)
         assert np.array_equal(k, ek), f"{name}: beamlet {bl} keys differ from sparsify of the dense column"
         assert np.array_equal(v.view(np.uint32), ev.view(np.uint32))
+        # perc_nonzero / perc_dropped go into every coefficient file column: exact
+        assert (pc[0], pc[1], pc[2]) == (pn, pd, 0.0), f"{name}: beamlet {bl} percentages {pc} vs {(pn, pd)}"
         # against the oracle's column: every kept value is within tolerance of the oracle's dose
         rf = r.ravel()
         nx, ny = size[0], size[1]

@@ -41,22 +41,30 @@ def test_roi_reduced_columns_match_the_oracle(gantry, couch):
     nz, ny, nx = w.spec.density.shape
     print(f"[case] roi g{gantry} c{couch}", flush=True)
     with Engine(w.spec) as eng:
-        plain = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
+        plain = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL, percs=True)
         eng.set_roi_masks(rois)
-        red = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
+        red = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL, percs=True)
         with pytest.raises(abi.CCCSError):
             eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_PLAIN)
         eng.set_roi_masks([])
         again = eng.beamlet_dose(w.beams, [f], threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)
     assert len(plain) == len(red) > 10
     kept = 0
-    for (b0, l0, k0, v0), (b1, l1, k1, v1, blocks) in zip(plain, red):
+    ntotal = int(np.prod(w.spec.bbox_size))
+    for (b0, l0, k0, v0, p0), (b1, l1, k1, v1, blocks, p1) in zip(plain, red):
         assert (b0, l0) == (b1, l1)
         ok, ov, ob = O.reduce_by_roi(k0, v0, (nx, ny, nz), rois)
         assert np.array_equal(blocks, ob)
         assert np.array_equal(k1, ok)
         assert np.array_equal(v1.view(np.uint32), ov.view(np.uint32))
+        # percentages of _reduce_by_roi (sparse_io.cpp:150-155): the reduced share of the calc_bbox
+        # in double, perc_reducedrop = (float) unreduced perc_nonzero - that share, perc_dropped kept
+        pn = 100.0 * len(ok) / ntotal
+        assert p1[0] == float(np.float32(pn))
+        assert p1[1] == p0[1]
+        assert p1[2] == float(np.float32(np.float64(np.float32(p0[0])) - pn))
         kept += len(k1)
     assert kept > 100
     # turning the reduction off restores the plain columns
     assert all(np.array_equal(a[2], b[2]) for a, b in zip(plain, again))
+    assert all(len(b) == 4 for b in again)

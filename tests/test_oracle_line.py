@@ -11,20 +11,16 @@ import pytest
 from endtoend_amd import engine as E
 from oracle import oracle as O
 
+from line_walk import brute_walk, kcum, mean_radii
+
 T = E.load_tables()
 KERN, _, _ = E.cumulative_kernel(T, 8, 8)
 R = T.radii.astype(np.float64)
-MEAN_R = np.concatenate([[0.5 * R[0]], 0.5 * (R[1:] + R[:-1])])
+MEAN_R = mean_radii(R)
 
 
 def K(row, r, radii):
-    """Piecewise-linear cumulative kernel of one row at radius r (K(0) = 0)."""
-    k = KERN[row].astype(np.float64)
-    if r <= radii[0]:
-        return r / radii[0] * k[0]
-    j = int(np.searchsorted(radii, r))
-    j = min(j, len(radii) - 1)
-    return k[j - 1] + (r - radii[j - 1]) / (radii[j] - radii[j - 1]) * (k[j] - k[j - 1])
+    return kcum(KERN[row], r, radii)
 
 
 @pytest.mark.parametrize("P", [0, 1, 3])
@@ -41,41 +37,7 @@ def test_uniform_line_telescopes(P):
 
 
 def _brute(dens, terma, P, d, nphi, radii, packed):
-    """Reference line walk restated on radiological depths from float64 prefix sums."""
-    n = len(dens)
-    ntheta = KERN.shape[0]
-    rmax = radii[-1]
-    out = np.zeros(n)
-    for i in range(n):
-        if dens[i] < 0:
-            continue
-        total = 0.0
-        for row, step in ((P, -1), (ntheta - 1 - P, +1)):
-            last = 0.0
-            ks = range(i if packed else i + step, -1 if step < 0 else n, step)
-            acc = 0.0  # radiological distance from the destination's reference point to the far edge
-            for k in ks:
-                if dens[k] < 0:
-                    if packed:
-                        break
-                    continue
-                if packed:
-                    r = 0.5 * dens[i] * d if k == i else acc + 0.5 * dens[k] * d
-                else:
-                    r = acc + 0.5 * dens[k] * d
-                if r > rmax:
-                    break
-                c = K(row, r, radii)
-                total += (c - last) * terma[k]
-                last = c
-                if not (packed and k == i):
-                    acc = r + 0.5 * dens[k] * d
-                else:
-                    acc = 0.5 * dens[i] * d
-                if r > radii[-2] and np.searchsorted(radii, r) >= len(radii) - 1:
-                    break
-        out[i] = total / nphi
-    return out
+    return brute_walk(dens, terma, KERN, P, KERN.shape[0] - 1 - P, d, nphi, radii, packed)
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
