@@ -1,5 +1,6 @@
-"""Build the CCCS HIP library (endtoend_amd/lib/libcccs.so) for gfx950 with hipcc, and the
-host-only HDF5 file layer (endtoend_amd/lib/libcccs_h5.so) with g++ where HDF5 is installed.
+"""Build the CCCS HIP library (endtoend_amd/lib/libcccs.so) for gfx950 with hipcc, the host-only
+HDF5 file layer (endtoend_amd/lib/libcccs_h5.so) with g++ where HDF5 is installed, and the
+reference-named drop-in shims over them (endtoend_amd/lib/libcccs_shim_{beam,beamlet}.so).
 
 Every source is compiled with -ffp-contract=off (the Siddon segment lists must match the host
 oracle bit for bit) and with packed FP32 disabled.  Objects go to endtoend_amd/_build/, the
@@ -95,6 +96,39 @@ def build_h5(force: bool = True, verbose: bool = False) -> str | None:
     return H5_LIB
 
 
+SHIM_DIR = os.path.join(HERE, "shim")
+SHIM_LIBS = {"beam": os.path.join(LIBDIR, "libcccs_shim_beam.so"),
+             "beamlet": os.path.join(LIBDIR, "libcccs_shim_beamlet.so")}
+
+
+def build_shim(force: bool = True, verbose: bool = False) -> dict:
+    """The reference-named drop-in (endtoend_amd/shim): libcccs_shim_beam.so (dosecalc-beam's entry
+    points) over libcccs.so, and libcccs_shim_beamlet.so (dosecalc-beamlet's, with its
+    SparsifyManager) over libcccs.so and libcccs_h5.so.  Host C++ only (g++); the beamlet shim is
+    skipped when the HDF5 layer is not built."""
+    src = os.path.join(SHIM_DIR, "reference_shim.cpp")
+    deps = [src, os.path.join(SHIM_DIR, "reference_shim.hpp"), os.path.join(SHIM_DIR, "reference_types.hpp"),
+            os.path.join(ROOT, "include", "cccs.h"), os.path.join(ROOT, "include", "cccs_h5.h"), LIB]
+    built = {}
+    for kind, out in SHIM_LIBS.items():
+        if kind == "beamlet" and not os.path.exists(H5_LIB):
+            continue
+        if force or _stale(out, deps):
+            tmp = out + ".tmp"
+            cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC", "-I", os.path.join(ROOT, "include"),
+                   "-I", SHIM_DIR] + (["-DCCCS_SHIM_BEAMLET"] if kind == "beamlet" else []) + \
+                  [src, "-L", LIBDIR, "-lcccs"] + (["-lcccs_h5"] if kind == "beamlet" else []) + \
+                  ["-Wl,-rpath,$ORIGIN", "-Wl,--allow-shlib-undefined", "-o", tmp]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"g++ failed for the {kind} shim:\n{r.stdout}\n{r.stderr}")
+            os.replace(tmp, out)
+            if verbose:
+                print(f"built {out}")
+        built[kind] = out
+    return built
+
+
 def build(force: bool = True, verbose: bool = False) -> str:
     """Compile every source and link libcccs.so (and libcccs_h5.so); returns the HIP library path."""
     os.makedirs(OBJ, exist_ok=True)
@@ -111,6 +145,7 @@ def build(force: bool = True, verbose: bool = False) -> str:
     if verbose:
         print(f"built {LIB}")
     build_h5(force=force, verbose=verbose)
+    build_shim(force=force, verbose=verbose)
     return LIB
 
 
