@@ -163,6 +163,7 @@ EXPORTS = [
                                              C.c_void_p, C.c_void_p]),
     ("cccs_beam_dose", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.POINTER(_fp), _fp]),
     ("cccs_beam_dose_device", C.c_int, [_P, C.POINTER(Beam), C.c_int, _P, _P]),
+    ("cccs_beam_dose_device_noalloc", C.c_int, [_P, C.POINTER(Beam), C.c_int, _P, _P]),
     ("cccs_beam_terma", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp]),
     ("cccs_beam_geometry_of", C.c_int, [_P, C.POINTER(Beam), _fp, C.POINTER(BeamGeometry)]),
     ("cccs_siddon_paths", C.c_int, [_P, _fp, _fp, C.c_int, C.c_int, C.POINTER(C.c_int32), _fp, _fp,

@@ -958,10 +958,53 @@ int beam_geometry(cccs_ctx* ctx, const cccs_beam& b, const float* red, BeamGeom&
     return CCCS_OK;
 }
 
-int run_crop(cccs_ctx* ctx, const float* d_terma, const cccs_beam& b, float* red_host) {
-    CCCS_TRY(ctx->partial.ensure(sizeof(float) * (kCropBlocks * 8 + 8)));
+// Device and host scratch of one full-beam dose computation: the context's grow-only buffers, or,
+// for the no-allocation entry point, consecutive 256-byte aligned pieces of the library's fixed
+// device array with host staging on the caller's stack.
+struct BeamScratch {
+    cccs_ctx* ctx = nullptr;
+    char* arena = nullptr;          // fixed device array, or nullptr: the context's buffers
+    size_t cap = 0, used = 0;
+    float* host_small = nullptr;    // >= 64 bytes of host staging
+    ConeDev* host_cones = nullptr;  // kMaxRays entries of host staging
+    int get(DevBuf& buf, size_t bytes, void** out) {
+        if (!arena) {
+            CCCS_TRY(buf.ensure(bytes));
+            *out = buf.p;
+            return CCCS_OK;
+        }
+        const size_t at = (used + 255) & ~size_t(255);
+        if (bytes > cap || at > cap - bytes)
+            return set_error(CCCS_ENOMEM, "beam needs more than the " + std::to_string(cap >> 20) +
+                                              " MiB fixed scratch of the no-allocation entry point");
+        used = at + bytes;
+        *out = arena + at;
+        return CCCS_OK;
+    }
+    template <class T>
+    int get(DevBuf& buf, size_t count, T** out) {
+        void* v = nullptr;
+        CCCS_TRY(get(buf, sizeof(T) * std::max<size_t>(1, count), &v));
+        *out = static_cast<T*>(v);
+        return CCCS_OK;
+    }
+};
+
+// Scratch from the context's own (allocating) buffers.
+int context_scratch(cccs_ctx* ctx, BeamScratch& S) {
+    S = BeamScratch();
+    S.ctx = ctx;
     CCCS_TRY(ctx->h_small.ensure(64));
-    float* part = ctx->partial.as<float>();
+    CCCS_TRY(ctx->h_cones.ensure(sizeof(ConeDev) * kMaxRays));
+    S.host_small = ctx->h_small.as<float>();
+    S.host_cones = ctx->h_cones.as<ConeDev>();
+    return CCCS_OK;
+}
+
+int run_crop(BeamScratch& S, const float* d_terma, const cccs_beam& b, float* red_host) {
+    cccs_ctx* ctx = S.ctx;
+    float* part = nullptr;
+    CCCS_TRY(S.get(ctx->partial, kCropBlocks * 8 + 8, &part));
     Rot2 rot = rot2_beam_fwd(b.azimuth, b.zenith, b.coll);
     const cccs_problem& p = ctx->prob;
     size_t n = bbox_voxels(ctx);
@@ -972,16 +1015,17 @@ int run_crop(cccs_ctx* ctx, const float* d_terma, const cccs_beam& b, float* red
     hipLaunchKernelGGL(crop_final_kernel, dim3(1), dim3(256), 0, ctx->stream, part, blocks, part + kCropBlocks * 8);
     prof_end(ctx, tok, 4.0 * (double)n, 0.0, n);
     CCCS_HIP(hipGetLastError());
-    CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, part + kCropBlocks * 8, 7 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    CCCS_HIP(hipMemcpyAsync(S.host_small, part + kCropBlocks * 8, 7 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
-    std::memcpy(red_host, ctx->h_small.p, 7 * sizeof(float));
+    std::memcpy(red_host, S.host_small, 7 * sizeof(float));
     return CCCS_OK;
 }
 
 // Everything after TERMA: geometry, cones, PVCS accumulation into d_dose.
-int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_dose, BeamGeom* gout) {
+int run_cones(BeamScratch& S, const cccs_beam& b, const float* d_terma, float* d_dose, BeamGeom* gout) {
+    cccs_ctx* ctx = S.ctx;
     float red[7];
-    CCCS_TRY(run_crop(ctx, d_terma, b, red));
+    CCCS_TRY(run_crop(S, d_terma, b, red));
     BeamGeom g;
     CCCS_TRY(beam_geometry(ctx, b, red, g));
     if (gout) *gout = g;
@@ -989,11 +1033,11 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     const cccs_problem& p = ctx->prob;
     if (g.total_lines > 0x7fffffffULL || g.total_rev > 0xffffffffULL)
         return set_error(CCCS_EGEOM, "REV grids exceed 2^32 samples");
-    CCCS_TRY(ctx->cones.ensure(sizeof(ConeDev) * kMaxRays));
-    CCCS_TRY(ctx->h_cones.ensure(sizeof(ConeDev) * kMaxRays));
-    CCCS_TRY(ctx->lines.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, g.total_lines)));
-    CCCS_TRY(ctx->h_small.ensure(64));
-    ConeDev* hc = ctx->h_cones.as<ConeDev>();
+    ConeDev* d_cones = nullptr;
+    LineMeta* d_lines = nullptr;
+    CCCS_TRY(S.get(ctx->cones, kMaxRays, &d_cones));
+    CCCS_TRY(S.get(ctx->lines, g.total_lines, &d_lines));
+    ConeDev* hc = S.host_cones;
     dm33 bfwd = dm_beam_fwd(b.azimuth, b.zenith, b.coll);
     dm33 binv = dm_beam_inv(b.azimuth, b.zenith, b.coll);
     v3 rv = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
@@ -1023,15 +1067,18 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
         line0 += (uint64_t)c.ny * c.nz;
         roff += (uint64_t)c.nx * c.ny * c.nz;
     }
-    CCCS_HIP(hipMemcpyAsync(ctx->cones.p, hc, sizeof(ConeDev) * g.nrays, hipMemcpyHostToDevice, ctx->stream));
+    CCCS_HIP(hipMemcpyAsync(d_cones, hc, sizeof(ConeDev) * g.nrays, hipMemcpyHostToDevice, ctx->stream));
+    // the host staging may be the caller's stack (no-allocation entry point): the copy completes first
+    if (S.arena) CCCS_HIP(hipStreamSynchronize(ctx->stream));
     const size_t nvox = vol_voxels(ctx);
     if (nvox >= (size_t(1) << 31)) return set_error(CCCS_EGEOM, "CT grid beyond 2^31 voxels");
-    CCCS_TRY(ctx->dt.ensure(sizeof(float2) * nvox));
+    float2* d_dt = nullptr;
+    CCCS_TRY(S.get(ctx->dt, nvox, &d_dt));
     ConeArgs ca;
-    ca.dt = ctx->dt.as<float2>();
+    ca.dt = d_dt;
     ca.tables = ctx->d_tables;
-    ca.cones = ctx->cones.as<ConeDev>();
-    ca.lines = ctx->lines.as<LineMeta>();
+    ca.cones = d_cones;
+    ca.lines = d_lines;
     ca.rev = nullptr;
     ca.nlines = g.total_lines;
     ca.ncones = g.nrays;
@@ -1052,22 +1099,23 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
         dim3 blk(64, 4, 1), grd((unsigned)((p.size[0] + 63) / 64), (unsigned)((p.size[1] + 3) / 4), p.size[2]);
         hipLaunchKernelGGL(pack_dt_kernel, grd, blk, 0, ctx->stream, (const float*)ctx->d_dens, d_terma,
                            (int)p.size[0], (int)p.size[1], (int)p.size[2], (int)p.bbox_start[0],
-                           (int)p.bbox_start[1], (int)p.bbox_start[2], ca.bbx, ca.bby, ca.bbz, ctx->dt.as<float2>());
+                           (int)p.bbox_start[1], (int)p.bbox_start[2], ca.bbx, ca.bby, ca.bbz, d_dt);
         CCCS_HIP(hipGetLastError());
     }
     // pass 1: stored interval of every line, then offsets into the compact store
     hipLaunchKernelGGL(line_interval_kernel, dim3((unsigned)((g.total_lines + 255) / 256)), dim3(256), 0, ctx->stream,
                        ca);
     CCCS_HIP(hipGetLastError());
-    CCCS_TRY(launch_line_scan(ctx, ca.lines, g.total_lines));
-    CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, ca.lines + (g.total_lines - 1), sizeof(LineMeta), hipMemcpyDeviceToHost,
+    uint32_t* d_scan = nullptr;
+    CCCS_TRY(S.get(ctx->scan, line_scan_words(g.total_lines), &d_scan));
+    CCCS_TRY(launch_line_scan(ctx, ca.lines, g.total_lines, d_scan));
+    CCCS_HIP(hipMemcpyAsync(S.host_small, ca.lines + (g.total_lines - 1), sizeof(LineMeta), hipMemcpyDeviceToHost,
                             ctx->stream));
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
     LineMeta last;
-    std::memcpy(&last, ctx->h_small.p, sizeof(LineMeta));
+    std::memcpy(&last, S.host_small, sizeof(LineMeta));
     const uint64_t stored = (uint64_t)last.off + (uint64_t)last.len;
-    CCCS_TRY(ctx->rev.ensure(sizeof(float) * std::max<uint64_t>(1, stored)));
-    ca.rev = ctx->rev.as<float>();
+    CCCS_TRY(S.get(ctx->rev, stored, &ca.rev));
     // pass 2: per-line resample + walks into the compact store
     ca.counts = prof_counter(ctx, g.total_lines);
     uint32_t maxn = 1;
@@ -1096,7 +1144,7 @@ int run_cones(cccs_ctx* ctx, const cccs_beam& b, const float* d_terma, float* d_
     // algorithmic traffic: density + TERMA in, cone dose out, per stored sample
     prof_end(ctx, tok, 12.0 * (double)stored, 0.0, stored);
     PvcsArgs pa;
-    pa.cones = ctx->cones.as<ConeDev>();
+    pa.cones = d_cones;
     pa.lines = ca.lines;
     pa.rev = ca.rev;
     pa.dose = d_dose;
@@ -1139,13 +1187,47 @@ int upload_fmap(cccs_ctx* ctx, const cccs_beam& b, const float* host_fmap, const
 
 }  // namespace
 
+int beam_dose_scratch(BeamScratch& S, const cccs_beam& b, const float* d_fmap, float* d_dose) {
+    cccs_ctx* ctx = S.ctx;
+    CCCS_TRY(check_beam(ctx, b));
+    float* d_terma = nullptr;
+    CCCS_TRY(S.get(ctx->terma, bbox_voxels(ctx), &d_terma));
+    TermaArgs ta = terma_args(ctx, b, d_fmap, d_terma, -1);
+    CCCS_TRY(launch_terma(ctx, ta));
+    return run_cones(S, b, d_terma, d_dose, nullptr);
+}
+
 // Shared by the C-ABI entry points below and by the beamlet path.
 int beam_dose_one(cccs_ctx* ctx, const cccs_beam& b, const float* d_fmap, float* d_dose) {
-    CCCS_TRY(check_beam(ctx, b));
-    CCCS_TRY(ctx->terma.ensure(sizeof(float) * bbox_voxels(ctx)));
-    TermaArgs ta = terma_args(ctx, b, d_fmap, ctx->terma.as<float>(), -1);
-    CCCS_TRY(launch_terma(ctx, ta));
-    return run_cones(ctx, b, ctx->terma.as<float>(), d_dose, nullptr);
+    BeamScratch S;
+    CCCS_TRY(context_scratch(ctx, S));
+    return beam_dose_scratch(S, b, d_fmap, d_dose);
+}
+
+// ---- fixed scratch of the no-allocation entry point ----------------------------------------------
+// The full-beam path's device scratch (TERMA, interleaved density/TERMA, line table, scan partials,
+// cone constants, compact REV store) for callers that cannot allocate: one array per GPU, held
+// until the process ends.  Host code uses the per-device address from hipGetSymbolAddress, never
+// the array's name.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return set_error(CCCS_EINVAL, "device ordinal beyond 64");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CCCS_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_error(CCCS_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return CCCS_OK;
 }
 
 }  // namespace cccs
@@ -1154,6 +1236,35 @@ using namespace cccs;
 
 // =================================================================================================
 // C-ABI
+
+extern "C" int cccs_beam_dose_device_noalloc(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
+                                             const float* d_fluence, float* d_dose) {
+    clear_error();
+    if (!ctx || (!beams && nbeams > 0) || nbeams < 0 || !d_dose || (!d_fluence && nbeams > 0))
+        return set_error(CCCS_EINVAL, "null argument");
+    if (ctx->prof.on) return set_error(CCCS_EINVAL, "per-kernel profiling allocates: turn it off for this call");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) return set_error(CCCS_EDEVICE, "fixed scratch not resolved for this device");
+    float host_small[16];
+    ConeDev host_cones[kMaxRays];
+    size_t fo = 0;
+    for (int i = 0; i < nbeams; i++) {
+        BeamScratch S;
+        S.ctx = ctx;
+        S.arena = static_cast<char*>(base);
+        S.cap = kNoAllocScratchBytes;
+        S.host_small = host_small;
+        S.host_cones = host_cones;
+        CCCS_TRY(beam_dose_scratch(S, beams[i], d_fluence + fo, d_dose));
+        fo += (size_t)beams[i].fmap_size[0] * beams[i].fmap_size[1];
+    }
+    return CCCS_OK;
+}
 
 extern "C" int cccs_beam_dose_device(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, const float* d_fluence,
                                      float* d_dose) {
@@ -1217,7 +1328,9 @@ extern "C" int cccs_beam_geometry_of(cccs_ctx* ctx, const cccs_beam* beam, const
     TermaArgs ta = terma_args(ctx, *beam, dfm, ctx->terma.as<float>(), -1);
     CCCS_TRY(launch_terma(ctx, ta));
     BeamGeom g;
-    CCCS_TRY(run_cones(ctx, *beam, ctx->terma.as<float>(), nullptr, &g));
+    BeamScratch S;
+    CCCS_TRY(context_scratch(ctx, S));
+    CCCS_TRY(run_cones(S, *beam, ctx->terma.as<float>(), nullptr, &g));
     std::memset(geom, 0, sizeof(*geom));
     geom->lim_min[0] = g.lim_min.x; geom->lim_min[1] = g.lim_min.y; geom->lim_min[2] = g.lim_min.z;
     for (int k = 0; k < 3; k++) geom->bev_size[k] = g.bev_size[k];

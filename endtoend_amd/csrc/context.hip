@@ -185,12 +185,18 @@ __global__ void __launch_bounds__(kScanBlock) line_scan_add(LineMeta* __restrict
     if (i < n) meta[i].off += bsum[blockIdx.x];
 }
 
+size_t line_scan_words(uint64_t n) { return (size_t)((n + kScanBlock - 1) / kScanBlock); }
+
 int launch_line_scan(cccs_ctx* c, LineMeta* meta, uint64_t n) {
     if (n == 0) return CCCS_OK;
-    const uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    CCCS_TRY(c->scan.ensure(sizeof(uint32_t) * line_scan_words(n)));
+    return launch_line_scan(c, meta, n, c->scan.as<uint32_t>());
+}
+
+int launch_line_scan(cccs_ctx* c, LineMeta* meta, uint64_t n, uint32_t* bsum) {
+    if (n == 0) return CCCS_OK;
+    const uint64_t nb = line_scan_words(n);
     if (nb > 0xffffffffULL) return set_error(CCCS_EGEOM, "too many REV lines");
-    CCCS_TRY(c->scan.ensure(sizeof(uint32_t) * nb));
-    uint32_t* bsum = c->scan.as<uint32_t>();
     hipLaunchKernelGGL(line_scan_local, dim3((unsigned)nb), dim3(kScanBlock), 0, c->stream, meta, n, bsum);
     hipLaunchKernelGGL(line_scan_blocks, dim3(1), dim3(kScanBlock), 0, c->stream, bsum, (uint32_t)nb);
     hipLaunchKernelGGL(line_scan_add, dim3((unsigned)nb), dim3(kScanBlock), 0, c->stream, meta, n, bsum);
@@ -293,6 +299,8 @@ extern "C" int cccs_ctx_create(const cccs_problem* p, int device, cccs_ctx** out
         if (hipMalloc((void**)&c->d_tables, tab.size() * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "tables"); break; }
         if (hipMemcpy(c->d_tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "tables copy"); break; }
         c->resident = nv * sizeof(float) + tab.size() * sizeof(float);
+        // the no-allocation entry point's fixed scratch: its address on this device, resolved once
+        if ((st = noalloc_scratch_init(device)) != CCCS_OK) break;
     } while (false);
     if (st != CCCS_OK) {
         cccs_ctx_destroy(c);

@@ -177,7 +177,13 @@ void prof_end(cccs_ctx* c, int token, double bytes, double flops, uint64_t items
 // context stream first.
 int download_staged(cccs_ctx* c, void* dst, const void* src, size_t bytes, bool add_f32);
 
-// Exclusive scan of LineMeta::len into LineMeta::off over n lines (on the context stream).
+// Exclusive scan of LineMeta::len into LineMeta::off over n lines (on the context stream), with
+// the block partials in the context's scan buffer, or in `bsum` (line_scan_words(n) words).
 int launch_line_scan(cccs_ctx* c, LineMeta* meta, uint64_t n);
+int launch_line_scan(cccs_ctx* c, LineMeta* meta, uint64_t n, uint32_t* bsum);
+size_t line_scan_words(uint64_t n);
 int check_beam(const cccs_ctx* ctx, const cccs_beam& b);
+// Resolves, once per device and with that device current, the address of the fixed scratch array
+// used by cccs_beam_dose_device_noalloc.
+int noalloc_scratch_init(int device);
 }  // namespace cccs

@@ -189,6 +189,17 @@ int cccs_beam_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
 int cccs_beam_dose_device(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
                           const float* d_fluence, float* d_dose);
 
+/* No-allocation variant of cccs_beam_dose_device (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and result; it calls no hipMalloc/hipFree/hipHostMalloc.  Its
+ * device scratch is a fixed 256 MiB array per GPU, held by the library until the process ends; a
+ * beam whose scratch (TERMA, density/TERMA pairs, line table, compact REV store) exceeds it fails
+ * with CCCS_ENOMEM before any dose of that beam is added.  Calls on one GPU share that array, so
+ * calls that overlap on one GPU (any contexts, any threads) race; per-kernel profiling must be off
+ * (CCCS_EINVAL otherwise).  The context itself comes from cccs_ctx_create beforehand. */
+int cccs_beam_dose_device_noalloc(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
+                                  const float* d_fluence, float* d_dose);
+
 /* Stage outputs of the beam path, for stage-wise parity checks. */
 int cccs_beam_terma(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int beamletnum,
                     float* terma_bbox);
