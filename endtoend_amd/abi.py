@@ -22,7 +22,6 @@ CCCS_EDEVICE = -5
 CCCS_EBUSY = -6
 
 CCCS_BEAMLET_FAITHFUL = 0
-CCCS_BEAMLET_FAST = 1
 CCCS_BEAMLET_PLAIN = 2
 
 _fp = C.POINTER(C.c_float)

@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <thread>
 #include <vector>
@@ -60,6 +61,7 @@ struct BeamletDev {
     int box0[3];     // calc_bbox-relative voxel box that can receive its dose
     int boxn[3];
     uint64_t arena;  // first element of its box in the column arena (i outer, k inner)
+    uint32_t row0;   // first of its box's (i, j) rows in the job's row counts
     m33 fwd_m;       // PVCS -> beamlet frame (beamlet forward * beam forward), matrix form
 };
 
@@ -67,7 +69,6 @@ struct BatchDev {
     BeamDev beam;
     Rot2 beam_inv;         // BEV -> PVCS
     const float* fmap;     // fluence map of the beam (device)
-    const float* tgrid;    // unit-TERMA grid of the beam (fast mode)
     int n, npx, npy;       // beamlets, pillars per row, rows
     int gd[3];             // packed grid dims (pX, pY, pZ)
     int pd[3];             // pillar dims
@@ -117,13 +118,12 @@ struct BlArgs {
     VolDesc vd;
     SpecDev spec;
     float bhc;
-    int fast;                   // unit-TERMA grid instead of a Siddon march per sample
-    int tg[3];                  // unit-TERMA grid dims (CT dims + 2)
+    uint32_t* line_cone;        // grid (= batch) of every line, written by the interval pass
     uint32_t* counts;
     const uint32_t* tile_batch; // batch of every gather tile (nullptr: a single batch)
 };
 
-__device__ __forceinline__ int find_cone(const BlConeDev* __restrict__ c, int n, uint64_t line) {
+CCCS_HD int find_cone(const BlConeDev* __restrict__ c, int n, uint64_t line) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -134,18 +134,14 @@ __device__ __forceinline__ int find_cone(const BlConeDev* __restrict__ c, int n,
 
 // REV sample (tX, tY, tZ) of a grid -> packed-BEV coordinate (cudaBeamletRaytrace,
 // nvbbRayConvolve_device.cu:33-41).
-__device__ __forceinline__ v3 rev_to_packed(const BlConeDev& c, v3 rrv, int tX, int tY, int tZ) {
+CCCS_HD v3 rev_to_packed(const BlConeDev& c, v3 rrv, int tX, int tY, int tZ) {
     v3 q = mk3(fmaf(c.minc.x, rrv.x, (float)tZ), fmaf(c.minc.y, rrv.y, (float)tX), fmaf(c.minc.z, rrv.z, (float)tY));
     return rot2(c.kinv, q) + 0.5f;
 }
 
 // Interval of stored samples of every line of one cone direction: the line clipped to the packed
 // grid box (packed coordinates are affine in the sample index), widened by 2/3 samples.
-__global__ void __launch_bounds__(256) bl_interval_kernel(BlArgs a) {
-    const uint64_t line = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (line >= a.nlines) return;
-    const BlConeDev& c = a.cones[find_cone(a.cones, a.nbatch, line)];
-    const BatchDev& B = a.batches[c.batch];
+CCCS_HD LineMeta line_interval(const BlArgs& a, const BlConeDev& c, const BatchDev& B, uint64_t line) {
     const uint64_t l = line - c.line0;
     const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
     v3 q0 = mk3(fmaf(c.minc.x, a.rrv.x, (float)tZ), c.minc.y * a.rrv.y, fmaf(c.minc.z, a.rrv.z, (float)tY));
@@ -165,31 +161,20 @@ __global__ void __launch_bounds__(256) bl_interval_kernel(BlArgs a) {
         m.i0 = (int16_t)i0;
         m.len = (int16_t)max(0, i1 - i0);
     }
-    a.lines[line] = m;
+    return m;
 }
 
-// Unit TERMA from the beam's grid (CT voxel centres with a one-voxel rim), trilinear with clamping.
-__device__ __forceinline__ float tgrid_fetch(const float* __restrict__ g, const int* tg, v3 gi) {
-    float x = fminf(fmaxf(gi.x + 1.f, 0.f), (float)(tg[0] - 1));
-    float y = fminf(fmaxf(gi.y + 1.f, 0.f), (float)(tg[1] - 1));
-    float z = fminf(fmaxf(gi.z + 1.f, 0.f), (float)(tg[2] - 1));
-    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-    const int i = (int)fx, j = (int)fy, k = (int)fz;
-    const int i1 = min(i + 1, tg[0] - 1), j1 = min(j + 1, tg[1] - 1), k1 = min(k + 1, tg[2] - 1);
-    const float u = x - fx, v = y - fy, w = z - fz;
-    const size_t sx = 1, sy = (size_t)tg[0], sz = (size_t)tg[0] * tg[1];
-    auto at = [&](int ii, int jj, int kk) { return g[ii * sx + jj * sy + kk * sz]; };
-    const float c00 = at(i, j, k) + u * (at(i1, j, k) - at(i, j, k));
-    const float c10 = at(i, j1, k) + u * (at(i1, j1, k) - at(i, j1, k));
-    const float c01 = at(i, j, k1) + u * (at(i1, j, k1) - at(i, j, k1));
-    const float c11 = at(i, j1, k1) + u * (at(i1, j1, k1) - at(i, j1, k1));
-    const float c0 = c00 + v * (c10 - c00), c1 = c01 + v * (c11 - c01);
-    return c0 + w * (c1 - c0);
+__global__ void __launch_bounds__(256) bl_interval_kernel(BlArgs a) {
+    const uint64_t line = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (line >= a.nlines) return;
+    const int ci = find_cone(a.cones, a.nbatch, line);
+    a.line_cone[line] = (uint32_t)ci;
+    a.lines[line] = line_interval(a, a.cones[ci], a.batches[ci], line);
 }
 
 // fmodf(x, y) for 0 <= x < 2^24 and a positive integer y <= 2^20: n*y is exact, x - n*y is exact,
 // and the fix-ups absorb a quotient rounded across an integer.  Equal to fmodf bit for bit there.
-__device__ __forceinline__ float fmod_pos(float x, float y) {
+CCCS_HD float fmod_pos(float x, float y) {
     const float n = truncf(x / y);
     float r = fmaf(-n, y, x);
     if (r < 0.f) r += y;
@@ -200,8 +185,8 @@ __device__ __forceinline__ float fmod_pos(float x, float y) {
 // Pillar mapping of one REV sample (cudaBeamletRaytrace, nvbbRayConvolve_device.cu:33-68):
 // false for a wall, a sample outside the packed grid or past the last pillar; otherwise the
 // beamlet (pidx, batch-local) and the pillar-local coordinate inp (already shifted by -0.5).
-__device__ __forceinline__ bool bl_locate(const BlArgs& a, const BlConeDev& c, const BatchDev& B, int tX, int tY,
-                                          int tZ, int& pidx, v3& inp) {
+CCCS_HD bool bl_locate(const BlArgs& a, const BlConeDev& c, const BatchDev& B, int tX, int tY, int tZ, int& pidx,
+                       v3& inp) {
     const v3 pb = rev_to_packed(c, a.rrv, tX, tY, tZ);
     const v3 gdf = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]);
     if (pb.x < 0.f || pb.y < 0.f || pb.z < 0.f || pb.x >= gdf.x || pb.y >= gdf.y || pb.z >= gdf.z) return false;
@@ -215,14 +200,28 @@ __device__ __forceinline__ bool bl_locate(const BlArgs& a, const BlConeDev& c, c
     return true;
 }
 
+// What a REV sample's density and TERMA points need of its beamlet (the head of BeamletDev).
+struct PillarRec {
+    v3 pst;
+    Rot1 inv;
+    int fidx;
+};
+CCCS_HD PillarRec pillar_rec(const BeamletDev& d) {
+    PillarRec r;
+    r.pst = d.pst;
+    r.inv = d.inv;
+    r.fidx = d.fidx;
+    return r;
+}
+
 // PVCS point [cm] where the sample's TERMA is evaluated (device.cu:170-180).
-__device__ __forceinline__ v3 bl_terma_point(const BlArgs& a, const BatchDev& B, const BeamletDev& bl, v3 inp) {
+CCCS_HD v3 bl_terma_point(const BlArgs& a, const BatchDev& B, const PillarRec& bl, v3 inp) {
     const v3 g = rot2(B.beam_inv, rot1(bl.inv, inp));
     return mk3(fmaf(g.x, a.rvs.x, bl.pst.x), fmaf(g.y, a.rvs.y, bl.pst.y), fmaf(g.z, a.rvs.z, bl.pst.z));
 }
 
 // Density at the sample (device.cu:213-223): trilinear, border 0.
-__device__ __forceinline__ float bl_density(const BlArgs& a, const BatchDev& B, const BeamletDev& bl, v3 inp) {
+__device__ __forceinline__ float bl_density(const BlArgs& a, const BatchDev& B, const PillarRec& bl, v3 inp) {
     const v3 g2 = rot2(B.beam_inv, rot1(bl.inv, inp * a.rvs)) + bl.pst;
     const v3 gi = (g2 - a.vd.start) / a.vd.voxel;
     return tex3_border(a.dens, a.vd.nx, a.vd.ny, a.vd.nz, gi.x + 0.5f, gi.y + 0.5f, gi.z + 0.5f);
@@ -237,38 +236,43 @@ __device__ __forceinline__ float kc_at(float r, const float4* __restrict__ tab, 
     return iv.y + (r - iv.x) * iv.z;
 }
 
-
-// FP32 operations counted per sample by the sample stage: the affine pre-cull, the exact pillar
-// mapping (two Rodrigues rotations, two divisions, three fmods, bounds), the fluence-cell test of
-// a TERMA point (three rotations, projection, two divisions), and the trilinear density with its
-// point (three rotations, 8-tap interpolation).
-constexpr uint32_t kOpsCull = 12, kOpsLocate = 70, kOpsFluence = 150, kOpsDensity = 130;
+constexpr int kStagePillars = 64;  // footprint pillars of one line whose beamlet data sits in LDS
 
 // Stage 1 of a cone direction, one wave per REV line: pillar mapping of every stored sample and
 // the beamlet-footprint test (fluence cell of the sample == its beamlet).  A line without footprint
 // samples carries no TERMA, so all its cone dose is zero: it is marked empty.  Otherwise the half
 // steps (0.5 * delr * density, -1 for walls) go to H, TERMA slots are zeroed, and the footprint
-// samples are queued for their Siddon TERMA (fast mode evaluates TERMA here instead).
+// samples are queued for their Siddon TERMA.
+//
+// The per-sample work is a chain of dependent memory trips (line record, grid tables, beamlet
+// record, density taps), so the kernel keeps each chain short: the line's grid comes from the line
+// table built by the interval pass (no search), its grid and batch tables load together, and the
+// beamlet data of the line's footprint pillars is staged in LDS once, while the pillar
+// classification runs, instead of being read per sample by the density pass.
 __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restrict__ H, TermaReq* __restrict__ q,
                                                        uint32_t subcap, uint32_t* __restrict__ qcnt) {
     __shared__ uint8_t s_fp[kMaxLine];            // bit 0: footprint sample, bit 1: on the exact list
     __shared__ short s_list[kMaxLine];            // compacted work list of the current pass
     __shared__ uint32_t s_pil[kPillarBits / 32];  // pillars with footprint samples on this line
+    __shared__ uint16_t s_rank[kPillarBits / 32]; // footprint pillars in the words before each word
+    __shared__ PillarRec s_rec[kStagePillars];    // their beamlet data, in pillar order
     const int tid = threadIdx.x;
     const uint64_t lt = (1ull << tid) - 1ull;
-    uint32_t ops = 0;
+    uint32_t stored = 0;
     // each wave walks a stride of lines; waves append to one of kSubQueues queues (fewer
     // collisions on the counters)
     const uint32_t sq = blockIdx.x % kSubQueues;
     TermaReq* qs = q + (size_t)sq * subcap;
     for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
     const LineMeta meta = a.lines[line];
+    const uint32_t ci = a.line_cone[line];
     if (meta.len <= 0) continue;
-    const BlConeDev& c = a.cones[find_cone(a.cones, a.nbatch, line)];
-    const BatchDev& B = a.batches[c.batch];
+    const BlConeDev& c = a.cones[ci];
+    const BatchDev& B = a.batches[ci];  // grid ci of a direction belongs to batch ci
     const uint64_t l = line - c.line0;
     const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
     const int i0 = meta.i0, m = meta.len;
+    if (tid == 0) stored += (uint32_t)m;  // summed over the wave's lanes by wave_count
     // affine form of the packed coordinate along the line (matrix rotation): used only to skip
     // samples that are far from every footprint or inside pillars without footprint samples;
     // the exact mapping decides every other sample
@@ -295,7 +299,6 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         if (cand) s_list[ncand + __popcll(mask & lt)] = (short)j;
         ncand += __popcll(mask);
     }
-    ops += (uint32_t)m * kOpsCull / 64u + 1u;
     if (ncand == 0) {
         if (tid == 0) a.lines[line].len = 0;
         __syncthreads();
@@ -311,12 +314,10 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             const int j = s_list[r + tid];
             int pidx;
             v3 inp;
-            ops += kOpsLocate;
             if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
                 fabsf(inp.z - B.fc[1]) <= B.fw) {
-                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+                const PillarRec bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
                 fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
-                ops += kOpsFluence;
                 if (fp) {
                     s_fp[j] = 1;
                     if (track) atomicOr(&s_pil[pidx >> 5], 1u << (pidx & 31));
@@ -335,8 +336,34 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     // one reservation per line in the sub-queue (its latency overlaps pass B.1): every footprint
     // sample, and no other, queues one TERMA request in pass B.2
     uint32_t qres = 0;
-    if (tid == 0 && !a.fast) qres = atomicAdd(qcnt + sq * kCntStride, nfoot);
+    if (tid == 0) qres = atomicAdd(qcnt + sq * kCntStride, nfoot);
     __syncthreads();
+    // the beamlet data of the footprint pillars into LDS (its loads overlap pass B.1): word w of
+    // s_pil is lane w's (w < 64) or lane w-64's; a pillar's slot is its rank among the set bits
+    if (track) {
+        const uint32_t w0 = s_pil[tid], w1 = s_pil[tid + 64];
+        const uint32_t c0 = (uint32_t)__popc(w0), c1 = (uint32_t)__popc(w1);
+        uint32_t i0s = c0, i1s = c1;  // inclusive scans over the lanes
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u0 = __shfl_up(i0s, o, 64), u1 = __shfl_up(i1s, o, 64);
+            if (tid >= o) { i0s += u0; i1s += u1; }
+        }
+        const uint32_t tot0 = __shfl(i0s, 63, 64);
+        const uint32_t r0 = i0s - c0, r1 = tot0 + i1s - c1;
+        s_rank[tid] = (uint16_t)r0;
+        s_rank[tid + 64] = (uint16_t)r1;
+        uint32_t w = w0, slot = r0;
+        for (int h = 0; h < 2; h++) {
+            const int wbase = (h == 0 ? tid : tid + 64) * 32;
+            for (; w != 0u && slot < (uint32_t)kStagePillars; slot++) {
+                const int b = __ffs(w) - 1;
+                w &= w - 1u;
+                s_rec[slot] = pillar_rec(a.beamlets[B.beamlet0 + wbase + b]);
+            }
+            w = w1;
+            slot = r1;
+        }
+    }
     // pass B.1: a pillar without footprint samples on this line gets no dose on it, so its samples
     // act as walls (a pillar crossing is exactly one segment of the walk).  Samples confidently
     // in such pillars (or outside the grid's pillars) are walls; the others go to the exact list
@@ -368,13 +395,11 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         if (exact) s_list[nex + __popcll(mask & lt)] = (short)j;
         nex += __popcll(mask);
     }
-    ops += (uint32_t)m * kOpsCull / 64u + 1u;
     const uint32_t qbase = __shfl(qres, 0, 64);
     if (nex == 0) {  // every sample is a wall: no dose on this line, and the reservation stays empty
         if (tid == 0) a.lines[line].len = 0;
-        if (!a.fast)
-            for (uint32_t k = (uint32_t)tid; k < nfoot; k += 64)
-                if (qbase + k < subcap) qs[qbase + k].slot = kNoSlot;
+        for (uint32_t k = (uint32_t)tid; k < nfoot; k += 64)
+            if (qbase + k < subcap) qs[qbase + k].slot = kNoSlot;
         __syncthreads();
         continue;
     }
@@ -399,24 +424,21 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             j = s_list[r + tid];
             int pidx;
             v3 inp;
-            ops += kOpsLocate;
             if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) &&
                 (!track || ((s_pil[pidx >> 5] >> (pidx & 31)) & 1u))) {
-                const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+                PillarRec bl;
+                const uint32_t slot = track ? (uint32_t)s_rank[pidx >> 5] +
+                                                  (uint32_t)__popc(s_pil[pidx >> 5] & ((1u << (pidx & 31)) - 1u))
+                                            : (uint32_t)kStagePillars;
+                if (slot < (uint32_t)kStagePillars) bl = s_rec[slot];
+                else bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
                 h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
-                ops += kOpsDensity;
                 if (s_fp[j] & 1) {
                     const v3 gcs = bl_terma_point(a, B, bl, inp);
-                    if (a.fast) {
-                        const float fl = point_fluence(gcs, B.fmap, B.beam, bl.fidx);
-                        const v3 gi = (gcs - a.vd.start) / a.vd.voxel;
-                        t = fl * tgrid_fetch(B.tgrid, a.tg, gi) * a.bhc * inv_square(gcs, B.beam);
-                    } else {
-                        need = true;
-                        rq.slot = (uint32_t)(meta.off + j);
-                        rq.beamlet = (uint32_t)(B.beamlet0 + pidx);
-                        rq.x = gcs.x; rq.y = gcs.y; rq.z = gcs.z;
-                    }
+                    need = true;
+                    rq.slot = (uint32_t)(meta.off + j);
+                    rq.beamlet = (uint32_t)(B.beamlet0 + pidx);
+                    rq.x = gcs.x; rq.y = gcs.y; rq.z = gcs.z;
                 }
             }
             Hl[j] = h;
@@ -435,11 +457,10 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     }
     // The reservation counted footprint samples of pass A.2; pass B.2 requests TERMA for those of
     // them it maps to the same pillar again, so any shortfall is closed with empty entries.
-    if (!a.fast)
-        for (uint32_t k = qdone + (uint32_t)tid; k < nfoot; k += 64) {
-            const uint32_t pos = qbase + k;
-            if (pos < subcap) qs[pos].slot = kNoSlot;
-        }
+    for (uint32_t k = qdone + (uint32_t)tid; k < nfoot; k += 64) {
+        const uint32_t pos = qbase + k;
+        if (pos < subcap) qs[pos].slot = kNoSlot;
+    }
     // Walls before the first and after the last non-wall sample bound no segment and receive no
     // dose: the line's stored interval shrinks to [nlo, nhi], so the walk and the gather skip them
     // (the gather reads samples outside a line's interval as zero, the dose the walk gives a wall).
@@ -456,7 +477,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     }
     __syncthreads();  // the next line reuses the LDS arrays
     }
-    if (a.counts) wave_count(ops, a.counts, blockIdx.x);
+    if (a.counts) wave_count(stored, a.counts, blockIdx.x);
 }
 
 // Stage 2: the queued samples' TERMA (cudaSiddon_device with the beamlet filter), one sample per
@@ -713,28 +734,6 @@ __global__ void __launch_bounds__(256) bl_fill_kernel(float* __restrict__ p, uin
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = v;
 }
 
-// Unit TERMA at every CT voxel centre plus a one-voxel rim: sum_e phi_e E_e mu_e exp(-mu_e rpl)
-// with rpl from the same Siddon march as the per-sample TERMA.
-__global__ void __launch_bounds__(256) bl_tgrid_kernel(BlArgs a, BeamDev beam, float* __restrict__ g,
-                                                       uint32_t* counts) {
-    const uint64_t n = (uint64_t)a.tg[0] * a.tg[1] * a.tg[2];
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    uint32_t steps = 0;
-    if (i < n) {
-        const int x = (int)(i % (uint64_t)a.tg[0]);
-        const int y = (int)((i / (uint64_t)a.tg[0]) % (uint64_t)a.tg[1]);
-        const int z = (int)(i / ((uint64_t)a.tg[0] * a.tg[1]));
-        const v3 p = mk3(a.vd.start.x + (float)(x - 1) * a.vd.voxel.x, a.vd.start.y + (float)(y - 1) * a.vd.voxel.y,
-                         a.vd.start.z + (float)(z - 1) * a.vd.voxel.z);
-        int it = 0;
-        const float rpl = siddon_march(p, a.vd.start, a.vd.end, a.vd.voxel, beam.src, a.dens, a.vd.nx, a.vd.ny,
-                                       a.vd.nz, [](float, float, float, float) {}, &it);
-        g[i] = unit_terma(rpl, a.spec);
-        steps = (uint32_t)it;
-    }
-    if (counts) block_count(steps, counts, blockIdx.x);
-}
-
 struct UnpackArgs {
     const BatchDev* batches;
     const BeamletDev* beamlets;
@@ -796,6 +795,7 @@ struct ColArgs {
     int nx, ny;        // full volume dims (keys)
     float thresh;
     uint32_t* stat;    // per beamlet {kept, dropped, max bits, 0}
+    uint16_t* rowcnt;  // kept entries of every (i, j) row of every beamlet's box (from the scan)
     const uint64_t* off;
     uint64_t* keys;
     float* vals;
@@ -867,11 +867,14 @@ __global__ void __launch_bounds__(256) bl_colscan_kernel(ColArgs a) {
         const int i = r / bl.boxn[1], j = r - i * bl.boxn[1];
         int k0, k1;
         pillar_row(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, k0, k1);
+        uint32_t rk = 0;
         for (int k = k0; k < k1; k++) {
             const float v = pillar_value(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, bl.box0[2] + k);
-            if (v > cut) kept++;
+            if (v > cut) rk++;
             else if (v > 0.f) dropped++;
         }
+        kept += rk;
+        a.rowcnt[(size_t)bl.row0 + (size_t)r] = (uint16_t)rk;
     }
     for (int o = 32; o > 0; o >>= 1) {
         kept += __shfl_xor(kept, o, 64);
@@ -909,10 +912,16 @@ __global__ void __launch_bounds__(1024) bl_offsets_kernel(uint32_t* __restrict__
     if (threadIdx.x == 0) off[n] = carry;
 }
 
-// Writes each beamlet's kept entries at its offset (one workgroup per beamlet): chunks of 256 rows
-// in order; each thread counts its row, a workgroup scan places the rows, and each thread writes
-// its row's entries in k order.  key = full-volume linear index, value = dose.
+// Writes each beamlet's kept entries at its offset (one workgroup per beamlet), key = full-volume
+// linear index, value = dose, in the emission order of _sparsify.  The rows go in chunks of 256,
+// one per thread, with their kept counts from the column scan; each wave's 64 rows form one
+// contiguous range of the output.  A wave fills that range window by window: every lane evaluates
+// its row's values once, in k order, and stages the kept entries that fall in the window in LDS;
+// then the wave copies the window out with whole-wave, contiguous stores.
+constexpr uint32_t kEmitWindow = 512;  // staged entries per wave and round (6 KB)
 __global__ void __launch_bounds__(256) bl_emit_kernel(ColArgs a) {
+    __shared__ uint64_t s_key[4][kEmitWindow];
+    __shared__ float s_val[4][kEmitWindow];
     __shared__ uint32_t s_w[4];
     const BeamletDev& bl = a.beamlets[blockIdx.x];
     const BatchDev& B = a.batches[bl.batch];
@@ -922,16 +931,8 @@ __global__ void __launch_bounds__(256) bl_emit_kernel(ColArgs a) {
     uint64_t pos = a.off[blockIdx.x];
     for (int base = 0; base < rows; base += 256) {
         const int r = base + t;
-        int i = 0, j = 0, k0 = 0, k1 = 0;
-        uint32_t n = 0;
-        if (r < rows) {
-            i = r / bl.boxn[1];
-            j = r - i * bl.boxn[1];
-            pillar_row(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, k0, k1);
-            for (int k = k0; k < k1; k++)
-                n += pillar_value(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, bl.box0[2] + k) > cut ? 1u : 0u;
-        }
-        // exclusive scan of n over the workgroup (wave scans + 4 wave totals)
+        const uint32_t n = r < rows ? (uint32_t)a.rowcnt[(size_t)bl.row0 + (size_t)r] : 0u;
+        // inclusive scan of n over the wave, and the waves' totals
         uint32_t inc = n;
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t u = __shfl_up(inc, o, 64);
@@ -939,25 +940,42 @@ __global__ void __launch_bounds__(256) bl_emit_kernel(ColArgs a) {
         }
         if (lane == 63) s_w[w] = inc;
         __syncthreads();
-        uint32_t before = 0, total = 0;
+        uint32_t before = 0, total = 0, most = 0;
         for (int q = 0; q < 4; q++) {
             before += q < w ? s_w[q] : 0u;
             total += s_w[q];
+            most = max(most, s_w[q]);
         }
+        const uint32_t W = s_w[w];
         __syncthreads();
-        uint64_t p = pos + before + inc - n;
+        int X = 0, Y = 0, k = 0, k1 = 0;
         if (n) {
-            const int X = bl.box0[0] + i, Y = bl.box0[1] + j;
-            for (int k = k0; k < k1; k++) {
+            const int i = r / bl.boxn[1], j = r - i * bl.boxn[1];
+            pillar_row(a, bl, B, bl.box0[0] + i, bl.box0[1] + j, k, k1);
+            X = bl.box0[0] + i;
+            Y = bl.box0[1] + j;
+        }
+        uint32_t p = inc - n;  // next output of this lane, relative to the wave's range
+        const uint64_t wout = pos + before;
+        // the same number of rounds in every wave (the barriers), enough for the fullest wave
+        for (uint32_t wb = 0; wb < most; wb += kEmitWindow) {
+            const uint32_t we = min(W, wb + kEmitWindow);
+            while (p < inc && p < we && k < k1) {
                 const float v = pillar_value(a, bl, B, X, Y, bl.box0[2] + k);
                 if (v > cut) {
-                    a.keys[p] = ((uint64_t)(bl.box0[2] + k + a.bbs[2]) * (uint64_t)a.ny + (uint64_t)(Y + a.bbs[1])) *
-                                    (uint64_t)a.nx +
-                                (uint64_t)(X + a.bbs[0]);
-                    a.vals[p] = v;
+                    s_key[w][p - wb] = ((uint64_t)(bl.box0[2] + k + a.bbs[2]) * (uint64_t)a.ny +
+                                        (uint64_t)(Y + a.bbs[1])) * (uint64_t)a.nx + (uint64_t)(X + a.bbs[0]);
+                    s_val[w][p - wb] = v;
                     p++;
                 }
+                k++;
             }
+            __syncthreads();
+            for (uint32_t e = wb + (uint32_t)lane; e < we; e += 64) {
+                a.keys[wout + e] = s_key[w][e - wb];
+                a.vals[wout + e] = s_val[w][e - wb];
+            }
+            __syncthreads();
         }
         pos += total;
     }
@@ -985,7 +1003,7 @@ __global__ void __launch_bounds__(256) plain_ray_kernel(BlArgs a, float* __restr
     int pidx;
     v3 inp;
     if (bl_locate(a, c, B, tX, tY, tZ, pidx, inp)) {
-        const BeamletDev& bl = a.beamlets[B.beamlet0 + pidx];
+        const PillarRec bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
         t = point_terma(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, nullptr);
         d = bl_density(a, B, bl, inp);
     }
@@ -1230,7 +1248,7 @@ struct Job {
     std::vector<BeamletDev> beamlets;
     std::vector<BlConeDev> cones;          // [ray][batch]
     std::vector<uint64_t> ray_lines;       // lines per ray
-    uint64_t max_lines = 0, max_rev = 0, pdose = 0, arena = 0;
+    uint64_t max_lines = 0, max_rev = 0, pdose = 0, arena = 0, rows = 0;
     uint64_t pvox = 0;                     // pillar voxels of all beamlets (bounds the sparse output)
     uint32_t tiles = 0;
 };
@@ -1260,8 +1278,20 @@ void pillar_box(const cccs_problem& p, const cccs_beam& b, const BeamletPlan& o,
     }
 }
 
+// A finished job's sparse columns in pinned host memory, and the event that marks them copied.
+struct Delivery {
+    bool live = false;
+    hipEvent_t ev = nullptr;
+    uint64_t* keys = nullptr;
+    float* vals = nullptr;
+    uint32_t* stat = nullptr;  // per column: kept, dropped, (two unused)
+    uint32_t* rcnt = nullptr;  // per column and ROI: kept entries (ROI reduction)
+    int nroi = 0;
+    std::vector<std::pair<int, int>> cols;  // (request beam, fluence index) per column
+};
+
 int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vector<BatchRef>& refs,
-              const std::vector<const float*>& d_fmaps, const std::vector<const float*>& d_tgrids, Job& J) {
+              const std::vector<const float*>& d_fmaps, Job& J) {
     const cccs_problem& p = ctx->prob;
     const v3 rvs = mk3(p.rev_latspacing, p.rev_longspacing, p.rev_latspacing);
     const int nb = (int)refs.size(), nrays = ctx->nrays;
@@ -1280,7 +1310,6 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
         B.beam = make_beam_dev(b);
         B.beam_inv = rot2_beam_inv(b.azimuth, b.zenith, b.coll);
         B.fmap = d_fmaps[R.plan];
-        B.tgrid = d_tgrids.empty() ? nullptr : d_tgrids[R.plan];
         B.n = R.n;
         B.npx = (int)std::ceil(std::sqrt((float)R.n));
         B.npy = (int)std::ceil((float)R.n / (float)B.npx);
@@ -1320,6 +1349,11 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
             pillar_box(p, b, o, P.pd, d.box0, d.boxn);
             d.arena = J.arena;
             J.arena += (uint64_t)d.boxn[0] * d.boxn[1] * d.boxn[2];
+            // row counts are 16-bit and row offsets 32-bit
+            if (d.boxn[2] > 65535 || J.rows + (uint64_t)d.boxn[0] * d.boxn[1] > 0xffffffffULL)
+                return set_error(CCCS_EGEOM, "column boxes too large for one job");
+            d.row0 = (uint32_t)J.rows;
+            J.rows += (uint64_t)d.boxn[0] * d.boxn[1];
             J.pvox += (uint64_t)P.pd[0] * P.pd[1] * P.pd[2];
             J.beamlets.push_back(d);
         }
@@ -1354,7 +1388,7 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
     return CCCS_OK;
 }
 
-BlArgs base_args(cccs_ctx* ctx, int fast) {
+BlArgs base_args(cccs_ctx* ctx) {
     const cccs_problem& p = ctx->prob;
     BlArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -1369,37 +1403,14 @@ BlArgs base_args(cccs_ctx* ctx, int fast) {
     a.vd = ctx->vol;
     a.spec = ctx->spec;
     a.bhc = p.beamhard_correct;
-    a.fast = fast;
-    a.tg[0] = (int)p.size[0] + 2;
-    a.tg[1] = (int)p.size[1] + 2;
-    a.tg[2] = (int)p.size[2] + 2;
     return a;
 }
 
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)std::min<uint64_t>((n + block - 1) / block, 1u << 20); }
 
-// Unit-TERMA grids of the job's beams (fast mode).
-int run_tgrids(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, std::vector<const float*>& d_tgrids) {
-    const cccs_problem& p = ctx->prob;
-    const uint64_t ng = (uint64_t)(p.size[0] + 2) * (p.size[1] + 2) * (p.size[2] + 2);
-    CCCS_TRY(ctx->bl_tgrid.ensure(sizeof(float) * ng * std::max<size_t>(1, plans.size())));
-    BlArgs a = base_args(ctx, 1);
-    d_tgrids.resize(plans.size());
-    for (size_t i = 0; i < plans.size(); i++) {
-        float* g = ctx->bl_tgrid.as<float>() + ng * i;
-        d_tgrids[i] = g;
-        const unsigned blocks = (unsigned)((ng + 255) / 256);
-        uint32_t* cnt = prof_counter(ctx, blocks);
-        int tok = prof_begin(ctx, K_BL_TGRID);
-        hipLaunchKernelGGL(bl_tgrid_kernel, dim3(blocks), dim3(256), 0, ctx->stream, a, make_beam_dev(*plans[i].b), g, cnt);
-        prof_end(ctx, tok, 4.0 * (double)ctx->vol.nx * ctx->vol.ny * ctx->vol.nz + 4.0 * (double)ng, 0.0, ng);
-        CCCS_HIP(hipGetLastError());
-    }
-    return CCCS_OK;
-}
-
-// Cone passes of one job: every cone direction over every batch, dose into the packed grids.
-int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
+// `while_busy` (may be empty) runs on the host once every launch of the job's first pass is
+// enqueued, before the host waits for them: host work there overlaps the job on the device.
+int run_job_cones(cccs_ctx* ctx, const Job& J, const std::function<int()>& while_busy) {
     const cccs_problem& p = ctx->prob;
     const int nb = (int)J.batches.size(), nrays = ctx->nrays;
     // device tables: batches | beamlets | cones
@@ -1427,7 +1438,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     uint64_t subcap = std::max<uint64_t>(uint64_t(1) << 14, J.max_rev / 8 / kSubQueues);
     CCCS_TRY(ctx->bl_qcnt.ensure(sizeof(uint32_t) * kSubQueues * kCntStride * (size_t)std::max(1, nrays)));
     CCCS_TRY(ctx->h_small.ensure(sizeof(uint32_t) * kSubQueues * kCntStride * kMaxRays));
-    BlArgs a = base_args(ctx, fast);
+    BlArgs a = base_args(ctx);
     a.batches = reinterpret_cast<const BatchDev*>(d);
     a.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l);
     a.lines = ctx->lines.as<LineMeta>();
@@ -1451,11 +1462,14 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
     S[0] = ctx->stream;
     set_lines[0] = &ctx->lines; set_rev[0] = &ctx->rev; set_h[0] = &ctx->bl_h; set_q[0] = &ctx->bl_queue;
     set_scan[0] = &ctx->scan;
+    DevBuf* set_cone[kRaySets];
+    set_cone[0] = &ctx->bl_lcone;
     for (int k = 1; k < kRaySets; k++) {
         S[k] = ctx->xstream[k - 1];
         set_lines[k] = &ctx->xlines[k - 1]; set_rev[k] = &ctx->xrev[k - 1]; set_h[k] = &ctx->xh[k - 1];
-        set_q[k] = &ctx->xqueue[k - 1]; set_scan[k] = &ctx->xscan[k - 1];
+        set_q[k] = &ctx->xqueue[k - 1]; set_scan[k] = &ctx->xscan[k - 1]; set_cone[k] = &ctx->xlcone[k - 1];
     }
+    for (int k = 0; k < nset; k++) CCCS_TRY(set_cone[k]->ensure(sizeof(uint32_t) * std::max<uint64_t>(1, J.max_lines)));
     for (int k = 1; k < nset; k++) {
         CCCS_TRY(set_lines[k]->ensure(sizeof(LineMeta) * std::max<uint64_t>(1, J.max_lines)));
         CCCS_TRY(set_rev[k]->ensure(sizeof(float) * std::max<uint64_t>(1, J.max_rev)));
@@ -1488,6 +1502,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             ctx->stream = S[k];
             std::swap(ctx->scan, *set_scan[k]);
             a.lines = set_lines[k]->as<LineMeta>();
+            a.line_cone = set_cone[k]->as<uint32_t>();
             a.rev = set_rev[k]->as<float>();
             float* H = set_h[k]->as<float>();
             TermaReq* q = set_q[k]->as<TermaReq>();
@@ -1506,7 +1521,7 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
                                    qcnt + (size_t)kSubQueues * kCntStride * r);
                 prof_end(ctx, tok, 0.0, 0.0, a.nlines);
                 if (hipGetLastError() != hipSuccess) { rc = set_error(CCCS_EDEVICE, "sample kernel"); break; }
-                if (!fast) {
+                {
                     const unsigned sb = kSubQueues * 64;
                     uint32_t* cnt = prof_counter(ctx, sb);
                     int ts = prof_begin(ctx, K_BL_SIDDON);
@@ -1559,7 +1574,13 @@ int run_job_cones(cccs_ctx* ctx, const Job& J, int fast) {
             for (int k = 0; k < nset; k++) (void)hipStreamSynchronize(S[k]);
             return rc;
         }
-        if (fast) return CCCS_OK;
+        if (attempt == 0 && while_busy) {
+            const int hr = while_busy();
+            if (hr != CCCS_OK) {
+                for (int k = 0; k < nset; k++) (void)hipStreamSynchronize(S[k]);
+                return hr;
+            }
+        }
         uint32_t* hc = ctx->h_small.as<uint32_t>();
         CCCS_HIP(hipMemcpyAsync(hc, qcnt, sizeof(uint32_t) * kSubQueues * kCntStride * nrays, hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -1631,7 +1652,7 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
                     const std::vector<const float*>& d_fmaps, float thresh, cccs_sparse_sink sink, void* user) {
     const cccs_problem& p = ctx->prob;
     Job J;
-    CCCS_TRY(build_job(ctx, plans, std::vector<BatchRef>{R}, d_fmaps, std::vector<const float*>(), J));
+    CCCS_TRY(build_job(ctx, plans, std::vector<BatchRef>{R}, d_fmaps, J));
     const uint64_t nbb = (uint64_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
     for (BeamletDev& d : J.beamlets) {
         for (int k = 0; k < 3; k++) { d.box0[k] = 0; d.boxn[k] = (int)p.bbox_size[k]; }
@@ -1661,7 +1682,7 @@ int run_plain_batch(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const Bat
     CCCS_TRY(ctx->lines.ensure(sizeof(LineMeta) * std::max<uint64_t>(1, max_lines)));
     CCCS_TRY(ctx->rev.ensure(sizeof(float) * std::max<uint64_t>(1, max_rev)));
     CCCS_TRY(ctx->bl_h.ensure(sizeof(float) * 2 * std::max<uint64_t>(1, max_rev)));
-    BlArgs a = base_args(ctx, 0);
+    BlArgs a = base_args(ctx);
     a.batches = reinterpret_cast<const BatchDev*>(d);
     a.beamlets = reinterpret_cast<const BeamletDev*>(d + o_l);
     a.lines = ctx->lines.as<LineMeta>();
@@ -1827,9 +1848,9 @@ int upload_fmaps(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const float*
 
 // Runs the cone passes and the unpack of one job (the arena then holds every beamlet's box).
 int run_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vector<BatchRef>& refs,
-            const std::vector<const float*>& d_fmaps, const std::vector<const float*>& d_tgrids, int fast, Job& J) {
-    CCCS_TRY(build_job(ctx, plans, refs, d_fmaps, d_tgrids, J));
-    CCCS_TRY(run_job_cones(ctx, J, fast));
+            const std::vector<const float*>& d_fmaps, Job& J) {
+    CCCS_TRY(build_job(ctx, plans, refs, d_fmaps, J));
+    CCCS_TRY(run_job_cones(ctx, J, std::function<int()>()));
     return run_job_unpack(ctx, J);
 }
 
@@ -2056,7 +2077,7 @@ extern "C" int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const fl
                                   int max_beamlets, int* nactive) {
     clear_error();
     if (!ctx || !beam || !fluence || !out) return set_error(CCCS_EINVAL, "null argument");
-    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST) return set_error(CCCS_EINVAL, "unknown beamlet mode");
+    if (mode != CCCS_BEAMLET_FAITHFUL) return set_error(CCCS_EINVAL, "unknown beamlet mode");
     CCCS_HIP(hipSetDevice(ctx->device));
     CCCS_TRY(check_beam(ctx, *beam));
     std::vector<BeamPlan> plans(1);
@@ -2068,15 +2089,14 @@ extern "C" int cccs_beamlet_dense(cccs_ctx* ctx, const cccs_beam* beam, const fl
     const cccs_problem& p = ctx->prob;
     const size_t nbb = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
     std::memset(out, 0, sizeof(float) * nbb * n);
-    std::vector<const float*> d_fmaps, d_tgrids;
+    std::vector<const float*> d_fmaps;
     const float* fl[1] = {fluence};
     CCCS_TRY(upload_fmaps(ctx, plans, fl, d_fmaps));
-    if (mode == CCCS_BEAMLET_FAST) CCCS_TRY(run_tgrids(ctx, plans, d_tgrids));
     // one job per batch, as the reference processes its batches
     int done = 0;
     for (const BatchRef& R : all_batches(plans)) {
         Job J;
-        CCCS_TRY(run_job(ctx, plans, std::vector<BatchRef>{R}, d_fmaps, d_tgrids, mode == CCCS_BEAMLET_FAST, J));
+        CCCS_TRY(run_job(ctx, plans, std::vector<BatchRef>{R}, d_fmaps, J));
         std::vector<float> arena(J.arena);
         CCCS_HIP(hipStreamSynchronize(ctx->stream));
         if (J.arena) CCCS_HIP(hipMemcpy(arena.data(), ctx->bl_arena.p, sizeof(float) * J.arena, hipMemcpyDeviceToHost));
@@ -2105,7 +2125,7 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     if (!sink && mode == CCCS_BEAMLET_PLAIN) return set_error(CCCS_EINVAL, "the plain path needs a sink");
     if (ctx && ctx->nroi > 0 && mode == CCCS_BEAMLET_PLAIN)
         return set_error(CCCS_EINVAL, "ROI reduction runs in the batched modes");
-    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_FAST && mode != CCCS_BEAMLET_PLAIN)
+    if (mode != CCCS_BEAMLET_FAITHFUL && mode != CCCS_BEAMLET_PLAIN)
         return set_error(CCCS_EINVAL, "unknown beamlet mode");
     if (!(threshold >= 0.f)) return set_error(CCCS_EINVAL, "threshold must be >= 0");
     CCCS_HIP(hipSetDevice(ctx->device));
@@ -2121,9 +2141,8 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     }
     if (plans.empty()) return CCCS_OK;
     CCCS_TRY(ctx->h_small.ensure(64));
-    std::vector<const float*> d_fmaps, d_tgrids;
+    std::vector<const float*> d_fmaps;
     CCCS_TRY(upload_fmaps(ctx, plans, fluence, d_fmaps));
-    if (mode == CCCS_BEAMLET_FAST) CCCS_TRY(run_tgrids(ctx, plans, d_tgrids));
     const std::vector<BatchRef> refs = all_batches(plans);
     if (mode == CCCS_BEAMLET_PLAIN) {
         for (const BatchRef& R : refs) CCCS_TRY(run_plain_batch(ctx, plans, R, d_fmaps, threshold, sink, user));
@@ -2141,6 +2160,44 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     std::vector<uint64_t> b_fixed(refs.size());
     for (size_t i = 0; i < refs.size(); i++)
         CCCS_TRY(batch_footprint(ctx, plans, refs[i], b_rev[i], b_lines[i], b_fixed[i]));
+    // columns of the last finished job, waiting in pinned host memory for the sink
+    for (hipEvent_t& e : ctx->stage_ev)
+        if (!e) CCCS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    Delivery pending;
+    int njob = 0;
+    auto deliver = [&]() -> int {
+        if (!pending.live) return CCCS_OK;
+        pending.live = false;
+        CCCS_HIP(hipEventSynchronize(pending.ev));
+        const int nroi = pending.nroi;
+        std::vector<uint64_t> blocks((size_t)nroi);
+        uint64_t pos = 0;
+        for (size_t q = 0; q < pending.cols.size(); q++) {
+            const uint32_t* st = pending.stat + 4 * q;
+            cccs_sparse_column col{};
+            col.beam_index = pending.cols[q].first;
+            col.beamlet_index = pending.cols[q].second;
+            col.nnz = st[0];
+            col.perc_nonzero = (float)(100.0 * (double)st[0] / (double)nvox);
+            col.perc_dropped = (float)(100.0 * (double)st[1] / (double)nvox);
+            if (nroi > 0) {
+                // _reduce_by_roi's statistics (sparse_io.cpp:150-156)
+                uint64_t kept = 0;
+                for (int r = 0; r < nroi; r++) kept += blocks[(size_t)r] = pending.rcnt[q * (size_t)nroi + (size_t)r];
+                const double pn = 100.0 * (double)kept / (double)nvox;
+                col.perc_reducedrop = (float)((double)col.perc_nonzero - pn);
+                col.perc_nonzero = (float)pn;
+                col.nnz = kept;
+                col.nblocks = nroi;
+                col.row_block_sizes = blocks.data();
+            }
+            col.keys = pending.keys + pos;
+            col.values = pending.vals + pos;
+            pos += col.nnz;
+            if (sink(user, &col) != 0) return set_error(CCCS_EIO, "sparse sink aborted the call");
+        }
+        return CCCS_OK;
+    };
     while (next < refs.size()) {
         // grow the job batch by batch while it fits the budget
         std::vector<uint64_t> rev(nrays, 0), lines(nrays, 0);
@@ -2165,9 +2222,10 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         }
         std::vector<BatchRef> sel(refs.begin() + (long)next, refs.begin() + (long)k);
         Job J;
-        CCCS_TRY(build_job(ctx, plans, sel, d_fmaps, d_tgrids, J));
+        CCCS_TRY(build_job(ctx, plans, sel, d_fmaps, J));
         next = k;
-        CCCS_TRY(run_job_cones(ctx, J, mode == CCCS_BEAMLET_FAST));
+        // the previous job's columns reach the sink while this job's cone passes run on the device
+        CCCS_TRY(run_job_cones(ctx, J, deliver));
         const int nbl = (int)J.beamlets.size();
         CCCS_TRY(ctx->bl_cnt.ensure(sizeof(uint32_t) * 4 * nbl + sizeof(uint64_t) * (nbl + 1) + 256));
         uint32_t* stat = ctx->bl_cnt.as<uint32_t>();
@@ -2185,6 +2243,8 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         ca.ny = (int)p.size[1];
         ca.thresh = threshold;
         ca.stat = stat;
+        CCCS_TRY(ctx->bl_rowcnt.ensure(sizeof(uint16_t) * std::max<uint64_t>(1, J.rows)));
+        ca.rowcnt = ctx->bl_rowcnt.as<uint16_t>();
         ca.off = offs;
         ca.keys = nullptr;
         ca.vals = nullptr;
@@ -2194,7 +2254,6 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         hipLaunchKernelGGL(bl_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, stat, nbl, offs);
         CCCS_HIP(hipGetLastError());
         // totals decide the output size
-        std::vector<uint32_t> hstat(4 * (size_t)nbl);
         uint64_t nnz_total = 0;
         CCCS_HIP(hipMemcpyAsync(ctx->h_small.p, offs + nbl, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
         CCCS_HIP(hipStreamSynchronize(ctx->stream));
@@ -2245,56 +2304,39 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
             CCCS_HIP(hipGetLastError());
         }
         if (!sink) continue;  // columns stay on the device (overwritten by the next job)
-        // columns to pinned host memory (full PCIe rate), then the sink sees them in place
+        // columns to pinned host memory (full PCIe rate, one of two buffers in turn); the sink sees
+        // them in place, during the next job or at the end of the call
         const uint64_t nout = nroi > 0 ? rtotal : nnz_total;
         const size_t kbytes = sizeof(uint64_t) * nout, vbytes = sizeof(float) * nout;
         const size_t sbytes = sizeof(uint32_t) * 4 * (size_t)nbl;
         const size_t rbytes = sizeof(uint32_t) * (size_t)nbl * (size_t)nroi;
-        CCCS_HIP(hipStreamSynchronize(ctx->stream));
-        CCCS_TRY(ctx->h_bl_out.ensure(kbytes + vbytes + sbytes + rbytes + 64));
-        uint8_t* hb = ctx->h_bl_out.as<uint8_t>();
-        uint64_t* hkeys = reinterpret_cast<uint64_t*>(hb);
-        float* hvals = reinterpret_cast<float*>(hb + kbytes);
-        uint32_t* hst = reinterpret_cast<uint32_t*>(hb + kbytes + vbytes);
-        uint32_t* hrc = hst + 4 * (size_t)nbl;
-        CCCS_HIP(hipMemcpyAsync(hst, stat, sbytes, hipMemcpyDeviceToHost, ctx->stream));
-        if (nroi > 0) CCCS_HIP(hipMemcpyAsync(hrc, rcnt, rbytes, hipMemcpyDeviceToHost, ctx->stream));
-        if (nout) {
-            CCCS_HIP(hipMemcpyAsync(hkeys, nroi > 0 ? ctx->roi_keys.p : ctx->bl_keys.p, kbytes, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-            CCCS_HIP(hipMemcpyAsync(hvals, nroi > 0 ? ctx->roi_vals.p : ctx->bl_vals.p, vbytes, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-        }
-        CCCS_HIP(hipStreamSynchronize(ctx->stream));
-        std::memcpy(hstat.data(), hst, sbytes);
-        std::vector<uint64_t> blocks((size_t)nroi);
-        uint64_t pos = 0;
+        HostBuf& hbuf = (njob & 1) ? ctx->h_bl_out2 : ctx->h_bl_out;
+        CCCS_TRY(hbuf.ensure(kbytes + vbytes + sbytes + rbytes + 64));
+        uint8_t* hb = hbuf.as<uint8_t>();
+        Delivery& D = pending;
+        D.keys = reinterpret_cast<uint64_t*>(hb);
+        D.vals = reinterpret_cast<float*>(hb + kbytes);
+        D.stat = reinterpret_cast<uint32_t*>(hb + kbytes + vbytes);
+        D.rcnt = D.stat + 4 * (size_t)nbl;
+        D.nroi = nroi;
+        D.cols.resize((size_t)nbl);
         for (int q = 0; q < nbl; q++) {
             const BeamletDev& d = J.beamlets[q];
-            const BatchRef& R = sel[d.batch];
-            const BeamPlan& P = plans[R.plan];
-            cccs_sparse_column col{};
-            col.beam_index = P.beam;
-            col.beamlet_index = d.fidx;
-            col.nnz = hstat[4 * q];
-            col.perc_nonzero = (float)(100.0 * (double)hstat[4 * q] / (double)nvox);
-            col.perc_dropped = (float)(100.0 * (double)hstat[4 * q + 1] / (double)nvox);
-            if (nroi > 0) {
-                // _reduce_by_roi's statistics (sparse_io.cpp:150-156)
-                uint64_t kept = 0;
-                for (int r = 0; r < nroi; r++) kept += blocks[(size_t)r] = hrc[(size_t)q * nroi + r];
-                const double pn = 100.0 * (double)kept / (double)nvox;
-                col.perc_reducedrop = (float)((double)col.perc_nonzero - pn);
-                col.perc_nonzero = (float)pn;
-                col.nnz = kept;
-                col.nblocks = nroi;
-                col.row_block_sizes = blocks.data();
-            }
-            col.keys = hkeys + pos;
-            col.values = hvals + pos;
-            pos += col.nnz;
-            if (sink(user, &col) != 0) return set_error(CCCS_EIO, "sparse sink aborted the call");
+            D.cols[(size_t)q] = {plans[sel[d.batch].plan].beam, d.fidx};
         }
+        CCCS_HIP(hipMemcpyAsync(D.stat, stat, sbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (nroi > 0) CCCS_HIP(hipMemcpyAsync(D.rcnt, rcnt, rbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (nout) {
+            CCCS_HIP(hipMemcpyAsync(D.keys, nroi > 0 ? ctx->roi_keys.p : ctx->bl_keys.p, kbytes, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+            CCCS_HIP(hipMemcpyAsync(D.vals, nroi > 0 ? ctx->roi_vals.p : ctx->bl_vals.p, vbytes, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        }
+        CCCS_HIP(hipEventRecord(ctx->stage_ev[njob & 1], ctx->stream));
+        D.ev = ctx->stage_ev[njob & 1];
+        D.live = true;
+        njob++;
     }
+    CCCS_TRY(deliver());
     return CCCS_OK;
 }

@@ -64,7 +64,7 @@ void HostBuf::release() {
     bytes = 0;
 }
 
-const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_tgrid", "bl_sample", "bl_siddon",
+const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_sample", "bl_siddon",
                                             "bl_walk", "bl_gather", "bl_unpack", "bl_compact", "plain_ray",
                                             "plain_conv"};
 
@@ -322,14 +322,15 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     c->dose.release(); c->col.release(); c->lines.release(); c->scan.release(); c->dt.release();
     for (int i = 0; i < kRaySets - 1; i++) {
         c->xlines[i].release(); c->xrev[i].release(); c->xh[i].release(); c->xqueue[i].release(); c->xscan[i].release();
+        c->xlcone[i].release();
     }
     c->h_small.release(); c->h_cones.release(); c->h_stage.release();
     for (hipEvent_t& e : c->stage_ev)
         if (e) (void)hipEventDestroy(e);
-    c->bl_tab.release(); c->bl_pdose.release(); c->bl_tgrid.release(); c->bl_arena.release();
+    c->bl_tab.release(); c->bl_pdose.release(); c->bl_lcone.release(); c->bl_rowcnt.release(); c->bl_arena.release();
     c->bl_cnt.release(); c->bl_keys.release(); c->bl_vals.release(); c->bl_fmap.release();
     c->bl_h.release(); c->bl_queue.release(); c->bl_qcnt.release(); c->bl_tiles.release();
-    c->h_bl.release(); c->h_bl_out.release();
+    c->h_bl.release(); c->h_bl_out.release(); c->h_bl_out2.release();
     c->roi_tab.release(); c->roi_rank.release(); c->roi_cnt.release(); c->roi_keys.release(); c->roi_vals.release();
     for (hipEvent_t e : c->prof.pool) (void)hipEventDestroy(e);
     for (uint32_t* p : c->prof.chunks) (void)hipFree(p);
@@ -517,6 +518,7 @@ extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
 // validity tests, plane choice, segment length, midpoint fetch coordinate, 8-tap trilinear: 40 ops).
 static const double kFlopsPerWalkIter = 10.0;
 static const double kFlopsPerSiddonStep = 40.0;
+static const double kBytesPerStoredSample = 16.0;
 
 extern "C" int cccs_ctx_profile(cccs_ctx* ctx, int enable) {
     clear_error();
@@ -570,9 +572,11 @@ extern "C" int cccs_ctx_profile_read(cccs_ctx* ctx, cccs_kernel_stats* out, int 
     rows[K_CONE].alg_flops += kFlopsPerWalkIter * (double)cnt[K_CONE];
     rows[K_BL_WALK].alg_flops += kFlopsPerWalkIter * (double)cnt[K_BL_WALK];
     rows[K_TERMA].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_TERMA];
-    rows[K_BL_TGRID].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_TGRID];
     rows[K_BL_SIDDON].alg_flops += kFlopsPerSiddonStep * (double)cnt[K_BL_SIDDON];
-    rows[K_BL_SAMPLE].alg_flops += (double)cnt[K_BL_SAMPLE];  // counted in FLOPs by the kernel
+    // the sample stage counts stored REV samples and is priced as the BEV resampling of a cone:
+    // density and TERMA read at each sample (8 B) and written line-major (8 B)
+    rows[K_BL_SAMPLE].alg_bytes += kBytesPerStoredSample * (double)cnt[K_BL_SAMPLE];
+    rows[K_BL_SAMPLE].work_items = cnt[K_BL_SAMPLE];
     int n = 0;
     for (int k = 0; k < K_COUNT && n < max_rows; k++) {
         if (rows[k].launches == 0) continue;

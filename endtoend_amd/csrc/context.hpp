@@ -60,7 +60,7 @@ struct BeamGeom {
 };
 
 // ---- profiling ---------------------------------------------------------------------------------
-enum KernelKind { K_TERMA = 0, K_CROP, K_CONE, K_PVCS, K_BL_TGRID, K_BL_SAMPLE, K_BL_SIDDON, K_BL_WALK, K_BL_GATHER,
+enum KernelKind { K_TERMA = 0, K_CROP, K_CONE, K_PVCS, K_BL_SAMPLE, K_BL_SIDDON, K_BL_WALK, K_BL_GATHER,
                   K_BL_UNPACK, K_BL_COMPACT, K_PL_RAY, K_PL_CONV, K_COUNT };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -137,15 +137,16 @@ struct cccs_ctx {
     float* d_tables = nullptr;  // kernel (ntheta*nradii) | radii | inverse widths | mean radii | inverse widths
     size_t resident = 0;
     cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan, dt;
-    // ray sets 1.. of the beamlet pipeline (set 0 is lines/rev/bl_h/bl_queue/scan): line table,
-    // REV store, half steps, TERMA queue, scan partials
+    // ray sets 1.. of the beamlet pipeline (set 0 is lines/rev/bl_h/bl_queue/scan/bl_lcone): line
+    // table, REV store, half steps, TERMA queue, scan partials, grid of every line
     cccs::DevBuf xlines[cccs::kRaySets - 1], xrev[cccs::kRaySets - 1], xh[cccs::kRaySets - 1],
-        xqueue[cccs::kRaySets - 1], xscan[cccs::kRaySets - 1];
+        xqueue[cccs::kRaySets - 1], xscan[cccs::kRaySets - 1], xlcone[cccs::kRaySets - 1];
     cccs::HostBuf h_small, h_cones, h_stage;  // h_stage: pinned double buffer of download_staged
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
-    // beamlet path: job tables, packed grids, unit-TERMA grids, column arena, sparse output
-    cccs::DevBuf bl_tab, bl_pdose, bl_tgrid, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt, bl_tiles;
-    cccs::HostBuf h_bl, h_bl_out;
+    // beamlet path: job tables, packed grids, column arena, sparse output, line grids
+    cccs::DevBuf bl_tab, bl_pdose, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt, bl_tiles,
+        bl_lcone, bl_rowcnt;
+    cccs::HostBuf h_bl, h_bl_out, h_bl_out2;  // h_bl_out/h_bl_out2: sparse columns of alternate jobs
     std::vector<uint8_t> scratch_host;
     cccs::Profiler prof;
     cccs::Schedule sched;

@@ -144,7 +144,7 @@ __device__ __forceinline__ float walk_both(const float2* __restrict__ s, int j, 
 
 // Narrows [t0, t1] to the parameters where o + t*d lies in [lo, hi) (widened by a margin that
 // covers FP32 rounding; the exact per-sample test decides membership).
-__device__ __forceinline__ void clip_slab(float o, float d, float lo, float hi, float& t0, float& t1) {
+CCCS_HD void clip_slab(float o, float d, float lo, float hi, float& t0, float& t1) {
     if (fabsf(d) < 1e-3f) {
         // the coordinate moves by less than one index unit over a whole line (<= 1024 samples)
         if (!(o >= lo - 2.f && o < hi + 2.f)) { t0 = 1.f; t1 = 0.f; }

@@ -119,13 +119,11 @@ typedef struct cccs_ctx cccs_ctx;
  *   CCCS_BEAMLET_FAITHFUL : batched pipeline; every footprint REV sample of every cone gets its
  *                           own Siddon TERMA, as the reference computes it
  *                           (dosecalc-beamlet/nvbbRayConvolve_device.cu:170-211).
- *   CCCS_BEAMLET_FAST     : batched pipeline; TERMA interpolated from a per-beam unit-TERMA field
- *                           on the CT grid (approximate: not bound by the 0.5 % Dmax tolerance).
  *   CCCS_BEAMLET_PLAIN    : the reference dataflow (one batch at a time, per-cone launches over
  *                           whole REV grids, dense per-beamlet unpack copied to the host and
- *                           sparsified there); the baseline of the batched pipeline.           */
+ *                           sparsified there); the baseline of the batched pipeline.
+ * Any other value is rejected with CCCS_EINVAL. */
 #define CCCS_BEAMLET_FAITHFUL 0
-#define CCCS_BEAMLET_FAST     1
 #define CCCS_BEAMLET_PLAIN    2
 
 /* ---- context: replaces initCudaConstandTex / freeCudaTexture --------------------------------
@@ -220,7 +218,7 @@ int cccs_siddon_paths(cccs_ctx* ctx, const float* source, const float* points, i
  * in beam order then active-beamlet order.  threshold: keep v > threshold * max(column).
  * sink may be NULL (not in the plain mode): the columns are then built on the device and not
  * copied out -- the device-resident throughput of the pipeline. 
- * mode: CCCS_BEAMLET_FAITHFUL, CCCS_BEAMLET_FAST or CCCS_BEAMLET_PLAIN. */
+ * mode: CCCS_BEAMLET_FAITHFUL or CCCS_BEAMLET_PLAIN. */
 int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
                       const float* const* fluence, float threshold, int mode,
                       cccs_sparse_sink sink, void* user);

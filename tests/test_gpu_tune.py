@@ -57,8 +57,7 @@ def test_two_ray_pipeline_gives_the_same_columns():
         out = {}
         for ns in (1, 2, 3, 4):
             eng.set_schedule("bl_streams", ns)
-            out[ns] = [eng.beamlet_dose(beams, fmaps, threshold=1e-4, mode=m)
-                       for m in (abi.CCCS_BEAMLET_FAITHFUL, abi.CCCS_BEAMLET_FAST)]
+            out[ns] = [eng.beamlet_dose(beams, fmaps, threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL)]
     for ns in (2, 3, 4):
         for c1, c2 in zip(out[1], out[ns]):
             assert len(c1) == len(c2) > 10
