@@ -163,6 +163,10 @@ EXPORTS = [
     ("cccs_beam_dose", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.POINTER(_fp), _fp]),
     ("cccs_beam_dose_device", C.c_int, [_P, C.POINTER(Beam), C.c_int, _P, _P]),
     ("cccs_beam_dose_device_noalloc", C.c_int, [_P, C.POINTER(Beam), C.c_int, _P, _P]),
+    ("cccs_angle_cache_create", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("cccs_angle_cache_info", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    ("cccs_ctx_use_angle_cache", C.c_int, [_P, _P]),
+    ("cccs_angle_cache_destroy", C.c_int, [_P]),
     ("cccs_beam_terma", C.c_int, [_P, C.POINTER(Beam), _fp, C.c_int, _fp]),
     ("cccs_beam_geometry_of", C.c_int, [_P, C.POINTER(Beam), _fp, C.POINTER(BeamGeometry)]),
     ("cccs_siddon_paths", C.c_int, [_P, _fp, _fp, C.c_int, C.c_int, C.POINTER(C.c_int32), _fp, _fp,
@@ -176,6 +180,7 @@ EXPORTS = [
     ("cccs_server_submit", C.c_int, [_P, C.c_int, C.POINTER(Beam), _fp, C.c_float, C.c_int, C.POINTER(C.c_uint64)]),
     ("cccs_server_wait", C.c_int, [_P, C.c_uint64, _fp, SPARSE_SINK, _P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     ("cccs_server_stats", C.c_int, [_P, C.POINTER(ServerInfo)]),
+    ("cccs_server_preload_angles", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.c_uint64, C.POINTER(C.c_int)]),
     ("cccs_server_destroy", C.c_int, [_P]),
     ("cccs_load_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_int, C.POINTER(C.c_int)]),
     ("cccs_load_omni_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_char_p, C.c_int,

@@ -20,7 +20,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <limits>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -72,6 +75,12 @@ struct TermaArgs {
     uint32_t* counts;         // per-block Siddon step counts (profiling) or nullptr
 };
 
+// Centre of calc_bbox voxel (X, Y, Z).
+CCCS_HD v3 terma_point(const TermaArgs& a, int X, int Y, int Z) {
+    return mk3(a.vd.start.x + ((float)a.ox + X) * a.vd.voxel.x, a.vd.start.y + ((float)a.oy + Y) * a.vd.voxel.y,
+               a.vd.start.z + ((float)a.oz + Z) * a.vd.voxel.z);
+}
+
 // One lane per calc_bbox voxel (x fastest => coalesced store).  Each lane marches its own ray
 // (the rpl-only march: same crossings and segments as cudaSiddon, fetch coordinates formed with
 // reciprocal voxel sizes); neighbouring lanes follow near-parallel rays, so the density taps stay
@@ -101,14 +110,39 @@ __global__ void __launch_bounds__(256) terma_kernel(TermaArgs a) {
         }
         acc /= (float)nsub;
     } else {
-        v3 p = mk3(a.vd.start.x + ((float)a.ox + X) * a.vd.voxel.x, a.vd.start.y + ((float)a.oy + Y) * a.vd.voxel.y,
-                   a.vd.start.z + ((float)a.oz + Z) * a.vd.voxel.z);
+        // fluence times the unit-fluence TERMA (the Siddon march only inside the field), the order
+        // in which a resident unit-TERMA grid of the same source gives the same value
+        const v3 p = terma_point(a, X, Y, Z);
+        const float fl = point_fluence(p, a.fmap, a.beam, a.beamletnum);
         int it = 0;
-        acc = point_terma_rpl(p, a.fmap, a.beam, a.spec, a.vd, ivox, a.dens, a.bhc, a.beamletnum, &it);
+        acc = fl > 0.f ? fl * unit_point_terma(p, a.beam, a.spec, a.vd, ivox, a.dens, a.bhc, &it) : 0.f;
         steps = (uint32_t)it;
     }
     if (in) a.terma[(size_t)X + (size_t)a.bx * ((size_t)Y + (size_t)a.by * Z)] = acc;
     if (a.counts) block_count(steps, a.counts, blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z));
+}
+
+// Unit-fluence TERMA of every calc_bbox voxel (the march runs for every voxel): one beam source's
+// entry of the angle cache.
+__global__ void __launch_bounds__(256) unit_terma_kernel(TermaArgs a, float* __restrict__ unit) {
+    const int X = blockIdx.x * 64 + threadIdx.x;
+    const int Y = blockIdx.y * 4 + threadIdx.y;
+    const int Z = blockIdx.z;
+    if (X >= a.bx || Y >= a.by || Z >= a.bz) return;
+    const v3 ivox = mk3(1.f / a.vd.voxel.x, 1.f / a.vd.voxel.y, 1.f / a.vd.voxel.z);
+    unit[(size_t)X + (size_t)a.bx * ((size_t)Y + (size_t)a.by * Z)] =
+        unit_point_terma(terma_point(a, X, Y, Z), a.beam, a.spec, a.vd, ivox, a.dens, a.bhc, nullptr);
+}
+
+// TERMA from a resident unit-TERMA grid of the beam's source: the voxel's fluence times its entry.
+__global__ void __launch_bounds__(256) terma_from_unit_kernel(TermaArgs a, const float* __restrict__ unit) {
+    const int X = blockIdx.x * 64 + threadIdx.x;
+    const int Y = blockIdx.y * 4 + threadIdx.y;
+    const int Z = blockIdx.z;
+    if (X >= a.bx || Y >= a.by || Z >= a.bz) return;
+    const size_t v = (size_t)X + (size_t)a.bx * ((size_t)Y + (size_t)a.by * Z);
+    const float fl = point_fluence(terma_point(a, X, Y, Z), a.fmap, a.beam, a.beamletnum);
+    a.terma[v] = fl > 0.f ? fl * unit[v] : 0.f;
 }
 
 struct PathArgs {
@@ -882,6 +916,19 @@ int launch_terma(cccs_ctx* ctx, const TermaArgs& a0) {
     return CCCS_OK;
 }
 
+const float* angle_cache_lookup(const cccs_ctx* ctx, const cccs_beam& b);
+
+// TERMA from the resident unit-TERMA grid of the beam's source (angle cache).
+int launch_terma_from_unit(cccs_ctx* ctx, const TermaArgs& a, const float* unit) {
+    dim3 blk(64, 4, 1), grd((a.bx + 63) / 64, (a.by + 3) / 4, a.bz);
+    const double nbb = (double)a.bx * a.by * a.bz;
+    int tok = prof_begin(ctx, K_TERMA_CACHED);
+    hipLaunchKernelGGL(terma_from_unit_kernel, grd, blk, 0, ctx->stream, a, unit);
+    prof_end(ctx, tok, 8.0 * nbb, 0.0, (uint64_t)nbb);
+    CCCS_HIP(hipGetLastError());
+    return CCCS_OK;
+}
+
 // Crop box + REV grids (nvbbRayConvolve_host.cu:429-587), from the reduced extent.
 int beam_geometry(cccs_ctx* ctx, const cccs_beam& b, const float* red, BeamGeom& g) {
     const cccs_problem& p = ctx->prob;
@@ -1193,7 +1240,8 @@ int beam_dose_scratch(BeamScratch& S, const cccs_beam& b, const float* d_fmap, f
     float* d_terma = nullptr;
     CCCS_TRY(S.get(ctx->terma, bbox_voxels(ctx), &d_terma));
     TermaArgs ta = terma_args(ctx, b, d_fmap, d_terma, -1);
-    CCCS_TRY(launch_terma(ctx, ta));
+    if (const float* unit = angle_cache_lookup(ctx, b)) CCCS_TRY(launch_terma_from_unit(ctx, ta, unit));
+    else CCCS_TRY(launch_terma(ctx, ta));
     return run_cones(S, b, d_terma, d_dose, nullptr);
 }
 
@@ -1203,6 +1251,56 @@ int beam_dose_one(cccs_ctx* ctx, const cccs_beam& b, const float* d_fmap, float*
     CCCS_TRY(context_scratch(ctx, S));
     return beam_dose_scratch(S, b, d_fmap, d_dose);
 }
+
+}  // namespace cccs
+
+// ---- angle cache: resident unit TERMA per beam source -------------------------------------------
+// The unit-fluence TERMA of a full-beam dose depends on the beam's source and SAD (the Siddon
+// march and the inverse square) and on the problem (density, spectrum, grids), not on the fluence
+// map or the collimator.  An optimiser asks for beams from a fixed candidate set, so each
+// candidate source's unit TERMA over the calc_bbox can stay in HBM; a request's TERMA is then the
+// voxel's fluence times that entry, bit-identical to the march (the direct path multiplies in the
+// same order).
+struct cccs_angle_cache {
+    int device = 0;
+    uint32_t size[3] = {0, 0, 0}, bbox_start[3] = {0, 0, 0}, bbox_size[3] = {0, 0, 0};
+    float voxel[3] = {0, 0, 0}, start[3] = {0, 0, 0};
+    float bhc = 1.f;
+    uint64_t dens_hash = 0;
+    cccs::SpecDev spec{};
+    size_t nbbox = 0;
+    float* data = nullptr;  // one calc_bbox grid per entry
+    std::map<std::array<uint32_t, 4>, size_t> index;  // (source bits, SAD bits) -> entry
+};
+
+namespace cccs {
+namespace {
+std::array<uint32_t, 4> angle_key(const cccs_beam& b) {
+    std::array<uint32_t, 4> k;
+    std::memcpy(&k[0], &b.source[0], 4);
+    std::memcpy(&k[1], &b.source[1], 4);
+    std::memcpy(&k[2], &b.source[2], 4);
+    std::memcpy(&k[3], &b.sad, 4);
+    return k;
+}
+
+bool cache_fits(const cccs_ctx* ctx, const cccs_angle_cache* c) {
+    const cccs_problem& p = ctx->prob;
+    for (int k = 0; k < 3; k++)
+        if (c->size[k] != p.size[k] || c->bbox_start[k] != p.bbox_start[k] || c->bbox_size[k] != p.bbox_size[k] ||
+            c->voxel[k] != p.voxel[k] || c->start[k] != p.start[k])
+            return false;
+    if (c->bhc != p.beamhard_correct || c->dens_hash != ctx->dens_hash || c->device != ctx->device) return false;
+    return std::memcmp(&c->spec, &ctx->spec, sizeof(SpecDev)) == 0;
+}
+
+const float* angle_cache_lookup(const cccs_ctx* ctx, const cccs_beam& b) {
+    const cccs_angle_cache* c = ctx->angle_cache;
+    if (!c || ctx->prob.ss_factor > 1) return nullptr;
+    auto it = c->index.find(angle_key(b));
+    return it == c->index.end() ? nullptr : c->data + it->second * c->nbbox;
+}
+}  // namespace
 
 // ---- fixed scratch of the no-allocation entry point ----------------------------------------------
 // The full-beam path's device scratch (TERMA, interleaved density/TERMA, line table, scan partials,
@@ -1236,6 +1334,93 @@ using namespace cccs;
 
 // =================================================================================================
 // C-ABI
+
+extern "C" int cccs_angle_cache_create(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, uint64_t max_bytes,
+                                      cccs_angle_cache** out) {
+    clear_error();
+    if (!ctx || !out || (!beams && nbeams > 0) || nbeams < 0) return set_error(CCCS_EINVAL, "null argument");
+    *out = nullptr;
+    if (ctx->prob.ss_factor > 1) return set_error(CCCS_EINVAL, "the angle cache needs ss_factor 1");
+    CCCS_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<cccs_angle_cache> c(new cccs_angle_cache());
+    const cccs_problem& p = ctx->prob;
+    for (int k = 0; k < 3; k++) {
+        c->size[k] = p.size[k];
+        c->bbox_start[k] = p.bbox_start[k];
+        c->bbox_size[k] = p.bbox_size[k];
+        c->voxel[k] = p.voxel[k];
+        c->start[k] = p.start[k];
+    }
+    c->bhc = p.beamhard_correct;
+    c->dens_hash = ctx->dens_hash;
+    c->spec = ctx->spec;
+    c->device = ctx->device;
+    c->nbbox = bbox_voxels(ctx);
+    const uint64_t per = sizeof(float) * (uint64_t)c->nbbox;
+    std::vector<int> keep;
+    for (int i = 0; i < nbeams; i++) {
+        // only the source and SAD enter the unit TERMA; the fluence-map fields are not used
+        const cccs_beam& b = beams[i];
+        if (!(b.sad > 0.f) || !std::isfinite(b.source[0]) || !std::isfinite(b.source[1]) || !std::isfinite(b.source[2]))
+            return set_error(CCCS_EINVAL, "angle cache: beam " + std::to_string(i) + " needs a finite source and sad > 0");
+        const auto k = angle_key(beams[i]);
+        if (c->index.count(k)) continue;
+        if ((uint64_t)(keep.size() + 1) * per > max_bytes) break;
+        c->index[k] = keep.size();
+        keep.push_back(i);
+    }
+    if (!keep.empty()) {
+        if (hipMalloc(&c->data, per * keep.size()) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(CCCS_ENOMEM, "angle cache of " + std::to_string(per * keep.size()) + " bytes");
+        }
+        for (size_t e = 0; e < keep.size(); e++) {
+            TermaArgs a = terma_args(ctx, beams[keep[e]], nullptr, nullptr, -1);
+            dim3 blk(64, 4, 1), grd((a.bx + 63) / 64, (a.by + 3) / 4, a.bz);
+            hipLaunchKernelGGL(unit_terma_kernel, grd, blk, 0, ctx->stream, a, c->data + e * c->nbbox);
+            const hipError_t err = hipGetLastError();
+            if (err != hipSuccess) {
+                (void)hipStreamSynchronize(ctx->stream);
+                (void)hipFree(c->data);
+                return set_error(CCCS_EDEVICE, std::string("unit TERMA kernel: ") + hipGetErrorString(err));
+            }
+        }
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            (void)hipFree(c->data);
+            return set_error(CCCS_EDEVICE, "angle cache build");
+        }
+    }
+    *out = c.release();
+    return CCCS_OK;
+}
+
+extern "C" int cccs_angle_cache_info(const cccs_angle_cache* cache, int* nangles, uint64_t* bytes) {
+    clear_error();
+    if (!cache) return set_error(CCCS_EINVAL, "null argument");
+    if (nangles) *nangles = (int)cache->index.size();
+    if (bytes) *bytes = sizeof(float) * (uint64_t)cache->nbbox * cache->index.size();
+    return CCCS_OK;
+}
+
+extern "C" int cccs_ctx_use_angle_cache(cccs_ctx* ctx, const cccs_angle_cache* cache) {
+    clear_error();
+    if (!ctx) return set_error(CCCS_EINVAL, "null argument");
+    if (cache && !cache_fits(ctx, cache))
+        return set_error(CCCS_EINVAL, "angle cache built for another problem, density or device");
+    ctx->angle_cache = cache;
+    return CCCS_OK;
+}
+
+extern "C" int cccs_angle_cache_destroy(cccs_angle_cache* cache) {
+    clear_error();
+    if (!cache) return CCCS_OK;
+    if (cache->data) {
+        CCCS_HIP(hipSetDevice(cache->device));
+        CCCS_HIP(hipFree(cache->data));
+    }
+    delete cache;
+    return CCCS_OK;
+}
 
 extern "C" int cccs_beam_dose_device_noalloc(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
                                              const float* d_fluence, float* d_dose) {

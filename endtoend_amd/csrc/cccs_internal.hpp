@@ -326,6 +326,15 @@ CCCS_HD float inv_square(v3 p, const BeamDev& b) {
     return (b.sad * b.sad) / (d * d);
 }
 
+// TERMA of one point for unit fluence: sum_e phi_e E_e mu_e exp(-mu_e rpl) * bhc * SAD^2/|p - src|^2
+// (cudaSiddon.cu:190-204 with the fluence factor taken out).  The full-beam path multiplies it by the
+// point's fluence, so a resident copy per beam source (the angle cache) gives its TERMA bit for bit.
+CCCS_HD float unit_point_terma(v3 p, const BeamDev& b, const SpecDev& sp, const VolDesc& vd, v3 ivoxel,
+                               const float* __restrict__ dens, float bhc, int* iters) {
+    const float rpl = siddon_rpl(p, vd.start, vd.end, vd.voxel, ivoxel, b.src, dens, vd.nx, vd.ny, vd.nz, iters);
+    return unit_terma(rpl, sp) * bhc * inv_square(p, b);
+}
+
 // TERMA of one point (cudaSiddon_device, cudaSiddon.cu:117-208).  Returns 0 outside the field
 // or for a zero-weight beamlet; beamletnum >= 0 keeps only that beamlet's footprint.
 CCCS_HD float point_terma(v3 p, const float* __restrict__ fmap, const BeamDev& b, const SpecDev& sp,

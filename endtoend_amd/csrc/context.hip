@@ -66,7 +66,7 @@ void HostBuf::release() {
 
 const char* const kKernelNames[K_COUNT] = {"terma", "crop", "cone", "pvcs", "bl_sample", "bl_siddon",
                                             "bl_walk", "bl_gather", "bl_unpack", "bl_compact", "plain_ray",
-                                            "plain_conv"};
+                                            "plain_conv", "terma_cached"};
 
 static int prof_new_chunk(Profiler& P) {
     uint32_t* p = nullptr;
@@ -299,6 +299,10 @@ extern "C" int cccs_ctx_create(const cccs_problem* p, int device, cccs_ctx** out
         if (hipMalloc((void**)&c->d_tables, tab.size() * sizeof(float)) != hipSuccess) { st = set_error(CCCS_ENOMEM, "tables"); break; }
         if (hipMemcpy(c->d_tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { st = set_error(CCCS_EDEVICE, "tables copy"); break; }
         c->resident = nv * sizeof(float) + tab.size() * sizeof(float);
+        uint64_t h = 1469598103934665603ull;
+        const uint32_t* bits = reinterpret_cast<const uint32_t*>(p->density);
+        for (size_t i = 0; i < nv; i++) h = (h ^ bits[i]) * 1099511628211ull;
+        c->dens_hash = h;
         // the no-allocation entry point's fixed scratch: its address on this device, resolved once
         if ((st = noalloc_scratch_init(device)) != CCCS_OK) break;
     } while (false);

@@ -61,7 +61,7 @@ struct BeamGeom {
 
 // ---- profiling ---------------------------------------------------------------------------------
 enum KernelKind { K_TERMA = 0, K_CROP, K_CONE, K_PVCS, K_BL_SAMPLE, K_BL_SIDDON, K_BL_WALK, K_BL_GATHER,
-                  K_BL_UNPACK, K_BL_COMPACT, K_PL_RAY, K_PL_CONV, K_COUNT };
+                  K_BL_UNPACK, K_BL_COMPACT, K_PL_RAY, K_PL_CONV, K_TERMA_CACHED, K_COUNT };
 extern const char* const kKernelNames[K_COUNT];
 
 struct ProfRec {
@@ -122,6 +122,8 @@ struct Schedule {
 
 struct cccs_ctx {
     int device = 0;
+    uint64_t dens_hash = 0;  // FNV-1a of the density values (matches an angle cache to its problem)
+    const struct cccs_angle_cache* angle_cache = nullptr;  // resident unit TERMA per beam source
     hipStream_t stream = nullptr;
     // extra streams of the beamlet path's multi-direction pipeline, and its events (one per set
     // plus a start event; created with the context, default priority like the first)

@@ -66,6 +66,8 @@ struct cccs_server {
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Request>> q_dose, q_bl;  // queued full-beam and beamlet requests
     int bl_running = 0;                                // slots running a beamlet batch
+    int running = 0;                                   // slots running any batch
+    cccs_angle_cache* angles = nullptr;                // resident unit TERMA shared by every slot
     std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
     uint64_t next_id = 1;
     uint64_t budget = 0, in_flight = 0;
@@ -223,12 +225,14 @@ void worker(cccs_server* s, int slot) {
                 q->pop_front();
             }
             s->in_flight += bytes;
+            s->running++;
             if (q == &s->q_bl) s->bl_running++;
         }
         run_batch(s, ctx, sb, batch);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->in_flight -= bytes;
+            s->running--;
             if (batch.front()->kind == CCCS_REQ_BEAMLET) s->bl_running--;
             const auto now = Clock::now();
             for (auto& r : batch) {
@@ -333,6 +337,37 @@ extern "C" int cccs_server_wait(cccs_server* s, uint64_t id, float* dose, cccs_s
     return CCCS_OK;
 }
 
+extern "C" int cccs_server_preload_angles(cccs_server* s, const cccs_beam* beams, int nbeams, uint64_t max_bytes,
+                                          int* nangles) {
+    clear_error();
+    if (!s || (!beams && nbeams > 0) || nbeams < 0) return set_error(CCCS_EINVAL, "bad argument");
+    // the lock is held while the cache is built: submissions wait, and no slot runs meanwhile
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return set_error(CCCS_EBUSY, "server is shutting down");
+    if (s->running > 0 || !s->q_dose.empty() || !s->q_bl.empty())
+        return set_error(CCCS_EBUSY, "requests in flight; preload before submitting or after they complete");
+    for (cccs_ctx* c : s->slots) cccs_ctx_use_angle_cache(c, nullptr);
+    cccs_angle_cache_destroy(s->angles);
+    s->angles = nullptr;
+    if (nangles) *nangles = 0;
+    if (nbeams == 0) return CCCS_OK;
+    cccs_angle_cache* c = nullptr;
+    int st = cccs_angle_cache_create(s->slots[0], beams, nbeams, max_bytes, &c);
+    if (st != CCCS_OK) return st;
+    for (cccs_ctx* x : s->slots) {
+        st = cccs_ctx_use_angle_cache(x, c);
+        if (st != CCCS_OK) {
+            const std::string e = cccs_last_error();
+            for (cccs_ctx* y : s->slots) cccs_ctx_use_angle_cache(y, nullptr);
+            cccs_angle_cache_destroy(c);
+            return set_error(st, e);
+        }
+    }
+    s->angles = c;
+    if (nangles) cccs_angle_cache_info(c, nangles, nullptr);
+    return CCCS_OK;
+}
+
 extern "C" int cccs_server_stats(cccs_server* s, cccs_server_info* out) {
     clear_error();
     if (!s || !out) return set_error(CCCS_EINVAL, "null argument");
@@ -366,6 +401,7 @@ extern "C" int cccs_server_destroy(cccs_server* s) {
     s->cv_work.notify_all();
     for (std::thread& t : s->workers) t.join();
     for (cccs_ctx* c : s->slots) cccs_ctx_destroy(c);
+    cccs_angle_cache_destroy(s->angles);
     delete s;
     return CCCS_OK;
 }

@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import os
-from typing import Sequence
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -165,6 +165,24 @@ def beams_array(beams: Sequence[abi.Beam]):
     return arr
 
 
+class AngleCache:
+    """An angle cache (cccs_angle_cache_*): unit-fluence TERMA per beam source, resident in HBM."""
+
+    def __init__(self, lib, h):
+        self.lib, self.h = lib, h
+
+    def info(self):
+        """(distinct sources cached, device bytes)."""
+        n, b = C.c_int(0), C.c_uint64(0)
+        abi.check(self.lib.cccs_angle_cache_info(self.h, C.byref(n), C.byref(b)), "cccs_angle_cache_info")
+        return n.value, b.value
+
+    def close(self):
+        if self.h:
+            abi.check(self.lib.cccs_angle_cache_destroy(self.h), "cccs_angle_cache_destroy")
+            self.h = None
+
+
 class Engine:
     """One resident cccs_ctx on a HIP device."""
 
@@ -180,6 +198,7 @@ class Engine:
         if self.ctx:
             self.lib.cccs_ctx_destroy(self.ctx)
             self.ctx = None
+            self._angle_cache = None
 
     def __enter__(self):
         return self
@@ -262,6 +281,19 @@ class Engine:
     def set_schedule(self, key: str, value: int):
         """Launch-schedule knob (cccs_ctx_set_schedule); e.g. "cone_ramp_min"."""
         abi.check(self.lib.cccs_ctx_set_schedule(self.ctx, key.encode(), int(value)), "cccs_ctx_set_schedule")
+
+    def angle_cache(self, beams: Sequence[abi.Beam], max_bytes: int = 64 << 30) -> "AngleCache":
+        """Resident unit-TERMA grids of the distinct beam sources in `beams` (cccs_angle_cache_create)."""
+        h = C.c_void_p()
+        abi.check(self.lib.cccs_angle_cache_create(self.ctx, beams_array(beams), len(beams), max_bytes, C.byref(h)),
+                  "cccs_angle_cache_create")
+        return AngleCache(self.lib, h)
+
+    def use_angle_cache(self, cache: Optional["AngleCache"]):
+        """Full-beam doses of this context take the TERMA of cached sources from `cache` (None: off)."""
+        abi.check(self.lib.cccs_ctx_use_angle_cache(self.ctx, cache.h if cache is not None else None),
+                  "cccs_ctx_use_angle_cache")
+        self._angle_cache = cache  # kept alive while the context uses it
 
     def beam_dose(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray]) -> np.ndarray:
         dose = np.zeros(self.spec.density.shape, np.float32)
@@ -550,6 +582,14 @@ class Server:
         cb = abi.SPARSE_SINK(sink_fn)
         abi.check(self.lib.cccs_server_wait(self.h, rid, None, cb, None, C.byref(lat), C.byref(bs)), "cccs_server_wait")
         return cols, lat.value, bs.value
+
+    def preload_angles(self, beams: Sequence[abi.Beam], max_bytes: int = 64 << 30) -> int:
+        """Angle cache of `beams` shared by every slot (cccs_server_preload_angles); returns the number
+        of distinct sources cached."""
+        n = C.c_int(0)
+        abi.check(self.lib.cccs_server_preload_angles(self.h, beams_array(beams), len(beams), max_bytes, C.byref(n)),
+                  "cccs_server_preload_angles")
+        return n.value
 
     def stats(self) -> dict:
         s = abi.ServerInfo()

@@ -198,6 +198,24 @@ int cccs_beam_dose_device(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
 int cccs_beam_dose_device_noalloc(cccs_ctx* ctx, const cccs_beam* beams, int nbeams,
                                   const float* d_fluence, float* d_dose);
 
+/* ---- angle cache (no counterpart in the reference; for serving loops and optimisers that ask
+ * for beams from a fixed candidate set) ---------------------------------------------------------
+ * The unit-fluence TERMA of a full-beam dose depends on the beam's source and SAD and on the
+ * problem, not on its fluence map or collimator.  A cache holds, per distinct (source, SAD) of
+ * `beams`, that TERMA over the calc_bbox in device memory (4 * bbox voxels bytes each; entries are
+ * added in list order while they fit in max_bytes).  A context using a cache takes a request's
+ * TERMA as the voxel's fluence times the cached entry, bit-identical to the march it replaces;
+ * beams with another source, and problems with ss_factor > 1, run the march.  The cache is
+ * read-only after creation and may be shared by contexts of the same problem, density and device
+ * (cccs_ctx_use_angle_cache rejects another); destroy it only after every context using it has
+ * finished its work or dropped it (use NULL). */
+typedef struct cccs_angle_cache cccs_angle_cache;
+int cccs_angle_cache_create(cccs_ctx* ctx, const cccs_beam* beams, int nbeams, uint64_t max_bytes,
+                            cccs_angle_cache** out);
+int cccs_angle_cache_info(const cccs_angle_cache* cache, int* nangles, uint64_t* bytes);
+int cccs_ctx_use_angle_cache(cccs_ctx* ctx, const cccs_angle_cache* cache);
+int cccs_angle_cache_destroy(cccs_angle_cache* cache);
+
 /* Stage outputs of the beam path, for stage-wise parity checks. */
 int cccs_beam_terma(cccs_ctx* ctx, const cccs_beam* beam, const float* fluence, int beamletnum,
                     float* terma_bbox);
@@ -283,6 +301,12 @@ int cccs_server_submit(cccs_server* server, int kind, const cccs_beam* beam, con
 int cccs_server_wait(cccs_server* server, uint64_t request_id, float* dose, cccs_sparse_sink sink,
                      void* user, double* latency_ms, int* batch_size);
 int cccs_server_stats(cccs_server* server, cccs_server_info* out);
+/* Builds an angle cache of `beams` (see cccs_angle_cache_create) shared by every slot, replacing
+ * any earlier one (nbeams 0 drops it).  Only while no request is queued or running (CCCS_EBUSY
+ * otherwise); submissions wait until it returns.  The cache is device memory outside hbm_budget,
+ * resident until the next preload or cccs_server_destroy.  No counterpart in the reference. */
+int cccs_server_preload_angles(cccs_server* server, const cccs_beam* beams, int nbeams, uint64_t max_bytes,
+                               int* nangles);
 /* Finishes every queued request, then releases the slots. */
 int cccs_server_destroy(cccs_server* server);
 
