@@ -267,20 +267,94 @@ def full_4pi(stride: int = 12, rate: float = 20.0, requests: bool = True):
     return out
 
 
-def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
+def serving_breakdown(nreq: int = 24, cached: bool = True):
+    """Where a serving request's time goes: `nreq` requests of the serving stream (REV grids within
+    the limit) on one context, fluence already in HBM, with the angle cache of their sources (or
+    without); per-kernel ms per request from the library's event timers, the device time per
+    request, and the host-inclusive time of cccs_beam_dose (dose volume back in host memory)."""
+    from endtoend_amd import workloads as W
+    from endtoend_amd.engine import Engine
+    w = W.cfg4(nreq=300)
+    with Engine(w.spec) as eng:
+        sel = []
+        for i, (b, f) in enumerate(zip(w.beams, w.fmaps)):
+            try:
+                eng.geometry(b, f)
+            except Exception:   # REV grid beyond the limit (the reference's ArraySizeError)
+                continue
+            sel.append(i)
+            if len(sel) == nreq:
+                break
+        cache = eng.angle_cache([w.beams[i] for i in sel]) if cached else None
+        if cache is not None:
+            eng.use_angle_cache(cache)
+        nf = max(int(w.fmaps[i].size) for i in sel)
+        dfl = eng.dev_alloc(4 * nf)
+        dd = eng.dev_alloc(4 * int(w.spec.density.size))
+        for i in sel[:3]:
+            eng.beam_dose([w.beams[i]], [w.fmaps[i]])
+        dev = []
+        for i in sel:
+            eng.dev_upload(dfl, np.ascontiguousarray(w.fmaps[i], np.float32))
+            eng.sync()
+            t0 = time.perf_counter()
+            eng.beam_dose_device([w.beams[i]], dfl, dd)
+            eng.sync()
+            dev.append(1e3 * (time.perf_counter() - t0))
+        eng.profile(True)
+        eng.profile_reset()
+        for i in sel:
+            eng.dev_upload(dfl, np.ascontiguousarray(w.fmaps[i], np.float32))
+            eng.beam_dose_device([w.beams[i]], dfl, dd)
+        eng.sync()
+        prof = eng.profile_read()
+        eng.profile(False)
+        host = []
+        for i in sel:
+            t0 = time.perf_counter()
+            eng.beam_dose([w.beams[i]], [w.fmaps[i]])
+            host.append(1e3 * (time.perf_counter() - t0))
+        eng.dev_free(dfl)
+        eng.dev_free(dd)
+        eng.use_angle_cache(None)
+        if cache is not None:
+            cache.close()
+    return {"requests": len(sel), "angle_cache": cached,
+            "kernel_ms_per_request": {k: v["ms"] / len(sel) for k, v in prof.items() if v["launches"]},
+            "device_ms": {"p50": _percentile(dev, 50), "max": max(dev)},
+            "host_inclusive_ms": {"p50": _percentile(host, 50), "max": max(host)}}
+
+
+def serve_profile(nreq: int):
+    """Profiling runs: `nreq` serving requests on one context with the angle cache, fluence in HBM."""
+    print(json.dumps(serving_breakdown(nreq=nreq, cached=True)))
+
+
+def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0, preload: bool = True):
     """Annealing-style request stream through the serving loop: Poisson arrivals of single-beam
     dose requests on the 256x256x180 thorax phantom (beams from the 1162-beam 4pi list, 20^2-64^2
-    fluence maps).  Latency is submit-to-completion (result in host memory), measured inside the
-    server.  Reported: the closed-loop capacity, one request at a time (the optimiser's sequential
-    proposals), the stream at the offered rate, and the same stream at 80 % of the measured
-    capacity (a rate the device sustains)."""
+    fluence maps).  With `preload` the server first keeps the unit TERMA of all 1162 candidate
+    sources resident (angle cache).  Latency is submit-to-completion (result in host memory),
+    measured inside the server.  Reported: the closed-loop capacity; one request at a time (the
+    optimiser's sequential proposals); the stream at the offered rate; the same stream at 80 % of
+    capacity; and a sweep of rates for the highest one whose p99 stays under 50 ms."""
     import threading
 
     from endtoend_amd import abi
+    from endtoend_amd import engine as E
     from endtoend_amd import workloads as W
     from endtoend_amd.engine import Server
     w = W.cfg4(nreq=nreq, rate=rate)
+    out = {"workload": "256x256x180 thorax @ 2 mm, single-beam dose requests, beams from the 1162-beam 4pi list, "
+                       "20^2-64^2 x 5 mm U[0,1] fluence, Poisson arrivals",
+           "slots": slots, "seed": w.seed}
     with Server(w.spec, slots=slots) as srv:
+        if preload:
+            cands = E.load_beam_list(os.path.join(W.DATA, "beamlist_4pi.txt"))
+            t0 = time.perf_counter()
+            n = srv.preload_angles(cands, max_bytes=96 << 30)
+            out["angle_cache"] = {"sources": n, "GB": 4.0 * n * w.spec.density.size / 1e9,
+                                  "build_s": time.perf_counter() - t0}
         for i in range(2 * slots):                       # warm every slot
             srv.wait(srv.submit(w.beams[i], w.fmaps[i]))
         # capacity: requests submitted at once (closed loop, the queue never empties)
@@ -315,7 +389,7 @@ def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
                     ready.acquire()
                     try:
                         _, l, b = srv.wait(ids[k])
-                    except abi.CCCSError as e:   # e.g. a REV grid beyond the limit (ArraySizeError)
+                    except abi.CCCSError as e:   # a REV grid beyond the limit (the reference's ArraySizeError)
                         failed.append(e.code)
                         continue
                     lat.append(l)
@@ -328,7 +402,7 @@ def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
                 dt = arrivals[i] - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(dt)
-                ids.append(srv.submit(w.beams[i], w.fmaps[i]))
+                ids.append(srv.submit(w.beams[i % nreq], w.fmaps[i % nreq]))
                 ready.release()
             th.join()
             span = time.perf_counter() - t0
@@ -341,11 +415,19 @@ def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0):
         sustain = 0.8 * capacity
         n2 = min(nreq, max(60, int(sustain * 8)))
         sus = stream(w.arrivals[:n2] * (rate / sustain))
-    return {"workload": "256x256x180 thorax @ 2 mm, single-beam dose requests, beams from the 1162-beam 4pi list, "
-                        f"20^2-64^2 x 5 mm U[0,1] fluence, Poisson arrivals",
-            "slots": slots, "capacity_per_s": capacity, "sequential": sequential, "at_offered_rate": offered,
-            "at_80pct_capacity": sus,
-            "seed": w.seed}
+        # rate sweep: the same arrival pattern scaled to each rate, about 6 s of arrivals each
+        sweep, best = [], None
+        for frac in (0.5, 0.65, 0.8, 0.9, 1.0):
+            r = frac * capacity
+            n3 = min(nreq, max(60, int(r * 6)))
+            res = stream(w.arrivals[:n3] * (rate / r))
+            sweep.append({"rate": res["offered_rate"], "p50_ms": res["latency_ms"]["p50"],
+                          "p99_ms": res["latency_ms"]["p99"], "requests": n3})
+            if res["latency_ms"]["p99"] < 50.0:
+                best = res["offered_rate"] if best is None else max(best, res["offered_rate"])
+    out.update({"capacity_per_s": capacity, "sequential": sequential, "at_offered_rate": offered,
+                "at_80pct_capacity": sus, "rate_sweep": sweep, "max_rate_p99_under_50ms": best})
+    return out
 
 
 def copy_calibration(mib: int, reps: int = 5):
@@ -379,6 +461,8 @@ def main():
     ap.add_argument("--plain-beam-only", action="store_true",
                     help="only the plain full-beam path on the thorax plan (profiling runs)")
     ap.add_argument("--serving-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--serve-profile", type=int, default=0,
+                    help="only this many serving requests on one context with the angle cache (profiling runs)")
     ap.add_argument("--full-4pi-only", action="store_true",
                     help="only the 4pi precompute sample with concurrent requests (profiling runs)")
     ap.add_argument("--full-4pi-stride", type=int, default=12,
@@ -404,6 +488,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps(serving()))
+        return
+    if args.serve_profile > 0:
+        serve_profile(args.serve_profile)
         return
     if args.full_4pi_only:
         print(json.dumps(full_4pi(stride=args.full_4pi_stride, requests=not args.full_4pi_no_requests)))
@@ -523,6 +610,7 @@ def main():
         eng.close()
         out["plain_beam_path"] = plain_beam_path(autotune=args.autotune)
         out["serving"] = serving()
+        out["serving"]["breakdown"] = serving_breakdown()
         out["full_4pi"] = full_4pi()
     else:
         eng.close()

@@ -251,6 +251,7 @@ struct ConeArgs {
     int ramp_min;             // valid segments at least this long use the segment-sum walk
     int maxn;                 // longest stored line of this launch (LDS sizing)
     int len_lo, len_hi;       // this launch takes the lines whose stored length is in [len_lo, len_hi]
+    const uint32_t* list;     // the lines of this launch (one workgroup each), or nullptr: every line
     VolDesc vd;
     m33 binv;                 // PVCS <- BEV rotation (index units)
     v3 off;                   // (bevStart - start) / voxel
@@ -313,17 +314,17 @@ __device__ __forceinline__ void line_interval(const ConeArgs& a, const ConeDev& 
 // the first/last sample of the sum.  Same result as the sequential walk up to FP32 rounding of the
 // accumulated distance.
 struct RampLine {
-    const float* c;       // centres, radiological cm from the segment start; c[n] = +inf
+    const float* c;       // centres, radiological cm from the segment start; c[n..n+3] = +inf
     const double* E;      // E[k]  = sum_{m<k} (T_m - T_{m-1}) c_m   (T_{-1} = 0), k in [0, n]
     const double* E2;     // E2[k] = sum_{m<k} (T_m - T_{m+1}) c_m   (T_n = 0),    k in [0, n]
     const float2* s;      // {half step, TERMA} of the segment's samples
-    const uint16_t* lb;   // lb[u] = first k with c_k >= c0 + u * D, u in [0, U + 1]
+    const uint16_t* lb;   // lb[u] = first k with c_k >= c0 + u * D, u in [0, U + 1] (D = range / U)
     int n, U;
     float c0, cn, invD;
 };
 
 // first k with c_k >= y
-__device__ __forceinline__ int ramp_lb(const RampLine& L, float y) {
+CCCS_HD int ramp_lb(const RampLine& L, float y) {
     if (!(y > L.c0)) return 0;
     if (y > L.cn) return L.n;
     // biased by 1% of a cell so that the cell's threshold never exceeds y after rounding
@@ -342,7 +343,7 @@ __device__ __forceinline__ int ramp_lb(const RampLine& L, float y) {
 }
 
 // first k with c_k > y
-__device__ __forceinline__ int ramp_ub(const RampLine& L, float y) {
+CCCS_HD int ramp_ub(const RampLine& L, float y) {
     if (y < L.c0) return 0;
     if (!(y < L.cn)) return L.n;
     int u = (int)((y - L.c0) * L.invD - 0.01f);
@@ -360,7 +361,7 @@ __device__ __forceinline__ int ramp_ub(const RampLine& L, float y) {
 }
 
 // Backward walk of destination j (toward lower indices), kernel row given as sigma/beta.
-__device__ __forceinline__ double ramp_back(const RampLine& L, int j, const double* __restrict__ sig,
+CCCS_HD double ramp_back(const RampLine& L, int j, const double* __restrict__ sig,
                                             const double* __restrict__ beta, const float* __restrict__ R, int nr,
                                             uint32_t& iters) {
     if (j == 0) return 0.0;
@@ -391,7 +392,7 @@ __device__ __forceinline__ double ramp_back(const RampLine& L, int j, const doub
 }
 
 // Forward walk of destination j (toward higher indices).
-__device__ __forceinline__ double ramp_fwd(const RampLine& L, int j, const double* __restrict__ sig,
+CCCS_HD double ramp_fwd(const RampLine& L, int j, const double* __restrict__ sig,
                                            const double* __restrict__ beta, const float* __restrict__ R, int nr,
                                            uint32_t& iters) {
     if (j >= L.n - 1) return 0.0;
@@ -421,6 +422,17 @@ __device__ __forceinline__ double ramp_fwd(const RampLine& L, int j, const doubl
     return sum;
 }
 
+// First cell u in [0, nu] whose threshold c0 + u * D exceeds v (nu when none does): an estimate
+// from the cell width, corrected against the thresholds themselves (as the table defines them).
+CCCS_HD int ramp_cell_after(float c0, float D, float invD, int nu, float v) {
+    if (!(D > 0.f)) return v < c0 ? 0 : nu;
+    const float g = fminf(fmaxf((v - c0) * invD, -1.f), (float)nu);
+    int u = min(max((int)floorf(g) + 1, 0), nu);
+    while (u > 0 && c0 + (float)(u - 1) * D > v) u--;
+    while (u < nu && !(c0 + (float)u * D > v)) u++;
+    return u;
+}
+
 // Exclusive prefix sum over the 64 lanes of a wave.
 __device__ __forceinline__ double wave_excl_sum(double v) {
     const int lane = threadIdx.x & 63;
@@ -436,8 +448,8 @@ __device__ __forceinline__ double wave_excl_sum(double v) {
 // LDS bytes of the cone kernel for lines of up to maxn samples (with the segment-sum arrays).
 inline size_t cone_lds_bytes(uint32_t maxn, bool ramp) {
     if (!ramp) return sizeof(float2) * maxn;
-    const size_t n1 = (size_t)maxn + 1, nc = ((size_t)maxn + 2 + 1) & ~(size_t)1;
-    return 2 * sizeof(double) * n1 + sizeof(float) * nc + sizeof(float2) * maxn + sizeof(uint16_t) * (maxn + 2);
+    const size_t n1 = (size_t)maxn + 1, nc = ((size_t)maxn + 4 + 1) & ~(size_t)1;
+    return 2 * sizeof(double) * n1 + sizeof(float) * nc + sizeof(float2) * maxn + sizeof(uint16_t) * (2 * maxn + 2);
 }
 
 // Pass 1 of the cone pipeline: the stored interval of every REV line (offsets come from a scan).
@@ -490,7 +502,8 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
     __shared__ double s_tot[NW];
     __shared__ int s_red[NW][3];
     __shared__ uint32_t s_cnt[NW];
-    const uint64_t line = xcd_remap(blockIdx.x, gridDim.x);
+    const uint64_t slot = xcd_remap(blockIdx.x, gridDim.x);
+    const uint64_t line = a.list ? (uint64_t)a.list[slot] : slot;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const LineMeta meta = a.lines[line];
     const bool mine = meta.len >= a.len_lo && meta.len <= a.len_hi;
@@ -501,7 +514,7 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
         const int m = meta.len;
         const bool ramp_lds = a.ramp_min <= a.maxn;
         // LDS: [E | E2 | c | line | lb] with the segment-sum arrays, else [line]
-        const int n1 = a.maxn + 1, nc = (a.maxn + 3) & ~1;
+        const int n1 = a.maxn + 1, nc = (a.maxn + 5) & ~1;
         double* rE = s_dyn;
         double* rE2 = rE + n1;
         float* rc = reinterpret_cast<float*>(rE2 + n1);
@@ -610,10 +623,10 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
             if (ca < cb && cb == n) {
                 rE[n] = e1;
                 rE2[n] = e2;
-                rc[n] = INFINITY;
+                for (int q = 0; q < 4; q++) rc[n + q] = INFINITY;  // lookups probe up to 3 past n
             }
             __syncthreads();
-            // lookup table: U = n cells over [c0, c_{n-1}]
+            // lookup table: lb[u] = first k with c_k >= c0 + u * D, U = 2n cells over [c0, c_{n-1}]
             RampLine L;
             L.c = rc;
             L.E = rE;
@@ -621,26 +634,26 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
             L.s = sv;
             L.lb = rlb;
             L.n = n;
-            L.U = n;
+            L.U = 2 * n;
             L.c0 = rc[0];
             L.cn = rc[n - 1];
             const float range = L.cn - L.c0;
-            const float D = range > 0.f ? range / (float)n : 0.f;
-            L.invD = range > 0.f ? (float)n / range : 0.f;
-            for (int u = tid; u <= n + 1; u += NT) {
-                const float t = L.c0 + (float)u * D;
-                int lo = 0, hi = n;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (rc[mid] < t) lo = mid + 1; else hi = mid;
-                }
-                rlb[u] = (uint16_t)lo;
+            const float D = range > 0.f ? range / (float)L.U : 0.f;
+            L.invD = range > 0.f ? (float)L.U / range : 0.f;
+            // sample k owns the cells whose threshold lies in (c_{k-1}, c_k] (c_{-1} = -inf, c_n =
+            // +inf): each of the U + 2 entries is written by exactly one sample
+            for (int k = tid; k <= n; k += NT) {
+                const int u0 = k == 0 ? 0 : ramp_cell_after(L.c0, D, L.invD, L.U + 2, rc[k - 1]);
+                const int u1 = k == n ? L.U + 2 : ramp_cell_after(L.c0, D, L.invD, L.U + 2, rc[k]);
+                for (int u = u0; u < u1; u++) rlb[u] = (uint16_t)k;
             }
             __syncthreads();
-            for (int j = tid; j < n; j += NT) {
-                const double v = ramp_back(L, j, sSig[0], sBeta[0], sR, nr, iters) +
-                                 ramp_fwd(L, j, sSig[1], sBeta[1], sR, nr, iters);
-                out[vlo + j] = (float)v / nphi_f;
+            {
+                for (int j = tid; j < n; j += NT) {
+                    const double v = ramp_back(L, j, sSig[0], sBeta[0], sR, nr, iters) +
+                                     ramp_fwd(L, j, sSig[1], sBeta[1], sR, nr, iters);
+                    out[vlo + j] = (float)v / nphi_f;
+                }
             }
         } else {
             for (int base = vlo; base <= vhi; base += NT) {
@@ -651,18 +664,19 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
             }
         }
     }
-    // every line's slot is written by exactly one of the length-class launches
-    if (a.counts && (mine || (meta.len == 0 && a.len_lo == 1))) {
+    // every line's slot is written by exactly one of the length-class launches (empty lines' slots
+    // by the class-list pass when the launches take lists)
+    if (a.counts && (mine || (!a.list && meta.len == 0 && a.len_lo == 1))) {
         for (int o = 32; o > 0; o >>= 1) iters += __shfl_xor(iters, o, 64);
         if (NW == 1) {
-            if (tid == 0) a.counts[blockIdx.x] = iters;
+            if (tid == 0) a.counts[line] = iters;
         } else {
             if (lane == 0) s_cnt[wid] = iters;
             __syncthreads();
             if (tid == 0) {
                 uint32_t t = 0;
                 for (int w = 0; w < NW; w++) t += s_cnt[w];
-                a.counts[blockIdx.x] = t;
+                a.counts[line] = t;
             }
         }
     }
@@ -1069,6 +1083,103 @@ int run_crop(BeamScratch& S, const float* d_terma, const cccs_beam& b, float* re
 }
 
 // Everything after TERMA: geometry, cones, PVCS accumulation into d_dose.
+// ---- per-class line lists -----------------------------------------------------------------------
+// The cone kernel runs one launch per stored-length class (LDS sized for the class's longest line).
+// Each launch takes the list of its own lines, in line order, so that no workgroup is spent on an
+// empty line or on a line of another class.  Three passes: per-block class counts, one block that
+// turns them into list offsets (class by class), and a stable scatter of the line indices.
+constexpr int kClassBlock = 1024;
+
+struct ClassArgs {
+    const LineMeta* lines;
+    uint64_t n;
+    int hi[kConeClasses];  // longest stored line of each class (ascending)
+    int ncls;
+    uint32_t* bcnt;        // [blocks][kConeClasses]: counts, then the blocks' list offsets
+    uint32_t* tot;         // [kConeClasses]: lines per class
+    uint32_t* list;        // line indices, class by class
+    uint32_t* counts;      // profiling: the slots of empty lines are zeroed here (or nullptr)
+};
+
+__device__ __forceinline__ int class_of_line(const ClassArgs& a, int len) {
+    if (len <= 0) return -1;
+    for (int c = 0; c < a.ncls - 1; c++)
+        if (len <= a.hi[c]) return c;
+    return a.ncls - 1;
+}
+
+__global__ void __launch_bounds__(kClassBlock) class_count_kernel(ClassArgs a) {
+    __shared__ uint32_t s[kClassBlock / 64][kConeClasses];
+    const uint64_t i = (uint64_t)blockIdx.x * kClassBlock + threadIdx.x;
+    const int cls = i < a.n ? class_of_line(a, a.lines[i].len) : -1;
+    if (a.counts && i < a.n && cls < 0) a.counts[i] = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c = 0; c < kConeClasses; c++) {
+        const uint64_t m = __ballot(cls == c);
+        if (lane == 0) s[w][c] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < kConeClasses) {
+        uint32_t t = 0;
+        for (int q = 0; q < kClassBlock / 64; q++) t += s[q][threadIdx.x];
+        a.bcnt[(size_t)blockIdx.x * kConeClasses + threadIdx.x] = t;
+    }
+}
+
+// One workgroup: exclusive scan of the block counts of each class, the classes one after another.
+__global__ void __launch_bounds__(kClassBlock) class_scan_kernel(ClassArgs a, uint32_t nb) {
+    __shared__ uint32_t s[kClassBlock];
+    const int t = threadIdx.x;
+    uint32_t base = 0;
+    for (int c = 0; c < kConeClasses; c++) {
+        const uint32_t start = base;
+        for (uint32_t b0 = 0; b0 < nb; b0 += kClassBlock) {
+            const uint32_t b = b0 + t;
+            const uint32_t v = b < nb ? a.bcnt[(size_t)b * kConeClasses + c] : 0u;
+            s[t] = v;
+            __syncthreads();
+            for (int off = 1; off < kClassBlock; off <<= 1) {
+                const uint32_t add = t >= off ? s[t - off] : 0u;
+                __syncthreads();
+                s[t] += add;
+                __syncthreads();
+            }
+            if (b < nb) a.bcnt[(size_t)b * kConeClasses + c] = base + s[t] - v;
+            base += s[kClassBlock - 1];
+            __syncthreads();
+        }
+        if (t == 0) a.tot[c] = base - start;
+    }
+}
+
+__global__ void __launch_bounds__(kClassBlock) class_scatter_kernel(ClassArgs a) {
+    __shared__ uint32_t s[kClassBlock / 64][kConeClasses];
+    const uint64_t i = (uint64_t)blockIdx.x * kClassBlock + threadIdx.x;
+    const int cls = i < a.n ? class_of_line(a, a.lines[i].len) : -1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    uint32_t rank = 0;
+    for (int c = 0; c < kConeClasses; c++) {
+        const uint64_t m = __ballot(cls == c);
+        if (lane == 0) s[w][c] = (uint32_t)__popcll(m);
+        if (cls == c) rank = (uint32_t)__popcll(m & below);
+    }
+    __syncthreads();
+    if (cls >= 0) {
+        uint32_t off = a.bcnt[(size_t)blockIdx.x * kConeClasses + cls] + rank;
+        for (int q = 0; q < w; q++) off += s[q][cls];
+        a.list[off] = (uint32_t)i;
+    }
+}
+
+void launch_cone(int threads, dim3 grid, size_t lds, hipStream_t st, const ConeArgs& ca) {
+    switch (threads) {
+        case 256: hipLaunchKernelGGL(cone_kernel<256>, grid, dim3(256), lds, st, ca); break;
+        case 128: hipLaunchKernelGGL(cone_kernel<128>, grid, dim3(128), lds, st, ca); break;
+        default: hipLaunchKernelGGL(cone_kernel<64>, grid, dim3(64), lds, st, ca); break;
+    }
+}
+
 int run_cones(BeamScratch& S, const cccs_beam& b, const float* d_terma, float* d_dose, BeamGeom* gout) {
     cccs_ctx* ctx = S.ctx;
     float red[7];
@@ -1122,6 +1233,7 @@ int run_cones(BeamScratch& S, const cccs_beam& b, const float* d_terma, float* d
     float2* d_dt = nullptr;
     CCCS_TRY(S.get(ctx->dt, nvox, &d_dt));
     ConeArgs ca;
+    ca.list = nullptr;
     ca.dt = d_dt;
     ca.tables = ctx->d_tables;
     ca.cones = d_cones;
@@ -1156,37 +1268,60 @@ int run_cones(BeamScratch& S, const cccs_beam& b, const float* d_terma, float* d
     uint32_t* d_scan = nullptr;
     CCCS_TRY(S.get(ctx->scan, line_scan_words(g.total_lines), &d_scan));
     CCCS_TRY(launch_line_scan(ctx, ca.lines, g.total_lines, d_scan));
+    // stored-length classes: one launch each, with LDS for the class's longest line, so that the
+    // many short lines are not held to the occupancy of the longest
+    uint32_t maxn = 1;
+    for (int r = 0; r < g.nrays; r++) maxn = std::max(maxn, g.rev_size[r][0]);
+    int cls_lo[kConeClasses], cls_hi[kConeClasses], cls_threads[kConeClasses], ncls = 0;
+    for (int ci = 0, lo = 1; ci < kConeClasses && lo <= (int)maxn; ci++) {
+        const int hi = ci == kConeClasses - 1 ? (int)maxn : std::min<int>(ctx->sched.cone_class_len[ci], (int)maxn);
+        if (hi < lo) continue;
+        cls_lo[ncls] = lo;
+        cls_hi[ncls] = hi;
+        cls_threads[ncls] = ctx->sched.cone_class_threads[ci];
+        ncls++;
+        lo = hi + 1;
+    }
+    ca.counts = prof_counter(ctx, g.total_lines);
+    ClassArgs cl;
+    cl.lines = ca.lines;
+    cl.n = g.total_lines;
+    for (int c = 0; c < kConeClasses; c++) cl.hi[c] = c < ncls ? cls_hi[c] : 0;
+    cl.ncls = ncls;
+    cl.counts = ca.counts;
+    const uint64_t ncb = (g.total_lines + kClassBlock - 1) / kClassBlock;
+    CCCS_TRY(S.get(ctx->cls_cnt, (size_t)ncb * kConeClasses + kConeClasses, &cl.bcnt));
+    cl.tot = cl.bcnt + (size_t)ncb * kConeClasses;
+    CCCS_TRY(S.get(ctx->cls_list, g.total_lines, &cl.list));
+    hipLaunchKernelGGL(class_count_kernel, dim3((unsigned)ncb), dim3(kClassBlock), 0, ctx->stream, cl);
+    hipLaunchKernelGGL(class_scan_kernel, dim3(1), dim3(kClassBlock), 0, ctx->stream, cl, (uint32_t)ncb);
+    hipLaunchKernelGGL(class_scatter_kernel, dim3((unsigned)ncb), dim3(kClassBlock), 0, ctx->stream, cl);
+    CCCS_HIP(hipGetLastError());
     CCCS_HIP(hipMemcpyAsync(S.host_small, ca.lines + (g.total_lines - 1), sizeof(LineMeta), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    CCCS_HIP(hipMemcpyAsync(S.host_small + 4, cl.tot, sizeof(uint32_t) * kConeClasses, hipMemcpyDeviceToHost,
                             ctx->stream));
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
     LineMeta last;
     std::memcpy(&last, S.host_small, sizeof(LineMeta));
+    uint32_t cls_n[kConeClasses];
+    std::memcpy(cls_n, S.host_small + 4, sizeof(cls_n));
     const uint64_t stored = (uint64_t)last.off + (uint64_t)last.len;
     CCCS_TRY(S.get(ctx->rev, stored, &ca.rev));
-    // pass 2: per-line resample + walks into the compact store
-    ca.counts = prof_counter(ctx, g.total_lines);
-    uint32_t maxn = 1;
-    for (int r = 0; r < g.nrays; r++) maxn = std::max(maxn, g.rev_size[r][0]);
+    // pass 2: per-line resample + walks into the compact store, one workgroup per listed line
     ca.ramp_min = ctx->sched.cone_ramp_min;
-    // One launch per stored-length class, each with LDS for its longest line, so that the many
-    // short lines are not held to the occupancy of the longest; a workgroup whose line belongs to
-    // another class leaves at once.
     int tok = prof_begin(ctx, K_CONE);
-    for (int ci = 0, lo = 1; ci < kConeClasses && lo <= (int)maxn; ci++) {
-        const int hi = std::min<int>(ctx->sched.cone_class_len[ci], (int)maxn);
-        if (hi < lo) continue;
-        ca.len_lo = lo;
-        ca.len_hi = ci == kConeClasses - 1 ? (int)maxn : hi;
+    uint64_t list_off = 0;
+    for (int c = 0; c < ncls; c++) {
+        if (cls_n[c] == 0) continue;
+        ca.len_lo = cls_lo[c];
+        ca.len_hi = cls_hi[c];
         ca.maxn = ca.len_hi;
+        ca.list = cl.list + list_off;
+        list_off += cls_n[c];
         const size_t lds = cone_lds_bytes((uint32_t)ca.maxn, ca.ramp_min <= ca.maxn);
-        const dim3 grid((unsigned)g.total_lines);
-        switch (ctx->sched.cone_class_threads[ci]) {
-            case 256: hipLaunchKernelGGL(cone_kernel<256>, grid, dim3(256), lds, ctx->stream, ca); break;
-            case 128: hipLaunchKernelGGL(cone_kernel<128>, grid, dim3(128), lds, ctx->stream, ca); break;
-            default: hipLaunchKernelGGL(cone_kernel<64>, grid, dim3(64), lds, ctx->stream, ca); break;
-        }
+        launch_cone(cls_threads[c], dim3(cls_n[c]), lds, ctx->stream, ca);
         CCCS_HIP(hipGetLastError());
-        lo = ca.len_hi + 1;
     }
     // algorithmic traffic: density + TERMA in, cone dose out, per stored sample
     prof_end(ctx, tok, 12.0 * (double)stored, 0.0, stored);

@@ -324,6 +324,7 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     if (c->d_tables) (void)hipFree(c->d_tables);
     c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
     c->dose.release(); c->col.release(); c->lines.release(); c->scan.release(); c->dt.release();
+    c->cls_cnt.release(); c->cls_list.release();
     for (int i = 0; i < kRaySets - 1; i++) {
         c->xlines[i].release(); c->xrev[i].release(); c->xh[i].release(); c->xqueue[i].release(); c->xscan[i].release();
         c->xlcone[i].release();

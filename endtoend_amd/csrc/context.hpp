@@ -139,6 +139,7 @@ struct cccs_ctx {
     float* d_tables = nullptr;  // kernel (ntheta*nradii) | radii | inverse widths | mean radii | inverse widths
     size_t resident = 0;
     cccs::DevBuf terma, fmap, partial, rev, cones, dose, col, lines, scan, dt;
+    cccs::DevBuf cls_cnt, cls_list;  // plain path: per-block class counts/offsets, per-class line lists
     // ray sets 1.. of the beamlet pipeline (set 0 is lines/rev/bl_h/bl_queue/scan/bl_lcone): line
     // table, REV store, half steps, TERMA queue, scan partials, grid of every line
     cccs::DevBuf xlines[cccs::kRaySets - 1], xrev[cccs::kRaySets - 1], xh[cccs::kRaySets - 1],

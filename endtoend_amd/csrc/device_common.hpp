@@ -99,7 +99,7 @@ __device__ __forceinline__ float rev_fetch(const LineMeta* __restrict__ meta, co
 // reaching the last radial interval) become per-lane flags.  The cumulative kernel is linear
 // between two radii, so a lane keeps its current interval in registers and reads the LDS tables
 // only when the radiological depth crosses a radius.
-__device__ __forceinline__ float walk_both(const float2* __restrict__ s, int j, int vlo, int vhi,
+CCCS_HD float walk_both(const float2* __restrict__ s, int j, int vlo, int vhi,
                                            const float* __restrict__ R, const float* __restrict__ IW,
                                            const float* __restrict__ rowF, const float* __restrict__ rowB, int nr,
                                            bool active, uint32_t& iters) {
