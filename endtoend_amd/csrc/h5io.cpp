@@ -20,6 +20,11 @@
 
 #include "cccs_h5.h"
 
+#include <errno.h>
+#include <ftw.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 namespace {
 
 thread_local std::string g_err;
@@ -218,10 +223,62 @@ bool write_beam_attrs(hid_t g, int uid, unsigned nbeamlets, const cccs_beam& b, 
            attr_scalar(g, "beam_specs", ftype, mtype, &r);
 }
 
+// the attributes of one beamlet's group (_write_beamlet_metadata, sparse_io.cpp:209-245)
+bool write_beamlet_attrs(hid_t g, const cccs_sparse_column* c) {
+    const unsigned bu = (unsigned)c->beamlet_index;
+    const uint64_t n = c->nnz;
+    bool ok = attr_scalar(g, "beamlet_uid", H5T_STD_U16LE, H5T_NATIVE_UINT, &bu) &&
+              attr_scalar(g, "N_coeffs", H5T_STD_U64LE, H5T_NATIVE_UINT64, &n) &&
+              attr_scalar(g, "perc_nonzero", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, &c->perc_nonzero) &&
+              attr_scalar(g, "perc_dropped", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, &c->perc_dropped);
+    if (ok && c->nblocks > 0)
+        ok = attr_vector(g, "row_block_sizes", H5T_STD_U64LE, H5T_NATIVE_UINT64, c->nblocks, c->row_block_sizes) &&
+             attr_scalar(g, "perc_reducedrop", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, &c->perc_reducedrop);
+    return ok;
+}
+
+// the lindex / coeffs datasets of one beamlet (_write_beamlet_to_hdf5, sparse_io.cpp:178-207)
+bool write_beamlet_data(hid_t g, const cccs_sparse_column* c, int compress_lvl, uint64_t chunksize) {
+    const uint64_t n = c->nnz;
+    Id plist(H5Pcreate(H5P_DATASET_CREATE), H5Pclose);
+    if (compress_lvl >= 0) {
+        const hsize_t chunk = std::min<hsize_t>(n, chunksize);
+        if (H5Pset_chunk(plist, 1, &chunk) < 0 || H5Pset_deflate(plist, (unsigned)std::min(9, compress_lvl)) < 0)
+            return false;
+    }
+    const hsize_t dims = n;
+    Id s = simple_space(1, &dims);
+    Id k(H5Dcreate2(g, "lindex", H5T_STD_U64LE, s, H5P_DEFAULT, plist, H5P_DEFAULT), H5Dclose);
+    if (!k.ok() || H5Dwrite(k, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, c->keys) < 0) return false;
+    Id v(H5Dcreate2(g, "coeffs", H5T_IEEE_F32LE, s, H5P_DEFAULT, plist, H5P_DEFAULT), H5Dclose);
+    return v.ok() && H5Dwrite(v, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, c->values) >= 0;
+}
+
+int rm_entry(const char* p, const struct stat*, int, struct FTW*) { return ::remove(p); }
+
+// removes the directory tree at `dir` (no symlinks followed); true when nothing is left
+bool remove_tree(const std::string& dir) {
+    struct stat st;
+    if (::lstat(dir.c_str(), &st) != 0) return true;
+    return ::nftw(dir.c_str(), rm_entry, 16, FTW_DEPTH | FTW_PHYS) == 0;
+}
+
+bool make_dir(const std::string& dir) { return ::mkdir(dir.c_str(), 0755) == 0 || errno == EEXIST; }
+
+// last path component, trailing separators ignored (POSIX basename)
+std::string base_name(std::string p) {
+    while (p.size() > 1 && p.back() == '/') p.pop_back();
+    const size_t k = p.find_last_of('/');
+    return k == std::string::npos ? p : p.substr(k + 1);
+}
+
 }  // namespace
 
 struct cccs_h5_coeff_file {
     hid_t file = -1;
+    std::string path;
+    int strategy = CCCS_H5_WRITE_CENTRAL;
+    std::string sidecar;  // PER_BEAMLET: "<path>_external/"
     int compress_lvl = -1;
     uint64_t chunksize = 50;
     std::vector<int> request_uids;  // request beam index -> beam uid
@@ -347,9 +404,27 @@ int cccs_h5_coeff_create(const char* path, const cccs_problem* prob, const cccs_
     }
     auto* h = new cccs_h5_coeff_file();
     h->file = fid;
+    h->path = path;
     h->compress_lvl = compress_lvl;
     h->chunksize = chunksize;
     *out = h;
+    return CCCS_OK;
+}
+
+int cccs_h5_coeff_set_write_strategy(cccs_h5_coeff_file* f, int strategy) {
+    g_err.clear();
+    if (!f) return fail(CCCS_EINVAL, "null argument");
+    if (strategy != CCCS_H5_WRITE_CENTRAL && strategy != CCCS_H5_WRITE_PER_BEAMLET)
+        return fail(CCCS_EINVAL, "unknown write strategy");
+    std::lock_guard<std::mutex> lk(g_lock);
+    if (!f->beams_written.empty() || f->columns) return fail(CCCS_EINVAL, "set the write strategy before any beam");
+    f->strategy = strategy;
+    f->sidecar.clear();
+    if (strategy == CCCS_H5_WRITE_PER_BEAMLET) {
+        // SparsifyManager::init (sparsify_manager.cpp:196-204): a fresh "<file>_external/" directory
+        f->sidecar = f->path + "_external/";
+        if (!remove_tree(f->sidecar) || !make_dir(f->sidecar)) return fail(CCCS_EIO, "cannot create " + f->sidecar);
+    }
     return CCCS_OK;
 }
 
@@ -382,6 +457,11 @@ int cccs_h5_coeff_add_beams(cccs_h5_coeff_file* f, const cccs_beam* beams, int n
         if (!dg.ok() || !g.ok()) return fail(CCCS_EIO, "cannot create the groups of " + meta);
         if (!write_beam_attrs(g, uid, (unsigned)active, b, fluence[i], labels ? &labels[i] : nullptr))
             return fail(CCCS_EIO, "cannot write the attributes of " + meta);
+        if (f->strategy == CCCS_H5_WRITE_PER_BEAMLET) {
+            char sub[32];
+            std::snprintf(sub, sizeof(sub), "beam_%05d", uid);
+            if (!make_dir(f->sidecar + sub)) return fail(CCCS_EIO, "cannot create " + f->sidecar + sub);
+        }
         f->beams_written.insert(uid);
     }
     return CCCS_OK;
@@ -407,32 +487,35 @@ int cccs_h5_coeff_add_column(cccs_h5_coeff_file* f, const cccs_sparse_column* c)
     if (!f->beams_written.count(uid)) return fail(CCCS_EINVAL, "column of a beam with no active beamlet");
     char name[96];
     std::snprintf(name, sizeof(name), "/beams/data/beam_%05d/beamlet_%05d", uid, c->beamlet_index);
-    Id g = group_create(f->file, name);
-    if (!g.ok()) return fail(CCCS_EIO, std::string("cannot create ") + name + " (written twice?)");
-    const unsigned bu = (unsigned)c->beamlet_index;
     const uint64_t n = c->nnz;
-    bool ok = attr_scalar(g, "beamlet_uid", H5T_STD_U16LE, H5T_NATIVE_UINT, &bu) &&
-              attr_scalar(g, "N_coeffs", H5T_STD_U64LE, H5T_NATIVE_UINT64, &n) &&
-              attr_scalar(g, "perc_nonzero", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, &c->perc_nonzero) &&
-              attr_scalar(g, "perc_dropped", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, &c->perc_dropped);
-    if (ok && c->nblocks > 0)
-        ok = attr_vector(g, "row_block_sizes", H5T_STD_U64LE, H5T_NATIVE_UINT64, c->nblocks, c->row_block_sizes) &&
-             attr_scalar(g, "perc_reducedrop", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, &c->perc_reducedrop);
-    if (!ok) return fail(CCCS_EIO, std::string("cannot write the attributes of ") + name);
-    Id plist(H5Pcreate(H5P_DATASET_CREATE), H5Pclose);
-    if (f->compress_lvl >= 0) {
-        const hsize_t chunk = std::min<hsize_t>(n, f->chunksize);
-        if (H5Pset_chunk(plist, 1, &chunk) < 0 || H5Pset_deflate(plist, (unsigned)std::min(9, f->compress_lvl)) < 0)
-            return fail(CCCS_EIO, "cannot set chunking / deflate");
+    if (f->strategy == CCCS_H5_WRITE_PER_BEAMLET) {
+        // the column goes to its own sidecar file (metadata + datasets at its root); the central
+        // group keeps the metadata and external links to the sidecar's datasets
+        // (W2FWorker_PerBeamlet::write_beamlet, sparsify_manager.cpp:129-174)
+        char sub[64];
+        std::snprintf(sub, sizeof(sub), "beam_%05d/beam%05d_beamlet%05d.bmlt", uid, uid, c->beamlet_index);
+        const std::string file = f->sidecar + sub;
+        {
+            Id sf(H5Fcreate(file.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+            if (!sf.ok()) return fail(CCCS_EIO, "cannot create " + file);
+            Id root(H5Gopen2(sf, "/", H5P_DEFAULT), H5Gclose);
+            if (!root.ok() || !write_beamlet_attrs(root, c) || !write_beamlet_data(root, c, f->compress_lvl, f->chunksize))
+                return fail(CCCS_EIO, "cannot write " + file);
+        }
+        Id g = group_create(f->file, name);
+        if (!g.ok()) return fail(CCCS_EIO, std::string("cannot create ") + name + " (written twice?)");
+        if (!write_beamlet_attrs(g, c)) return fail(CCCS_EIO, std::string("cannot write the attributes of ") + name);
+        const std::string target = base_name(f->sidecar) + "/" + sub;
+        if (H5Lcreate_external(target.c_str(), "/lindex", g, "lindex", H5P_DEFAULT, H5P_DEFAULT) < 0 ||
+            H5Lcreate_external(target.c_str(), "/coeffs", g, "coeffs", H5P_DEFAULT, H5P_DEFAULT) < 0)
+            return fail(CCCS_EIO, std::string("cannot link ") + name + " to " + target);
+    } else {
+        Id g = group_create(f->file, name);
+        if (!g.ok()) return fail(CCCS_EIO, std::string("cannot create ") + name + " (written twice?)");
+        if (!write_beamlet_attrs(g, c)) return fail(CCCS_EIO, std::string("cannot write the attributes of ") + name);
+        if (!write_beamlet_data(g, c, f->compress_lvl, f->chunksize))
+            return fail(CCCS_EIO, std::string("cannot write the datasets of ") + name);
     }
-    const hsize_t dims = n;
-    Id s = simple_space(1, &dims);
-    Id k(H5Dcreate2(g, "lindex", H5T_STD_U64LE, s, H5P_DEFAULT, plist, H5P_DEFAULT), H5Dclose);
-    if (!k.ok() || H5Dwrite(k, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, c->keys) < 0)
-        return fail(CCCS_EIO, std::string("cannot write ") + name + "/lindex");
-    Id v(H5Dcreate2(g, "coeffs", H5T_IEEE_F32LE, s, H5P_DEFAULT, plist, H5P_DEFAULT), H5Dclose);
-    if (!v.ok() || H5Dwrite(v, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, c->values) < 0)
-        return fail(CCCS_EIO, std::string("cannot write ") + name + "/coeffs");
     f->columns++;
     f->coeffs += n;
     return CCCS_OK;

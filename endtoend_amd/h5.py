@@ -16,6 +16,7 @@ import numpy as np
 from . import abi
 
 LIB_PATH = os.path.join(abi.HERE, "lib", "libcccs_h5.so")
+WRITE_CENTRAL, WRITE_PER_BEAMLET = 0, 1
 
 
 class CalcSpecs(C.Structure):
@@ -44,6 +45,7 @@ EXPORTS = [
     ("cccs_h5_read_volume", C.c_int, [C.c_char_p, C.c_char_p, _f32p, C.c_size_t, _u32p, _f32p, _f32p]),
     ("cccs_h5_coeff_create", C.c_int, [C.c_char_p, C.POINTER(abi.Problem), C.POINTER(CalcSpecs), C.c_int, C.c_uint64,
                                        C.POINTER(_P)]),
+    ("cccs_h5_coeff_set_write_strategy", C.c_int, [_P, C.c_int]),
     ("cccs_h5_coeff_add_beams", C.c_int, [_P, C.POINTER(abi.Beam), C.c_int, C.POINTER(_f32p), C.c_int,
                                           C.POINTER(BeamLabels)]),
     ("cccs_h5_coeff_add_column", C.c_int, [_P, C.POINTER(abi.SparseColumn)]),
@@ -141,7 +143,8 @@ class CoeffFile:
 
     def __init__(self, path: str, problem: abi.Problem, nbeams: int, sparsity_thresh: float = 1e-4,
                  beam_spectrum: str = "", target_structure: str = "", version: str = "endtoend_amd",
-                 rois: tuple | None = None, compress_lvl: int = -1, chunksize: int = 50):
+                 rois: tuple | None = None, compress_lvl: int = -1, chunksize: int = 50,
+                 write_strategy: int = 0):
         self.lib = load()
         self._keep = []
         specs = CalcSpecs(version.encode(), nbeams, sparsity_thresh, beam_spectrum.encode(), target_structure.encode(),
@@ -157,6 +160,13 @@ class CoeffFile:
         check(self.lib.cccs_h5_coeff_create(path.encode(), C.byref(problem), C.byref(specs), compress_lvl, chunksize,
                                             C.byref(h)), "cccs_h5_coeff_create")
         self.handle = h.value
+        if write_strategy:
+            st = self.lib.cccs_h5_coeff_set_write_strategy(self.handle, write_strategy)
+            if st != 0:
+                msg = self.lib.cccs_h5_last_error().decode(errors="replace")
+                self.lib.cccs_h5_coeff_close(self.handle)
+                self.handle = None
+                raise abi.CCCSError(st, "cccs_h5_coeff_set_write_strategy", msg)
         self.sink = abi.SPARSE_SINK(C.cast(self.lib.cccs_h5_coeff_sink, C.c_void_p).value)
 
     def add_beams(self, beams: Sequence[abi.Beam], fmaps: Sequence[np.ndarray], uid_base: int = 0,

@@ -80,6 +80,17 @@ typedef struct cccs_h5_coeff_file cccs_h5_coeff_file;
 int cccs_h5_coeff_create(const char* path, const cccs_problem* prob, const cccs_h5_calc_specs* specs,
                          int compress_lvl, uint64_t chunksize, cccs_h5_coeff_file** out);
 
+/* Output layout of the columns (WRITE_STRATEGY, dosecalc-beamlet/sparsify_manager.h:25-28):
+ * CENTRAL writes every column's datasets into the file; PER_BEAMLET writes each column to its own
+ * HDF5 file "<path>_external/beam_%05d/beam%05d_beamlet%05d.bmlt" (beamlet attributes and the
+ * lindex/coeffs datasets at its root) and keeps in the central group the attributes and two external
+ * links, lindex and coeffs, to "<basename(path)>_external/beam_.../...bmlt" /lindex and /coeffs
+ * (sparsify_manager.cpp:129-174; the sidecar directory is removed and created afresh, :196-204).
+ * Call after cccs_h5_coeff_create, before any beam is declared. */
+#define CCCS_H5_WRITE_CENTRAL     0
+#define CCCS_H5_WRITE_PER_BEAMLET 1
+int cccs_h5_coeff_set_write_strategy(cccs_h5_coeff_file* f, int strategy);
+
 /* Declares the beams of the next cccs_beamlet_dose request: request beam i gets uid
  * uid_base + i.  Each beam with at least one beamlet of fluence > 0 gets
  * /beams/metadata/beam_%05d (beam_uid, N_beamlets, beam_specs compound, isocenter_type,

@@ -222,3 +222,29 @@ def coefficient_matrix(root: Group):
     for j, (_, _, g) in enumerate(cols):
         A[g.datasets["lindex"].data.astype(np.int64), j] = g.datasets["coeffs"].data
     return A, [(b, l) for b, l, _ in cols]
+
+
+def external_link(path: str, link: str):
+    """(file name, object path) of the external link `link` (a path inside the file at `path`), or
+    None when `link` is not an external link."""
+    lib = _load()
+    f = lib.H5Fopen(path.encode(), 0, _hid(0))
+    if f < 0:
+        raise OSError(f"cannot open {path}")
+    try:
+        info = (C.c_ubyte * 64)()
+        if lib.H5Lget_info(_hid(f), link.encode(), info, _hid(0)) < 0:
+            raise KeyError(link)
+        # H5L_info_t: H5L_type_t type (int) first; H5L_TYPE_EXTERNAL = 64
+        if C.cast(info, C.POINTER(C.c_int))[0] != 64:
+            return None
+        buf = (C.c_char * 4096)()
+        if lib.H5Lget_val(_hid(f), link.encode(), buf, C.c_size_t(4096), _hid(0)) < 0:
+            raise OSError("H5Lget_val failed")
+        fn, on = C.c_char_p(), C.c_char_p()
+        flags = C.c_uint(0)
+        if lib.H5Lunpack_elink_val(buf, C.c_size_t(4096), C.byref(flags), C.byref(fn), C.byref(on)) < 0:
+            raise OSError("H5Lunpack_elink_val failed")
+        return fn.value.decode(), on.value.decode()
+    finally:
+        lib.H5Fclose(_hid(f))

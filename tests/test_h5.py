@@ -190,6 +190,59 @@ def test_coefficient_file_layout_and_matrix(tmp_path):
         assert np.array_equal(A[:, j], dense)
 
 
+def test_per_beamlet_write_strategy(tmp_path):
+    """PER_BEAMLET (sparsify_manager.cpp:129-174): each column in its own sidecar file, the central
+    group holding the beamlet attributes and external links lindex/coeffs to it; read through the
+    links, the coefficient matrix equals the CENTRAL file's."""
+    rng = np.random.default_rng(11)
+    spec = _spec()
+    p = spec.struct()
+    nvox = int(np.prod(spec.size))
+    beams, fmaps = _coeff_case(rng)
+    cols = (_columns(rng, nvox, 0, [0, 2, 3, 4, 6, 11], empty=(3,)) + _columns(rng, nvox, 2, [0, 1, 7, 8, 11]))
+    mats = {}
+    for strat in (H.WRITE_CENTRAL, H.WRITE_PER_BEAMLET):
+        d = tmp_path / f"s{strat}"
+        d.mkdir()
+        path = str(d / "Dose_Coefficients.h5")
+        side = d / "Dose_Coefficients.h5_external"
+        if strat == H.WRITE_PER_BEAMLET:   # a stale sidecar tree is replaced
+            (side / "beam_00009").mkdir(parents=True)
+            (side / "beam_00009" / "old.bmlt").write_bytes(b"x")
+        with H.CoeffFile(path, p, nbeams=3, sparsity_thresh=1e-4, compress_lvl=4, chunksize=7,
+                         write_strategy=strat) as f:
+            f.add_beams(beams, fmaps, uid_base=0)
+            for c in cols:
+                f.add_column(*c)
+        root = HW.walk(path)
+        mats[strat] = HW.coefficient_matrix(root)
+        if strat == H.WRITE_CENTRAL:
+            assert not side.exists()
+            continue
+        assert sorted(x.name for x in side.iterdir()) == ["beam_00000", "beam_00002"]
+        for bi, bl, keys, vals, pnz, pdr in cols:
+            fn = side / f"beam_{bi:05d}" / f"beam{bi:05d}_beamlet{bl:05d}.bmlt"
+            assert fn.exists() == (len(keys) > 0)
+            if not len(keys):
+                continue
+            grp = f"/beams/data/beam_{bi:05d}/beamlet_{bl:05d}"
+            target = f"Dose_Coefficients.h5_external/beam_{bi:05d}/beam{bi:05d}_beamlet{bl:05d}.bmlt"
+            assert HW.external_link(path, grp + "/lindex") == (target, "/lindex")
+            assert HW.external_link(path, grp + "/coeffs") == (target, "/coeffs")
+            side_root = HW.walk(str(fn))
+            assert set(side_root.datasets) == {"lindex", "coeffs"} and not side_root.groups
+            assert int(side_root.attrs["beamlet_uid"].value) == bl and int(side_root.attrs["N_coeffs"].value) == len(keys)
+            assert side_root.attrs["perc_nonzero"].value == np.float32(pnz)
+            li = side_root.datasets["lindex"]
+            assert (li.dtype, li.layout, li.chunk, li.nfilters) == ("<u8", "chunked", (min(7, len(keys)),), 1)
+            assert np.array_equal(li.data, keys)
+            central = root.groups["beams"].groups["data"].groups[f"beam_{bi:05d}"].groups[f"beamlet_{bl:05d}"]
+            assert {k: a.dtype for k, a in central.attrs.items()} == {
+                "beamlet_uid": "<u2", "N_coeffs": "<u8", "perc_nonzero": "<f4", "perc_dropped": "<f4"}
+    (A0, c0), (A1, c1) = mats[H.WRITE_CENTRAL], mats[H.WRITE_PER_BEAMLET]
+    assert c0 == c1 and np.array_equal(A0, A1)
+
+
 def test_reduced_coefficient_file(tmp_path):
     rng = np.random.default_rng(3)
     spec = _spec()
