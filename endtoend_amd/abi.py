@@ -103,6 +103,12 @@ class SparseColumn(C.Structure):
     ]
 
 
+class CTInfo(C.Structure):
+    _fields_ = [("size", C.c_uint32 * 3), ("voxel_mm", C.c_float * 3), ("start_mm", C.c_float * 3),
+                ("orientation", C.c_float * 6), ("rescale_slope", C.c_float), ("rescale_intercept", C.c_float),
+                ("sop_type", C.c_int)]
+
+
 class TuneReport(C.Structure):
     _fields_ = [
         ("nkeys", C.c_int),
@@ -182,6 +188,25 @@ EXPORTS = [
     ("cccs_server_stats", C.c_int, [_P, C.POINTER(ServerInfo)]),
     ("cccs_server_preload_angles", C.c_int, [_P, C.POINTER(Beam), C.c_int, C.c_uint64, C.POINTER(C.c_int)]),
     ("cccs_server_destroy", C.c_int, [_P]),
+    ("cccs_dicom_ct_info", C.c_int, [C.c_char_p, C.POINTER(CTInfo)]),
+    ("cccs_dicom_ct_read", C.c_int, [C.c_char_p, _fp, C.c_uint64]),
+    ("cccs_dicom_rtstruct_open", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("cccs_rtstruct_count", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("cccs_rtstruct_roi_info", C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_uint64)]),
+    ("cccs_rtstruct_roi_points", C.c_int, [_P, C.c_int, C.POINTER(C.c_uint32), _fp]),
+    ("cccs_rtstruct_find", C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    ("cccs_rtstruct_close", C.c_int, [_P]),
+    ("cccs_roi_extents", C.c_int, [_fp, C.c_uint64, C.POINTER(C.c_uint32), _fp, _fp, C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint32)]),
+    ("cccs_roi_centroid", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), _fp, _fp, _fp]),
+    ("cccs_contour_rasterize", C.c_int, [_fp, C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32), _fp, _fp,
+                                         C.c_void_p]),
+    ("cccs_ct_lut_load", C.c_int, [C.c_char_p, _fp, _fp, C.c_int, C.POINTER(C.c_int)]),
+    ("cccs_ct_to_density", C.c_int, [C.c_int, _fp, C.POINTER(C.c_uint32), _fp, C.c_float, _fp, _fp, C.c_int, _fp,
+                                     C.c_uint64, C.POINTER(C.c_uint32)]),
+    ("cccs_mask_resample", C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_uint32), _fp, _fp, C.POINTER(C.c_uint32),
+                                     _fp, _fp, C.c_void_p]),
     ("cccs_load_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_int, C.POINTER(C.c_int)]),
     ("cccs_load_omni_beam_list", C.c_int, [C.c_char_p, C.c_int, C.POINTER(Beam), C.c_char_p, C.c_int,
                                            C.POINTER(C.c_int)]),

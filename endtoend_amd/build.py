@@ -21,7 +21,8 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcccs.so")
 
-SOURCES = ["host.cpp", "io.cpp", "server.cpp", "tune.cpp", "context.hip", "beam.hip", "beamlet.hip"]
+SOURCES = ["host.cpp", "io.cpp", "server.cpp", "tune.cpp", "dicom.cpp", "context.hip", "beam.hip", "beamlet.hip",
+           "ingest.hip"]
 HEADERS = ["cccs_internal.hpp", "context.hpp", "device_common.hpp", "errors.hpp"]
 
 ARCH = os.environ.get("CCCS_OFFLOAD_ARCH", "gfx950")

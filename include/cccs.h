@@ -335,6 +335,73 @@ int cccs_fmap_aperture_fill(float* fluence, int width, int height, float fill_va
 /* 1 when every row has at most one recorded run (fmap_is_apertureready, fmapProcessing.cpp:26-32). */
 int cccs_fmap_is_aperture_ready(const float* fluence, int width, int height, int* ready);
 
+/* ---- DICOM ingestion (dosecalc-preprocess with --dicom, main.cu:139-320) -----------------------
+ * Host readers replace RTImage / RTStruct (CCCS/RTClasses/rtimages.cpp:20-400, rtstruct.cpp:100-380,
+ * DCMTK there); files are DICOM Part 10, explicit or implicit VR little endian.
+ * cccs_dicom_ct_info/_read: the files of `dir` whose names contain ".dcm" and whose SOP class is CT,
+ * MR or RT image, sorted from foot to head by ImagePositionPatient z.  size = {Columns, Rows,
+ * slices}; voxel_mm = {PixelSpacing[0], PixelSpacing[0], z1 - z0 (SliceThickness for one slice)}
+ * as the reference reads them; start_mm = ImagePositionPatient of the first slice.  The volume is
+ * [slice][row][column] in FP32: pixel (two's complement when PixelRepresentation is 1) times
+ * RescaleSlope plus RescaleIntercept for CT (16-bit pixels only). */
+typedef struct cccs_ct_info {
+    uint32_t size[3];
+    float    voxel_mm[3];
+    float    start_mm[3];
+    float    orientation[6];     /* ImageOrientationPatient of the first slice                   */
+    float    rescale_slope, rescale_intercept;
+    int      sop_type;           /* 1 CT, 2 MR, 3 RT image                                       */
+} cccs_ct_info;
+int cccs_dicom_ct_info(const char* dir, cccs_ct_info* out);
+int cccs_dicom_ct_read(const char* dir, float* hu, uint64_t cap);
+
+/* The first RT Structure Set file of `dir` (any name).  ROI r is item r of the Structure Set ROI
+ * Sequence (name, number) with the contours of item r of the ROI Contour Sequence: per contour
+ * NumberOfContourPoints x {x, y, z} of ContourData in mm, stored as FP32. */
+typedef struct cccs_rtstruct cccs_rtstruct;
+int cccs_dicom_rtstruct_open(const char* dir, cccs_rtstruct** out);
+int cccs_rtstruct_count(const cccs_rtstruct* s, int* nroi);
+int cccs_rtstruct_roi_info(const cccs_rtstruct* s, int roi, char* name, int name_cap, int* roi_number, int* nsub,
+                           uint64_t* npoints);
+int cccs_rtstruct_roi_points(const cccs_rtstruct* s, int roi, uint32_t* sub_counts, float* points_mm);
+/* getROIIndex (DoseCalcIO/roi.cpp:201-215): first ROI whose name equals (exact) or contains
+ * `search`; *roi = -1 when none does. */
+int cccs_rtstruct_find(const cccs_rtstruct* s, const char* search, int exact, int* roi);
+int cccs_rtstruct_close(cccs_rtstruct* s);
+
+/* getROIExtents (DoseCalcIO/roi.cpp:245-316): index box of the contour points on a frame (points in
+ * mm, frame in cm), floor/ceil of the extremes, clamped to the frame. */
+int cccs_roi_extents(const float* points_mm, uint64_t npoints, const uint32_t frame_size[3],
+                     const float frame_start_cm[3], const float frame_spacing_cm[3], uint32_t crop_start[3],
+                     uint32_t crop_size[3]);
+/* getROICentroid (DoseCalcIO/roi.cpp:319-345): mean of the voxel centres of a mask [cm]. */
+int cccs_roi_centroid(const uint8_t* mask, const uint32_t size[3], const float start_cm[3], const float spacing_cm[3],
+                      float centroid_cm[3]);
+/* The per-slice part of generateContourMask (cudaMemFunctions.cu:218-255) on the CT grid: each
+ * contour goes to slice round((z0 - start) / spacing) of its first point, its vertices to
+ * floor((x - start) / spacing), and its filled polygon is OR-ed into the slice (edges drawn as
+ * 8-connected lines, interior by scanlines between edge crossings). */
+int cccs_contour_rasterize(const float* points_mm, const uint32_t* sub_counts, int nsub, const uint32_t ct_size[3],
+                           const float ct_start_cm[3], const float ct_spacing_cm[3], uint8_t* mask);
+/* load_lookup_table (DoseCalcIO/ctlut.cpp:39-90): "name CT# density [rel]" lines, sorted by CT#.
+ * With hunits/massdens NULL only *n is set. */
+int cccs_ct_lut_load(const char* path, float* hunits, float* massdens, int cap, int* n);
+/* CT volume (FP32 HU, [z][y][x], voxel in cm) resampled on device `device` onto the isotropic grid
+ * of iso_voxel_cm (out_size = ceil(voxel * (size / iso)), origin at the CT origin) with linear
+ * interpolation (texture-unit weights: 8 fractional bits; zero outside the CT) and converted to
+ * density by the sorted lookup table (clamped below its first point, extrapolated above its last),
+ * or, with nlut = 0, by (HU + 1024) / 1024 clamped at 0.05 (cudaCreateTexIso and its kernels,
+ * cudaMemFunctions.cu:83-216, kernels.cuh:13-135).  density = NULL only reports out_size. */
+int cccs_ct_to_density(int device, const float* hu, const uint32_t size[3], const float voxel_cm[3],
+                       float iso_voxel_cm, const float* lut_hu, const float* lut_dens, int nlut,
+                       float* density, uint64_t cap, uint32_t out_size[3]);
+/* Mask on one grid point-sampled onto another (voxel index floor(u + 0.5) of the other grid's
+ * position, zero outside) and thresholded at 1 (cudaResample + cudaThreshold,
+ * CudaUtilities/array_sum.cuh:13-58), on device `device`. */
+int cccs_mask_resample(int device, const uint8_t* mask, const uint32_t in_size[3], const float in_start_cm[3],
+                       const float in_spacing_cm[3], const uint32_t out_size[3], const float out_start_cm[3],
+                       const float out_spacing_cm[3], uint8_t* out);
+
 /* ---- the reference's file formats (host only) -----------------------------------------------
  * Beam list "az zen [coll] [iso: x y z] [sad: s] [dir: x y z]" in degrees/cm, '#' comments,
  * default SAD 100 cm; each beam is reconfigured (from angles, or from dir: when given).

@@ -1389,3 +1389,173 @@ int orc_fluence_projection(const cccs_problem* p, const uint8_t* mask, const ccc
 }
 
 }  // extern "C"
+
+
+// ---- DICOM ingestion ----------------------------------------------------------------------------
+// Restated from dosecalc-preprocess (cudaMemFunctions.cu:83-294, kernels.cuh:13-135) and
+// DoseCalcIO/roi.cpp:245-345, one output element at a time.
+namespace {
+float orc_tex_weight(float f) { return std::floor(f * 256.f + 0.5f) * (1.f / 256.f); }
+float orc_ct_at(const float* hu, const uint32_t* n, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= (int)n[0] || j >= (int)n[1] || k >= (int)n[2]) return 0.f;
+    return hu[(size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * k)];
+}
+void orc_line(uint8_t* img, int w, int h, int x0, int y0, int x1, int y1) {
+    // 8-connected line, error form of Bresenham's algorithm
+    int dx = x1 > x0 ? x1 - x0 : x0 - x1, dy = y1 > y0 ? y1 - y0 : y0 - y1;
+    int sx = x0 < x1 ? 1 : -1, sy = y0 < y1 ? 1 : -1, err = dx - dy;
+    while (true) {
+        if (x0 >= 0 && x0 < w && y0 >= 0 && y0 < h) img[(size_t)y0 * w + x0] = 1;
+        if (x0 == x1 && y0 == y1) return;
+        int e2 = 2 * err;
+        if (e2 > -dy) { err -= dy; x0 += sx; }
+        if (e2 < dx) { err += dx; y0 += sy; }
+    }
+}
+}  // namespace
+
+extern "C" int orc_ct_to_density(const float* hu, const uint32_t* size, const float* vox, float iso,
+                                 const float* lut_hu, const float* lut_dens, int nlut, float* out, uint32_t* out_size) {
+    for (int a = 0; a < 3; a++) out_size[a] = (uint32_t)std::ceil(vox[a] * ((float)size[a] / iso));
+    if (!out) return 0;
+    for (uint32_t Z = 0; Z < out_size[2]; Z++)
+        for (uint32_t Y = 0; Y < out_size[1]; Y++)
+            for (uint32_t X = 0; X < out_size[0]; X++) {
+                const float p[3] = {((float)X * iso) / vox[0], ((float)Y * iso) / vox[1], ((float)Z * iso) / vox[2]};
+                float b[3], w[3];
+                int ix[3];
+                for (int a = 0; a < 3; a++) {
+                    b[a] = (p[a] + 0.5f) - 0.5f;
+                    const float f = std::floor(b[a]);
+                    ix[a] = (int)f;
+                    w[a] = orc_tex_weight(b[a] - f);
+                }
+                float c[2][2];
+                for (int dz = 0; dz < 2; dz++)
+                    for (int dy = 0; dy < 2; dy++)
+                        c[dz][dy] = (1.f - w[0]) * orc_ct_at(hu, size, ix[0], ix[1] + dy, ix[2] + dz) +
+                                    w[0] * orc_ct_at(hu, size, ix[0] + 1, ix[1] + dy, ix[2] + dz);
+                const float c0 = (1.f - w[1]) * c[0][0] + w[1] * c[0][1];
+                const float c1 = (1.f - w[1]) * c[1][0] + w[1] * c[1][1];
+                float v = (1.f - w[2]) * c0 + w[2] * c1;
+                if (nlut >= 2) {
+                    if (v <= lut_hu[0]) v = lut_dens[0];
+                    else if (v >= lut_hu[nlut - 1])
+                        v = lut_dens[nlut - 2] + (v - lut_hu[nlut - 2]) * (lut_dens[nlut - 1] - lut_dens[nlut - 2]) /
+                                                     (lut_hu[nlut - 1] - lut_hu[nlut - 2]);
+                    else {
+                        int lo = 0;
+                        while (lut_hu[lo] < v && lo < nlut - 1) lo++;
+                        lo--;
+                        v = lut_dens[lo] + (v - lut_hu[lo]) * (lut_dens[lo + 1] - lut_dens[lo]) / (lut_hu[lo + 1] - lut_hu[lo]);
+                    }
+                } else {
+                    v = (v + 1024.0f) / 1024.0f;
+                    if (v < 0.05f) v = 0.05f;
+                }
+                out[(size_t)X + (size_t)out_size[0] * ((size_t)Y + (size_t)out_size[1] * Z)] = v;
+            }
+    return 0;
+}
+
+extern "C" int orc_mask_resample(const uint8_t* in, const uint32_t* is, const float* ist, const float* isp,
+                                 const uint32_t* os, const float* ost, const float* osp, uint8_t* out) {
+    for (uint32_t Z = 0; Z < os[2]; Z++)
+        for (uint32_t Y = 0; Y < os[1]; Y++)
+            for (uint32_t X = 0; X < os[0]; X++) {
+                const uint32_t o[3] = {X, Y, Z};
+                int t[3];
+                bool inside = true;
+                for (int a = 0; a < 3; a++) {
+                    const float u = ((float)o[a] * osp[a] + ost[a] - ist[a]) / isp[a];
+                    t[a] = (int)std::floor(u + 0.5f);
+                    inside = inside && t[a] >= 0 && t[a] < (int)is[a];
+                }
+                const uint8_t v = inside ? in[(size_t)t[0] + (size_t)is[0] * ((size_t)t[1] + (size_t)is[1] * t[2])] : 0;
+                out[(size_t)X + (size_t)os[0] * ((size_t)Y + (size_t)os[1] * Z)] = v >= 1 ? 1 : 0;
+            }
+    return 0;
+}
+
+extern "C" int orc_roi_extents(const float* pts, uint64_t n, const uint32_t* size, const float* start, const float* sp,
+                               uint32_t* cs, uint32_t* cz) {
+    for (int a = 0; a < 3; a++) {
+        float mn = std::numeric_limits<float>::max(), mx = std::numeric_limits<float>::min();
+        for (uint64_t i = 0; i < n; i++) {
+            mn = pts[3 * i + a] < mn ? pts[3 * i + a] : mn;
+            mx = pts[3 * i + a] > mx ? pts[3 * i + a] : mx;
+        }
+        int s = (int)std::floor((0.1f * mn - start[a]) / sp[a]), e = (int)std::ceil((0.1f * mx - start[a]) / sp[a]);
+        s = s < 0 ? 0 : (s > (int)size[a] ? (int)size[a] : s);
+        e = e < 0 ? 0 : (e > (int)size[a] ? (int)size[a] : e);
+        cs[a] = (uint32_t)s;
+        cz[a] = (uint32_t)(e - s);
+    }
+    return 0;
+}
+
+extern "C" int orc_roi_centroid(const uint8_t* mask, const uint32_t* size, const float* start, const float* sp,
+                                float* c) {
+    float sx = 0.f, sy = 0.f, sz = 0.f, cnt = 0.f;
+    for (uint32_t Z = 0; Z < size[2]; Z++)
+        for (uint32_t Y = 0; Y < size[1]; Y++)
+            for (uint32_t X = 0; X < size[0]; X++)
+                if (mask[(size_t)X + (size_t)size[0] * ((size_t)Y + (size_t)size[1] * Z)]) {
+                    sx += start[0] + ((float)X + 0.5f) * sp[0];
+                    sy += start[1] + ((float)Y + 0.5f) * sp[1];
+                    sz += start[2] + ((float)Z + 0.5f) * sp[2];
+                    cnt += 1.f;
+                }
+    c[0] = sx / cnt;
+    c[1] = sy / cnt;
+    c[2] = sz / cnt;
+    return 0;
+}
+
+extern "C" int orc_contour_rasterize(const float* pts, const uint32_t* counts, int nsub, const uint32_t* size,
+                                     const float* start, const float* sp, uint8_t* mask) {
+    const int w = (int)size[0], h = (int)size[1];
+    const size_t plane = (size_t)w * h;
+    std::fill(mask, mask + plane * size[2], (uint8_t)0);
+    std::vector<uint8_t> img(plane);
+    size_t mark = 0;
+    for (int c = 0; c < nsub; c++) {
+        const int np = (int)counts[c];
+        if (!np) continue;
+        const int z = (int)std::round((0.1f * pts[3 * mark + 2] - start[2]) / sp[2]);
+        std::vector<int> xs(np), ys(np);
+        for (int q = 0; q < np; q++) {
+            xs[q] = (int)std::floor((0.1f * pts[3 * (mark + q)] - start[0]) / sp[0]);
+            ys[q] = (int)std::floor((0.1f * pts[3 * (mark + q) + 1] - start[1]) / sp[1]);
+        }
+        mark += (size_t)np;
+        if (z < 0 || z >= (int)size[2]) continue;
+        std::fill(img.begin(), img.end(), (uint8_t)0);
+        for (int q = 0; q < np; q++) orc_line(img.data(), w, h, xs[q], ys[q], xs[(q + 1) % np], ys[(q + 1) % np]);
+        if (np >= 3)
+            for (int y = 0; y < h; y++) {
+                // crossings x = x0 + (y - y0)(x1 - x0)/(y1 - y0) of edges with y0 <= y < y1, as doubles
+                // compared exactly through their rational form
+                std::vector<std::pair<long long, long long>> xc;
+                for (int q = 0; q < np; q++) {
+                    int x0 = xs[q], y0 = ys[q], x1 = xs[(q + 1) % np], y1 = ys[(q + 1) % np];
+                    if (y0 == y1) continue;
+                    if (y0 > y1) { std::swap(x0, x1); std::swap(y0, y1); }
+                    if (y < y0 || y >= y1) continue;
+                    xc.emplace_back((long long)x0 * (y1 - y0) + (long long)(y - y0) * (x1 - x0), (long long)(y1 - y0));
+                }
+                std::sort(xc.begin(), xc.end(), [](const std::pair<long long, long long>& a,
+                                                   const std::pair<long long, long long>& b) {
+                    return (__int128)a.first * b.second < (__int128)b.first * a.second;
+                });
+                for (size_t k = 0; k + 1 < xc.size(); k += 2) {
+                    const long long lo = (long long)std::ceil((long double)xc[k].first / xc[k].second);
+                    const long long hi = (long long)std::floor((long double)xc[k + 1].first / xc[k + 1].second);
+                    for (long long x = lo < 0 ? 0 : lo; x <= hi && x < w; x++) img[(size_t)y * w + x] = 1;
+                }
+            }
+        uint8_t* dst = mask + plane * z;
+        for (size_t k = 0; k < plane; k++) dst[k] = (uint8_t)((img[k] + dst[k]) > 0);
+    }
+    return 0;
+}

@@ -129,6 +129,27 @@ int64_t orc_sparsify(const float* dense, const uint32_t* size, const uint32_t* c
                      const uint32_t* crop_size, float thresh, uint64_t* keys, float* vals,
                      int64_t cap, float* perc_nonzero, float* perc_dropped);
 
+/* DICOM ingestion (dosecalc-preprocess with --dicom).
+ * cudaCreateTexIso + cudaMakeIsotropicWithLUT / deviceHU2dens (cudaMemFunctions.cu:83-216,
+ * kernels.cuh:13-135): output voxel (X, Y, Z) samples the CT at texel coordinates
+ * (X iso / vx, Y iso / vy, Z iso / vz) + 0.5 with the texture unit's linear filter (8-bit
+ * fractional weights, zero border), then the lookup table (nlut >= 2) or (HU + 1024) / 1024 >= 0.05. */
+int orc_ct_to_density(const float* hu, const uint32_t* size, const float* voxel_cm, float iso, const float* lut_hu,
+                      const float* lut_dens, int nlut, float* out, uint32_t* out_size);
+/* cudaResample (point filter, zero border) + cudaThreshold(1) (CudaUtilities/array_sum.cuh:13-58). */
+int orc_mask_resample(const uint8_t* in, const uint32_t* in_size, const float* in_start, const float* in_sp,
+                      const uint32_t* out_size, const float* out_start, const float* out_sp, uint8_t* out);
+/* getROIExtents / getROICentroid (DoseCalcIO/roi.cpp:245-345). */
+int orc_roi_extents(const float* pts_mm, uint64_t npts, const uint32_t* size, const float* start, const float* sp,
+                    uint32_t* crop_start, uint32_t* crop_size);
+int orc_roi_centroid(const uint8_t* mask, const uint32_t* size, const float* start, const float* sp, float* c);
+/* Per-slice polygon fill of generateContourMask (cudaMemFunctions.cu:218-255): 8-connected edge
+ * lines plus scanline interiors (pixels ceil(left)..floor(right) between sorted crossings of the
+ * edges spanning [y0, y1)), OR-ed per slice.  Parity unpinned against OpenCV's fillPoly (absent
+ * here) beyond the shapes the tests fix. */
+int orc_contour_rasterize(const float* pts_mm, const uint32_t* counts, int nsub, const uint32_t* size,
+                          const float* start, const float* sp, uint8_t* mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,14 @@ def load() -> C.CDLL:
         "orc_beamlet_anchor": [P, B, C.c_int, _fp, _fp, _fp, _fp],
         "orc_fluence_projection": [P, C.c_void_p, B, C.c_int, _fp],
         "orc_fluence_convolve": [_fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _fp],
+        "orc_ct_to_density": [_fp, C.POINTER(C.c_uint32), _fp, C.c_float, _fp, _fp, C.c_int, _fp,
+                              C.POINTER(C.c_uint32)],
+        "orc_mask_resample": [C.c_void_p, C.POINTER(C.c_uint32), _fp, _fp, C.POINTER(C.c_uint32), _fp, _fp,
+                              C.c_void_p],
+        "orc_roi_extents": [_fp, C.c_uint64, C.POINTER(C.c_uint32), _fp, _fp, C.POINTER(C.c_uint32),
+                            C.POINTER(C.c_uint32)],
+        "orc_roi_centroid": [C.c_void_p, C.POINTER(C.c_uint32), _fp, _fp, _fp],
+        "orc_contour_rasterize": [_fp, C.c_void_p, C.c_int, C.POINTER(C.c_uint32), _fp, _fp, C.c_void_p],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -334,3 +342,66 @@ def beamlet_unpack_linear(spec: ProblemSpec, beam: abi.Beam, fmap: np.ndarray, c
     _chk(load().orc_beamlet_unpack_linear(C.byref(p), C.byref(beam), fp(f), fp(c), fp(out), cap, C.byref(n)),
          "beamlet_unpack_linear")
     return out[:n.value]
+
+
+# ---- DICOM ingestion --------------------------------------------------------------------------
+def ct_to_density(hu: np.ndarray, voxel_cm, iso: float, lut=None):
+    """(density [z, y, x], out_size x/y/z) of a HU volume [z, y, x] (orc_ct_to_density)."""
+    h = np.ascontiguousarray(hu, np.float32)
+    size = (C.c_uint32 * 3)(h.shape[2], h.shape[1], h.shape[0])
+    vox = (C.c_float * 3)(*[float(v) for v in voxel_cm])
+    osz = (C.c_uint32 * 3)()
+    lh = ld = None
+    n = 0
+    if lut is not None:
+        lh = np.ascontiguousarray(lut[0], np.float32)
+        ld = np.ascontiguousarray(lut[1], np.float32)
+        n = len(lh)
+    L = load()
+    _chk(L.orc_ct_to_density(fp(h), size, vox, C.c_float(iso), fp(lh) if n else None, fp(ld) if n else None, n,
+                             None, osz), "ct_to_density")
+    out = np.zeros((osz[2], osz[1], osz[0]), np.float32)
+    _chk(L.orc_ct_to_density(fp(h), size, vox, C.c_float(iso), fp(lh) if n else None, fp(ld) if n else None, n,
+                             fp(out), osz), "ct_to_density")
+    return out, tuple(osz)
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def _u3(v):
+    return (C.c_uint32 * 3)(*[int(x) for x in v])
+
+
+def mask_resample(mask: np.ndarray, in_start, in_sp, out_size, out_start, out_sp) -> np.ndarray:
+    m = np.ascontiguousarray(mask, np.uint8)
+    out = np.zeros((out_size[2], out_size[1], out_size[0]), np.uint8)
+    _chk(load().orc_mask_resample(m.ctypes.data, _u3((m.shape[2], m.shape[1], m.shape[0])), _f3(in_start),
+                                  _f3(in_sp), _u3(out_size), _f3(out_start), _f3(out_sp), out.ctypes.data),
+         "mask_resample")
+    return out
+
+
+def roi_extents(points_mm: np.ndarray, size, start, sp):
+    p = np.ascontiguousarray(points_mm, np.float32).reshape(-1)
+    cs, cz = (C.c_uint32 * 3)(), (C.c_uint32 * 3)()
+    _chk(load().orc_roi_extents(fp(p), len(p) // 3, _u3(size), _f3(start), _f3(sp), cs, cz), "roi_extents")
+    return tuple(cs), tuple(cz)
+
+
+def roi_centroid(mask: np.ndarray, start, sp):
+    m = np.ascontiguousarray(mask, np.uint8)
+    c = (C.c_float * 3)()
+    _chk(load().orc_roi_centroid(m.ctypes.data, _u3((m.shape[2], m.shape[1], m.shape[0])), _f3(start), _f3(sp), c),
+         "roi_centroid")
+    return np.array(c[:], np.float32)
+
+
+def contour_rasterize(points_mm: np.ndarray, counts, size, start, sp) -> np.ndarray:
+    p = np.ascontiguousarray(points_mm, np.float32).reshape(-1)
+    cn = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros((size[2], size[1], size[0]), np.uint8)
+    _chk(load().orc_contour_rasterize(fp(p), cn.ctypes.data, len(cn), _u3(size), _f3(start), _f3(sp),
+                                      out.ctypes.data), "contour_rasterize")
+    return out
