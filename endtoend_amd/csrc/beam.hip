@@ -690,6 +690,7 @@ struct PvcsArgs {
     const float* rev;
     float* dose;
     int ncones;
+    int q0, q1;                 // cones of this pass
     int nx, ny;                 // full volume dims (x, y)
     int x0, y0, z0, sx, sy, sz; // PVCS index box
     int tx, ty, tz;             // tiles per axis
@@ -726,7 +727,7 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
     float* dst = a.dose + ((size_t)gx + (size_t)a.nx * ((size_t)gy + (size_t)a.ny * gz));
     const float old = *dst;
     float sum = 0.f;
-    for (int q0 = 0; q0 < a.ncones; q0 += kPvcsGroup) {
+    for (int q0 = a.q0; q0 < a.q1; q0 += kPvcsGroup) {
         LineMeta mt[kPvcsGroup][4];
         float fa[kPvcsGroup], fb[kPvcsGroup], fc[kPvcsGroup];
         int di[kPvcsGroup];
@@ -739,7 +740,7 @@ __global__ void __launch_bounds__(256) pvcs_kernel(PvcsArgs a) {
 #pragma unroll
             for (int t = 0; t < 4; t++) mt[u][t] = LineMeta{0u, 0, 0};
             const int q = q0 + u;
-            if (q >= a.ncones) continue;
+            if (q >= a.q1) continue;
             const ConeDev& c = sc[q];
             const v3 r = c.a(g) + c.t;
             if (r.y >= (float)c.nx || r.z >= (float)c.ny || r.x >= (float)c.nz || r.x < 0.f || r.y < 0.f || r.z < 0.f)
@@ -1348,8 +1349,21 @@ int run_cones(BeamScratch& S, const cccs_beam& b, const float* d_terma, float* d
         const uint64_t tiles = (uint64_t)pa.tx * pa.ty * pa.tz;
         if (tiles > 0xffffffffULL) return set_error(CCCS_EGEOM, "PVCS box too large");
         const double nbox = (double)pa.sx * pa.sy * pa.sz;
+        // cones in passes: the REV data the resident tiles gather at a time then belongs to fewer
+        // cones (better L2 / Infinity Cache reuse); each pass adds its cones' dose to the volume.
+        // Automatic (0): as many cones per pass as keep a pass's REV data within 128 MB (measured
+        // on the 256x256x180 serving requests, ~31 MB per cone: 4 cones per pass -20 %; small plans
+        // stay in one pass, where extra passes only add launches)
+        int per = ctx->sched.pvcs_pass_cones;
+        if (per <= 0) {
+            const double cone_bytes = 4.0 * (double)stored / std::max(1, g.nrays);
+            per = (int)std::min<double>(g.nrays, std::max(1.0, std::floor(128.0 * 1048576.0 / std::max(cone_bytes, 1.0))));
+        }
         int tk = prof_begin(ctx, K_PVCS);
-        hipLaunchKernelGGL(pvcs_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, pa);
+        for (pa.q0 = 0; pa.q0 < g.nrays; pa.q0 += per) {
+            pa.q1 = std::min(g.nrays, pa.q0 + per);
+            hipLaunchKernelGGL(pvcs_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, pa);
+        }
         // algorithmic traffic: every stored sample read once, the box's dose read and written once
         prof_end(ctx, tk, 4.0 * (double)stored + 8.0 * nbox, 0.0, (uint64_t)nbox);
         CCCS_HIP(hipGetLastError());

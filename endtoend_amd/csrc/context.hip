@@ -360,6 +360,11 @@ extern "C" int cccs_ctx_memory(const cccs_ctx* c, uint64_t* resident, uint64_t* 
 extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
     clear_error();
     if (!c || !key) return set_error(CCCS_EINVAL, "null argument");
+    if (std::strcmp(key, "pvcs_pass_cones") == 0) {
+        if (value < 0 || value > kMaxRays) return set_error(CCCS_EINVAL, "pvcs_pass_cones must be in [0, 48]");
+        c->sched.pvcs_pass_cones = value;
+        return CCCS_OK;
+    }
     if (std::strcmp(key, "cone_ramp_min") == 0) {
         if (value < 1) return set_error(CCCS_EINVAL, "cone_ramp_min must be >= 1");
         c->sched.cone_ramp_min = value;
@@ -404,6 +409,7 @@ extern "C" int cccs_ctx_get_schedule(const cccs_ctx* c, const char* key, int* va
     if (!c || !key || !value) return set_error(CCCS_EINVAL, "null argument");
     const Schedule& S = c->sched;
     if (std::strcmp(key, "cone_ramp_min") == 0) { *value = S.cone_ramp_min; return CCCS_OK; }
+    if (std::strcmp(key, "pvcs_pass_cones") == 0) { *value = S.pvcs_pass_cones; return CCCS_OK; }
     if (std::strcmp(key, "bl_sample_waves") == 0) { *value = S.bl_sample_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_walk_waves") == 0) { *value = S.bl_walk_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_streams") == 0) { *value = S.bl_streams; return CCCS_OK; }

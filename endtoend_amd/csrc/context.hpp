@@ -99,6 +99,9 @@ struct Schedule {
     // plain-path cone lines whose valid segment has at least this many samples use the
     // segment-sum walk (O(nradii) per destination) instead of the sequential walk (O(line))
     int cone_ramp_min = 64;
+    // cones per pass of the PVCS resample (each pass adds its cones' dose to the volume); 0: from
+    // the size of the beam's REV data
+    int pvcs_pass_cones = 0;
     // upper stored-line lengths of the cone-kernel launch classes (the last class takes the rest)
     int cone_class_len[kConeClasses] = {96, 160, 320, kMaxLine};
     // threads per line (64, 128 or 256) of each class: long lines spread over more waves

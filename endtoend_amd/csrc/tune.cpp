@@ -105,7 +105,8 @@ extern "C" int cccs_autotune(cccs_ctx* ctx, int kind, const cccs_beam* beams, in
                  {"cone_class_len2", {256, 320, 384}},
                  {"cone_class_threads1", {64, 128}},
                  {"cone_class_threads2", {64, 128, 256}},
-                 {"cone_class_threads3", {64, 128, 256}}};
+                 {"cone_class_threads3", {64, 128, 256}},
+                 {"pvcs_pass_cones", {0, 2, 4, 8, 48}}};
     } else {
         knobs = {{"bl_sample_waves", {65536, 131072, 262144, 524288}},
                  {"bl_walk_waves", {16384, 32768, 65536}},

@@ -146,12 +146,32 @@ def test_segment_sum_walk_small(small_case):
     np.testing.assert_array_equal(ramp != 0, seq != 0)
 
 
+def test_pvcs_cone_passes(small_case):
+    """The PVCS resample in passes of 1, 4 and all cones: one cone per pass adds the cones to the
+    volume one at a time, as the reference's per-cone REV2XYZdose does; any grouping stays within
+    FP32 rounding of the others."""
+    name, w, eng = small_case
+    orig = eng.get_schedule("pvcs_pass_cones")
+    try:
+        out = {}
+        for k in (1, 4, 48):
+            eng.set_schedule("pvcs_pass_cones", k)
+            out[k] = eng.beam_dose(w.beams, w.fmaps)
+    finally:
+        eng.set_schedule("pvcs_pass_cones", orig)
+    for k in (4, 48):
+        assert max_err_pct(out[1], out[k]) <= 1e-4
+        np.testing.assert_array_equal(out[1] != 0, out[k] != 0)
+
+
 def test_schedule_rejects_bad_keys(small_case):
     _, _, eng = small_case
     with pytest.raises(Exception):
         eng.set_schedule("no_such_knob", 1)
     with pytest.raises(Exception):
         eng.set_schedule("cone_ramp_min", 0)
+    with pytest.raises(Exception):
+        eng.set_schedule("pvcs_pass_cones", -1)
 
 
 def test_dose_linear_and_additive(small_case):
