@@ -223,8 +223,6 @@ SparsifyManager::~SparsifyManager() {
 void SparsifyManager::init(const std::string& h5filename, HEADER_PATIENT& h, unsigned int num_srworkers,
                            unsigned int, float sparsity_thresh, WRITE_STRATEGY write_strategy) {
     if (m_active) throw std::logic_error("Cannot initialize after activation of SparsifyManager");
-    if (write_strategy == WRITE_STRATEGY::PER_BEAMLET)
-        throw std::invalid_argument("the PER_BEAMLET write strategy is not provided: use CENTRAL");
     if (m_file) {
         (void)cccs_h5_coeff_close(m_file);
         m_file = nullptr;
@@ -261,6 +259,10 @@ void SparsifyManager::init(const std::string& h5filename, HEADER_PATIENT& h, uns
     specs.roi_names = names.empty() ? nullptr : names.data();
     specs.roi_capacities = m_header.row_block_capacities.empty() ? nullptr : m_header.row_block_capacities.data();
     check_h5(cccs_h5_coeff_create(h5filename.c_str(), &p, &specs, -1, 50, &m_file), "SparsifyManager::init");
+    // PER_BEAMLET: each column in its own sidecar file, linked from the central file
+    // (sparsify_manager.cpp:123-174); PER_BEAM writes the central layout, as the reference does
+    if (write_strategy == WRITE_STRATEGY::PER_BEAMLET)
+        check_h5(cccs_h5_coeff_set_write_strategy(m_file, CCCS_H5_WRITE_PER_BEAMLET), "SparsifyManager::init");
 }
 
 void SparsifyManager::activate() {

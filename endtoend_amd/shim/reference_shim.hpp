@@ -46,7 +46,7 @@ struct cccs_h5_coeff_file;
 // (dosecalc-beamlet/main.cu:462-463,479-480,519).  The library hands it finished sparse columns
 // (sparsified on the device), which it writes straight into the coefficient file: there are no
 // dense-column queues or sparsify threads to run.  CENTRAL and PER_BEAM write one file; PER_BEAMLET
-// is rejected (std::invalid_argument).
+// writes each column to its own sidecar file, linked from the central file.
 class SparsifyManager {
   public:
     SparsifyManager() {}

@@ -35,11 +35,13 @@ std::vector<float> read_raw(const std::string& path, size_t n) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s <case dir> <output>\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: %s <case dir> <output> [central|per_beamlet]\n", argv[0]);
         return 2;
     }
     const std::string dir = argv[1], out = argv[2];
+    const bool per_beamlet = argc == 4 && std::string(argv[3]) == "per_beamlet";
+    (void)per_beamlet;  // the beamlet program only
     std::ifstream cf(dir + "/case.txt");
     std::map<std::string, std::vector<double>> kv;
     std::vector<std::vector<double>> beam_lines;
@@ -148,7 +150,8 @@ int main(int argc, char** argv) {
         header.kernel_extent_cm = constants->kernel_extent;
         header.beam_spectrum = mono->spectrum_file;
         header.target_structure = constants->target_structure;
-        sparsifymanager.init(out, header, 4, 1, header.sparsity_thresh, WRITE_STRATEGY::CENTRAL);
+        sparsifymanager.init(out, header, 4, 1, header.sparsity_thresh,
+                             per_beamlet ? WRITE_STRATEGY::PER_BEAMLET : WRITE_STRATEGY::CENTRAL);
         sparsifymanager.activate();
         radconvolveTexture(mono, constants, beams.data(), (int)beams.size(), nrays, 0, gpuid_arr[0], 4);
         sparsifymanager.deactivate();

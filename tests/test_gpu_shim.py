@@ -5,7 +5,8 @@ points in main.cu's order (tests/shim/shim_driver.cpp):
   freeCudaTexture gives a dose bit-identical to cccs_beam_dose of the same beams;
 * dosecalc-beamlet: SparsifyManager::init/activate -> initCudaConstandTex -> radconvolveTexture ->
   deactivate -> freeCudaTexture writes a coefficient file whose columns (keys, values,
-  percentages) are bit-identical to cccs_beamlet_dose's, filed under each BEAM's uid."""
+  percentages) are bit-identical to cccs_beamlet_dose's, filed under each BEAM's uid, with the
+  CENTRAL write strategy and with PER_BEAMLET (each column in a sidecar file behind external links)."""
 import os
 import subprocess
 
@@ -22,9 +23,9 @@ from tests.shim_case import DRIVERS, write_case
 pytestmark = pytest.mark.gpu
 
 
-def _run(kind, case, out):
-    print(f"[case] shim driver {kind}", flush=True)
-    r = subprocess.run([DRIVERS[kind], case, out], capture_output=True, text=True, timeout=120)
+def _run(kind, case, out, *extra):
+    print(f"[case] shim driver {kind} {' '.join(extra)}", flush=True)
+    r = subprocess.run([DRIVERS[kind], case, out, *extra], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
 
 
@@ -44,7 +45,8 @@ def test_beam_entry_points_match_cccs_beam_dose(tmp_path):
 
 
 @pytest.mark.skipif(B.hdf5_root() is None, reason="no HDF5 C library")
-def test_beamlet_entry_points_write_the_library_columns(tmp_path):
+@pytest.mark.parametrize("strategy", ["central", "per_beamlet"])
+def test_beamlet_entry_points_write_the_library_columns(tmp_path, strategy):
     from tests import h5_walk as HW
     ws = [W.small(n=28, voxel=0.5, gantry=g, couch=c, hetero=True, fmap=(6, 6)) for g, c in
           ((30.0, 0.0), (120.0, 20.0), (300.0, -30.0))]
@@ -59,7 +61,7 @@ def test_beamlet_entry_points_write_the_library_columns(tmp_path):
     uids = [4, 5, 9]  # two consecutive uids share a request, the third goes alone
     write_case(str(tmp_path), spec, beams, fmaps, uids=uids)
     out = str(tmp_path / "coeffs.h5")
-    _run("beamlet", str(tmp_path), out)
+    _run("beamlet", str(tmp_path), out, strategy)
     with Engine(spec) as eng:
         cols = eng.beamlet_dose(beams, fmaps, threshold=1e-4, mode=abi.CCCS_BEAMLET_FAITHFUL, percs=True)
     data = HW.walk(out).groups["beams"].groups["data"]
@@ -73,5 +75,10 @@ def test_beamlet_entry_points_write_the_library_columns(tmp_path):
         assert np.array_equal(c.datasets["coeffs"].data.view(np.uint32), v.view(np.uint32))
         assert c.attrs["perc_nonzero"].value == np.float32(pc[0])
         assert c.attrs["perc_dropped"].value == np.float32(pc[1])
+        if strategy == "per_beamlet":   # read above through the link into the column's own file
+            side = f"coeffs.h5_external/beam_{uids[bi]:05d}/beam{uids[bi]:05d}_beamlet{bl:05d}.bmlt"
+            assert HW.external_link(out, f"/beams/data/beam_{uids[bi]:05d}/beamlet_{bl:05d}/coeffs") == \
+                (side, "/coeffs")
+            assert os.path.exists(os.path.join(str(tmp_path), side))
         n += 1
     assert n == sum(int((f > 0).sum()) for f in fmaps) > 20
