@@ -236,6 +236,11 @@ __device__ __forceinline__ float kc_at(float r, const float4* __restrict__ tab, 
     return iv.y + (r - iv.x) * iv.z;
 }
 
+// The grid and batch fields the mapping reads sit in LDS; this compiler-only barrier keeps their
+// reads inside each pass iteration (next to their use) instead of holding ~50 of them in VGPRs
+// across the pass.
+__device__ __forceinline__ void lds_reload_fence() { __asm__ volatile("" ::: "memory"); }
+
 constexpr int kStagePillars = 64;  // footprint pillars of one line whose beamlet data sits in LDS
 
 // Stage 1 of a cone direction, one wave per REV line: pillar mapping of every stored sample and
@@ -256,6 +261,10 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     __shared__ uint32_t s_pil[kPillarBits / 32];  // pillars with footprint samples on this line
     __shared__ uint16_t s_rank[kPillarBits / 32]; // footprint pillars in the words before each word
     __shared__ PillarRec s_rec[kStagePillars];    // their beamlet data, in pillar order
+    __shared__ BlConeDev s_cone;                  // the line's grid and batch records
+    __shared__ BatchDev s_batch;
+    static_assert(sizeof(BlConeDev) % 8 == 0 && sizeof(BatchDev) % 8 == 0 &&
+                      (sizeof(BlConeDev) + sizeof(BatchDev)) / 8 <= 64, "one 8-byte word per lane");
     const int tid = threadIdx.x;
     const uint64_t lt = (1ull << tid) - 1ull;
     uint32_t stored = 0;
@@ -263,12 +272,23 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     // collisions on the counters)
     const uint32_t sq = blockIdx.x % kSubQueues;
     TermaReq* qs = q + (size_t)sq * subcap;
+    uint32_t staged = 0xffffffffu;  // grid whose records sit in LDS
     for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
     const LineMeta meta = a.lines[line];
     const uint32_t ci = a.line_cone[line];
     if (meta.len <= 0) continue;
-    const BlConeDev& c = a.cones[ci];
-    const BatchDev& B = a.batches[ci];  // grid ci of a direction belongs to batch ci
+    // grid ci of a direction belongs to batch ci; both records go to LDS when the line's grid
+    // changes, so the passes below read their fields there instead of from global memory
+    if (ci != staged) {
+        constexpr int nc = (int)(sizeof(BlConeDev) / 8), nb = (int)(sizeof(BatchDev) / 8);
+        if (tid < nc) reinterpret_cast<uint64_t*>(&s_cone)[tid] = reinterpret_cast<const uint64_t*>(a.cones + ci)[tid];
+        else if (tid < nc + nb)
+            reinterpret_cast<uint64_t*>(&s_batch)[tid - nc] = reinterpret_cast<const uint64_t*>(a.batches + ci)[tid - nc];
+        staged = ci;
+        __syncthreads();
+    }
+    const BlConeDev& c = s_cone;
+    const BatchDev& B = s_batch;
     const uint64_t l = line - c.line0;
     const int tY = (int)(l % (uint64_t)c.ny), tZ = (int)(l / (uint64_t)c.ny);
     const int i0 = meta.i0, m = meta.len;
@@ -309,6 +329,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     bool any = false;
     uint32_t nfoot = 0;  // footprint samples of the line: exactly its TERMA requests below
     for (int r = 0; r < ncand; r += 64) {
+        lds_reload_fence();
         bool fp = false;
         if (r + tid < ncand) {
             const int j = s_list[r + tid];
@@ -316,8 +337,12 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             v3 inp;
             if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
                 fabsf(inp.z - B.fc[1]) <= B.fw) {
+                // point_fluence(...) > 0 without its map load: a job beamlet's own cell has a
+                // positive weight (plan_beam keeps only those), so the cell test decides alone
                 const PillarRec bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
-                fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
+                const v3 tp = bl_terma_point(a, B, bl, inp);
+                lds_reload_fence();
+                fp = fluence_cell(tp, B.beam) == bl.fidx;
                 if (fp) {
                     s_fp[j] = 1;
                     if (track) atomicOr(&s_pil[pidx >> 5], 1u << (pidx & 31));
@@ -416,43 +441,46 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     int nlo = m, nhi = -1;  // first and last sample that is not a wall
     uint32_t qdone = 0;
     for (int r = 0; r < nex; r += 64) {
-        float h = -1.f, t = 0.f;
-        bool need = false;
-        TermaReq rq;
-        int j = 0;
+        lds_reload_fence();
+        bool ok = false, need = false;
+        int j = 0, pidx = 0;
+        v3 inp;
+        PillarRec bl;
         if (r + tid < nex) {
             j = s_list[r + tid];
-            int pidx;
-            v3 inp;
             if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) &&
                 (!track || ((s_pil[pidx >> 5] >> (pidx & 31)) & 1u))) {
-                PillarRec bl;
+                ok = true;
                 const uint32_t slot = track ? (uint32_t)s_rank[pidx >> 5] +
                                                   (uint32_t)__popc(s_pil[pidx >> 5] & ((1u << (pidx & 31)) - 1u))
                                             : (uint32_t)kStagePillars;
                 if (slot < (uint32_t)kStagePillars) bl = s_rec[slot];
                 else bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
-                h = 0.5f * (a.delr * bl_density(a, B, bl, inp));
-                if (s_fp[j] & 1) {
-                    const v3 gcs = bl_terma_point(a, B, bl, inp);
-                    need = true;
-                    rq.slot = (uint32_t)(meta.off + j);
-                    rq.beamlet = (uint32_t)(B.beamlet0 + pidx);
-                    rq.x = gcs.x; rq.y = gcs.y; rq.z = gcs.z;
-                }
+                need = (s_fp[j] & 1) != 0;
             }
+        }
+        // the TERMA requests first, then the density: the two rotations are not live together
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            if (need) {
+                const uint32_t pos = qbase + qdone + (uint32_t)__popcll(mask & lt);
+                const v3 gcs = bl_terma_point(a, B, bl, inp);
+                TermaReq rq;
+                rq.slot = (uint32_t)(meta.off + j);
+                rq.beamlet = (uint32_t)(B.beamlet0 + pidx);
+                rq.x = gcs.x; rq.y = gcs.y; rq.z = gcs.z;
+                if (pos < subcap) qs[pos] = rq;
+            }
+            qdone += (uint32_t)__popcll(mask);
+        }
+        if (r + tid < nex) {
+            const float h = ok ? 0.5f * (a.delr * bl_density(a, B, bl, inp)) : -1.f;
             Hl[j] = h;
-            T[j] = t;
+            T[j] = 0.f;
             if (h >= 0.f) {
                 nlo = min(nlo, j);
                 nhi = max(nhi, j);
             }
-        }
-        const uint64_t mask = __ballot(need);
-        if (mask) {
-            const uint32_t pos = qbase + qdone + (uint32_t)__popcll(mask & lt);
-            if (need && pos < subcap) qs[pos] = rq;
-            qdone += (uint32_t)__popcll(mask);
         }
     }
     // The reservation counted footprint samples of pass A.2; pass B.2 requests TERMA for those of

@@ -293,10 +293,9 @@ struct SpecDev {
     float flu[kMaxSpec], en[kMaxSpec], mu[kMaxSpec];
 };
 
-// Fluence weight seen by point p: the beamlet-cell lookup of cudaSiddon_device
-// (cudaSiddon.cu:146-165).  0 outside the map, for a zero-weight cell, or (beamletnum >= 0) for
-// any cell other than beamletnum.
-CCCS_HD float point_fluence(v3 p, const float* __restrict__ fmap, const BeamDev& b, int beamletnum) {
+// Fluence-map cell seen by point p, -1 outside the map: the index part of the beamlet-cell lookup
+// of cudaSiddon_device (cudaSiddon.cu:146-165).
+CCCS_HD int fluence_cell(v3 p, const BeamDev& b) {
     v3 diff = p - b.src;
     v3 projection = dot3(diff, b.dir_n) * b.dir_n;
     v3 rejection = diff - projection;
@@ -305,9 +304,16 @@ CCCS_HD float point_fluence(v3 p, const float* __restrict__ fmap, const BeamDev&
     float fx = (pf.x / b.bsx) + ((float)b.fmx * 0.5f);
     float fz = (pf.z / b.bsy) + ((float)b.fmy * 0.5f);
     // a NaN coordinate (voxel on the central axis) passes these tests, as in the reference
-    if (fx < 0 || fx >= (float)b.fmx || fz < 0 || fz >= (float)b.fmy) return 0.f;
+    if (fx < 0 || fx >= (float)b.fmx || fz < 0 || fz >= (float)b.fmy) return -1;
     int fidx = f2i_rd(fx) + b.fmx * f2i_rd(fz);
-    if (fidx < 0 || fidx >= b.fmx * b.fmy || (beamletnum >= 0 && fidx != beamletnum)) return 0.f;
+    return (fidx < 0 || fidx >= b.fmx * b.fmy) ? -1 : fidx;
+}
+
+// Fluence weight seen by point p (cudaSiddon.cu:146-165): 0 outside the map, for a zero-weight
+// cell, or (beamletnum >= 0) for any cell other than beamletnum.
+CCCS_HD float point_fluence(v3 p, const float* __restrict__ fmap, const BeamDev& b, int beamletnum) {
+    const int fidx = fluence_cell(p, b);
+    if (fidx < 0 || (beamletnum >= 0 && fidx != beamletnum)) return 0.f;
     float fluence = fmap[fidx];
     return fluence <= 0.f ? 0.f : fluence;
 }

@@ -4,7 +4,7 @@
 // (plan_beam, build_job: this file includes endtoend_amd/csrc/beamlet.hip), computes every REV
 // line's stored interval and store offset with the interval kernel's own function, and replays
 // the sample kernel's passes A.1-B.2 line by line on the CPU with the kernel's own helpers
-// (bl_locate, rev_to_packed, bl_terma_point, point_fluence).  Every index the kernel forms is
+// (bl_locate, rev_to_packed, bl_terma_point, fluence_cell; point_fluence checks the cell test).  Every index the kernel forms is
 // checked against the bound of the array it addresses:
 //   LDS:    s_fp / s_list (< 1024 and < the line's stored length), s_pil / s_rank words (< 128),
 //           staged pillar slots (< kStagePillars), the affine pillar index read in pass B.1;
@@ -108,7 +108,11 @@ void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t n
             AUDIT((size_t)(B.beamlet0 + pidx) < nbeamlets, "beamlet record beyond the job's beamlets");
             if (pidx < 0 || (size_t)(B.beamlet0 + pidx) >= nbeamlets) continue;
             const PillarRec bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
-            const bool fp = point_fluence(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, bl.fidx) > 0.f;
+            // the kernel's cell test, and the fluence test it stands for (a job beamlet's own cell
+            // has a positive weight)
+            const v3 tp = bl_terma_point(a, B, bl, inp);
+            const bool fp = fluence_cell(tp, B.beam) == bl.fidx;
+            AUDIT(fp == (point_fluence(tp, B.fmap, B.beam, bl.fidx) > 0.f), "cell test differs from the fluence test");
             if (fp) {
                 s_fp[j] = 1;
                 nfoot++;
