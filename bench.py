@@ -473,6 +473,9 @@ def main():
                     help="run the schedule autotuner on a sample of each workload before timing it")
     ap.add_argument("--bl-streams", type=int, default=0,
                     help="cone directions in flight for the whole run (profiling runs: 1); 0 = the library default")
+    ap.add_argument("--bl-sample-waves", type=int, default=0,
+                    help="sample-stage waves for the whole run (profiling runs: 262144, as in the one-stream "
+                         "pass); 0 = the library default")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the streaming copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -511,6 +514,8 @@ def main():
     else:
         if args.bl_streams > 0:
             eng.set_schedule("bl_streams", args.bl_streams)
+        if args.bl_sample_waves > 0:
+            eng.set_schedule("bl_sample_waves", args.bl_sample_waves)
         schedule = {k: eng.get_schedule(k) for k in BEAMLET_KNOBS}
 
     def step():
@@ -526,9 +531,14 @@ def main():
     t1 = time.perf_counter()
     dt = t1 - t0
     # per-kernel times: one step with one cone direction in flight, so that no two kernels share
-    # the device and the kernel times add up to no more than that step
+    # the device and the kernel times add up to no more than that step.  Alone on the device the
+    # sample stage runs at its standalone best, 4 waves per SIMD (262144 waves); the headline
+    # keeps it at 2 waves per SIMD so that the other directions' stages run beside it.
     nstreams = eng.get_schedule("bl_streams")
-    eng.set_schedule("bl_streams", 1)
+    swaves = eng.get_schedule("bl_sample_waves")
+    alone = {"bl_streams": 1, "bl_sample_waves": 262144}
+    for k, v in alone.items():
+        eng.set_schedule(k, v)
     step()
     eng.sync()
     eng.profile(True)
@@ -540,6 +550,7 @@ def main():
     prof = eng.profile_read()
     eng.profile(False)
     eng.set_schedule("bl_streams", nstreams)
+    eng.set_schedule("bl_sample_waves", swaves)
     ms_per_step = 1e3 * dt / args.steps
     value = args.steps * nbl / dt
 
@@ -563,7 +574,7 @@ def main():
                    "path": "batched", "beamlet_doses_per_step": nbl, "beams_per_step": len(w.beams), "seed": w.seed,
                    "schedule": schedule},
         "roofline": roofline(prof),
-        "kernels_one_stream": {"ms_per_step": one_stream_ms, "kernels": _kernels(prof)},
+        "kernels_one_stream": {"ms_per_step": one_stream_ms, "schedule": alone, "kernels": _kernels(prof)},
     }
     if not args.no_extras:
         # columns copied to the host through the native counting sink (PCIe-inclusive)

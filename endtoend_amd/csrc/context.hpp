@@ -106,10 +106,11 @@ struct Schedule {
     int cone_class_len[kConeClasses] = {96, 160, 320, kMaxLine};
     // threads per line (64, 128 or 256) of each class: long lines spread over more waves
     int cone_class_threads[kConeClasses] = {64, 64, 128, 128};
-    // waves of the beamlet path's persistent per-line kernels (each takes a stride of lines); the
-    // sample stage's order of lines also orders the TERMA queue, and more waves there give the
-    // Siddon stage more coherent neighbouring requests
-    int bl_sample_waves = 262144;
+    // waves of the beamlet path's persistent per-line kernels (each takes a stride of lines).  The
+    // sample stage holds 2 waves per SIMD: with several cone directions in flight, the other
+    // streams' Siddon, walk and gather waves run beside it (its own 4-wave occupancy would take
+    // every register of the SIMD)
+    int bl_sample_waves = 2048;
     int bl_walk_waves = 32768;
     // the beamlet walk sizes its LDS for the longest line of its launch; it runs one launch per
     // length class: lines up to this many stored samples, up to twice that, ..., and the rest
