@@ -127,7 +127,7 @@ def cpu_baseline(w, eng, nbeamlets: int, runs: int = 5):
 
 BEAM_KNOBS = ("cone_ramp_min", "cone_class_len0", "cone_class_len1", "cone_class_len2", "cone_class_threads1",
               "cone_class_threads2", "cone_class_threads3", "pvcs_pass_cones")
-BEAMLET_KNOBS = ("bl_sample_waves", "bl_walk_waves", "bl_walk_split", "bl_streams")
+BEAMLET_KNOBS = ("bl_sample_waves", "bl_walk_waves", "bl_walk_split", "bl_streams", "bl_job_gb")
 
 
 def plain_beam_path(steps: int = 2, autotune: bool = False):

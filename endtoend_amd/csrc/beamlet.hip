@@ -567,9 +567,16 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
     const float rmax = sTab[0][nr - 1].w, rlast = sTab[0][nr - 2].w;
     const float nphi_f = (float)a.nphi;
     uint32_t iters = 0;
-    for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
-        const LineMeta meta = a.lines[line];
-        if (meta.len < len_lo || meta.len > maxn) continue;  // empty, or another launch's line
+    const uint64_t G = gridDim.x;
+    for (uint64_t base = xcd_remap(blockIdx.x, gridDim.x); base < a.nlines; base += 64 * G) {
+    LineMeta mine{};
+    if (base + (uint64_t)tid * G < a.nlines) mine = a.lines[base + (uint64_t)tid * G];
+    // empty lines and other launches' lines are skipped here
+    uint64_t todo = __ballot(mine.len >= len_lo && mine.len <= maxn);
+    while (todo) {
+        const int kl = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1ull;
+        const LineMeta meta = line_of_lane(mine, kl);
         const int m = meta.len;
         float* T = a.rev + meta.off;
         const float* Hl = H + meta.off;
@@ -676,6 +683,7 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
             T[j] = ej.x >= 0.f ? sum / nphi_f : 0.f;
         }
         __syncthreads();  // the next line reuses the LDS arrays
+    }
     }
     if (a.counts) wave_count(iters, a.counts, blockIdx.x);
 }
@@ -2179,7 +2187,7 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         return CCCS_OK;
     }
     const size_t nvox = (size_t)p.bbox_size[0] * p.bbox_size[1] * p.bbox_size[2];
-    const size_t budget = size_t(24) << 30;  // device bytes per job
+    const size_t budget = size_t(ctx->sched.bl_job_gb) << 30;  // device bytes per job
     const int nset = std::min(std::max(ctx->sched.bl_streams, 1), kRaySets);
     size_t next = 0;
     // device footprint of each batch, additive per cone direction: a job needs its batches' packed

@@ -370,6 +370,11 @@ extern "C" int cccs_ctx_set_schedule(cccs_ctx* c, const char* key, int value) {
         c->sched.cone_ramp_min = value;
         return CCCS_OK;
     }
+    if (std::strcmp(key, "bl_job_gb") == 0) {
+        if (value < 1 || value > 1024) return set_error(CCCS_EINVAL, "bl_job_gb must be in [1, 1024]");
+        c->sched.bl_job_gb = value;
+        return CCCS_OK;
+    }
     if (std::strcmp(key, "bl_streams") == 0) {
         if (value < 1 || value > kRaySets) return set_error(CCCS_EINVAL, "bl_streams must be in [1, 4]");
         c->sched.bl_streams = value;
@@ -413,6 +418,7 @@ extern "C" int cccs_ctx_get_schedule(const cccs_ctx* c, const char* key, int* va
     if (std::strcmp(key, "bl_sample_waves") == 0) { *value = S.bl_sample_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_walk_waves") == 0) { *value = S.bl_walk_waves; return CCCS_OK; }
     if (std::strcmp(key, "bl_streams") == 0) { *value = S.bl_streams; return CCCS_OK; }
+    if (std::strcmp(key, "bl_job_gb") == 0) { *value = S.bl_job_gb; return CCCS_OK; }
     if (std::strcmp(key, "bl_walk_split") == 0) { *value = S.bl_walk_split; return CCCS_OK; }
     for (int i = 0; i < kConeClasses; i++) {
         if (("cone_class_threads" + std::to_string(i)) == key) { *value = S.cone_class_threads[i]; return CCCS_OK; }

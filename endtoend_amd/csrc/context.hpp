@@ -112,6 +112,9 @@ struct Schedule {
     // every register of the SIMD)
     int bl_sample_waves = 2048;
     int bl_walk_waves = 32768;
+    // device memory of one beamlet job [GiB]: batches join a job while its packed grids, output
+    // bounds and per-direction buffers fit; fewer, larger jobs drain the direction pipeline less often
+    int bl_job_gb = 12;
     // the beamlet walk sizes its LDS for the longest line of its launch; it runs one launch per
     // length class: lines up to this many stored samples, up to twice that, ..., and the rest
     // (0: one launch)

@@ -49,6 +49,20 @@ struct LineMeta {
 };
 static_assert(sizeof(LineMeta) == 8, "line table entries are 8 bytes");
 
+// Persistent per-line kernels walk lines base, base + G, base + 2G, ... (G = grid size), so that
+// neighbouring waves work on neighbouring lines at the same time.  A wave loads the records of
+// its next 64 lines at once, one per lane (lane k: line base + k*G), and takes each from its lane:
+// one round trip per 64 lines instead of one per line, and lines it skips cost no memory access.
+__device__ __forceinline__ LineMeta line_of_lane(const LineMeta& m, int lane) {
+    const uint32_t w = (uint32_t)(uint16_t)m.i0 | ((uint32_t)(uint16_t)m.len << 16);
+    const uint32_t ow = (uint32_t)__shfl((int)w, lane, 64);
+    LineMeta r;
+    r.off = (uint32_t)__shfl((int)m.off, lane, 64);
+    r.i0 = (int16_t)(ow & 0xffffu);
+    r.len = (int16_t)(ow >> 16);
+    return r;
+}
+
 // Trilinear fetch from a compactly stored REV grid (nx samples per line, ny x nz lines) at texture
 // coordinates (u, v, w) = (sample, line y, line z), texel centres at +0.5, border 0: the same
 // operation sequence as tex3_border on the dense grid, so both give identical results.

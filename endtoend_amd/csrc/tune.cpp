@@ -111,7 +111,8 @@ extern "C" int cccs_autotune(cccs_ctx* ctx, int kind, const cccs_beam* beams, in
         knobs = {{"bl_sample_waves", {1024, 2048, 4096, 65536, 262144}},
                  {"bl_walk_waves", {16384, 32768, 65536}},
                  {"bl_walk_split", {0, 128, 256, 384}},
-                 {"bl_streams", {1, 2, 3, 4}}};
+                 {"bl_streams", {1, 2, 3, 4}},
+                 {"bl_job_gb", {8, 12, 24, 48}}};
     }
     Trial t{ctx, kind, beams, nbeams, fluence};
     int st = CCCS_OK;
