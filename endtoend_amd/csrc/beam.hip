@@ -237,6 +237,14 @@ __global__ void __launch_bounds__(256) crop_final_kernel(const float* __restrict
     if (t < 7) out[t] = s[t][0];
 }
 
+// A class-list entry: the line, its cone and its record, so that a workgroup starts from one
+// 16-byte load instead of a list load, a record load and a search over the cones.
+struct ConeLine {
+    uint32_t line;
+    uint32_t cone;
+    LineMeta meta;
+};
+
 struct ConeArgs {
     const float2* dt;         // {density, TERMA (0 outside the calc_bbox)} over the CT grid
     const float* tables;      // kernel | radii | inverse widths
@@ -251,7 +259,7 @@ struct ConeArgs {
     int ramp_min;             // valid segments at least this long use the segment-sum walk
     int maxn;                 // longest stored line of this launch (LDS sizing)
     int len_lo, len_hi;       // this launch takes the lines whose stored length is in [len_lo, len_hi]
-    const uint32_t* list;     // the lines of this launch (one workgroup each), or nullptr: every line
+    const ConeLine* list;     // the lines of this launch (one workgroup each), or nullptr: every line
     VolDesc vd;
     m33 binv;                 // PVCS <- BEV rotation (index units)
     v3 off;                   // (bevStart - start) / voxel
@@ -503,13 +511,24 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
     __shared__ int s_red[NW][3];
     __shared__ uint32_t s_cnt[NW];
     const uint64_t slot = xcd_remap(blockIdx.x, gridDim.x);
-    const uint64_t line = a.list ? (uint64_t)a.list[slot] : slot;
+    uint64_t line;
+    uint32_t cone;
+    LineMeta meta;
+    if (a.list) {
+        const ConeLine e = a.list[slot];
+        line = e.line;
+        cone = e.cone;
+        meta = e.meta;
+    } else {
+        line = slot;
+        meta = a.lines[line];
+        cone = (uint32_t)cone_of_line(a.cones, a.ncones, line);
+    }
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const LineMeta meta = a.lines[line];
     const bool mine = meta.len >= a.len_lo && meta.len <= a.len_hi;
     uint32_t iters = 0;
     if (mine) {
-        const ConeDev& c = a.cones[cone_of_line(a.cones, a.ncones, line)];
+        const ConeDev& c = a.cones[cone];
         const int nr = a.nradii;
         const int m = meta.len;
         const bool ramp_lds = a.ramp_min <= a.maxn;
@@ -520,13 +539,16 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
         float* rc = reinterpret_cast<float*>(rE2 + n1);
         float2* s_line = ramp_lds ? reinterpret_cast<float2*>(rc + nc) : reinterpret_cast<float2*>(s_dyn);
         uint16_t* rlb = reinterpret_cast<uint16_t*>(s_line + a.maxn);
+        // the kernel rows' loads are issued before the resample's and land in LDS after it, so
+        // their latency overlaps the density / TERMA fetches
+        float tR = 0.f, tIW = 0.f, tF = 0.f, tB = 0.f;
         if (tid < nr) {
             const float* K = a.tables;
             const float* R = a.tables + (size_t)a.ntheta * nr;
-            sR[tid] = R[tid];
-            sIW[tid] = R[nr + tid];
-            sF[tid] = K[(size_t)c.rowf * nr + tid];
-            sB[tid] = K[(size_t)c.rowb * nr + tid];
+            tR = R[tid];
+            tIW = R[nr + tid];
+            tF = K[(size_t)c.rowf * nr + tid];
+            tB = K[(size_t)c.rowb * nr + tid];
         }
         const uint64_t l = line - c.line0;
         const int tY = (int)(l % (uint64_t)c.ny);
@@ -548,6 +570,12 @@ __global__ void __launch_bounds__(NT) cone_kernel(ConeArgs a) {
                 }
             }
             s_line[j] = e;
+        }
+        if (tid < nr) {
+            sR[tid] = tR;
+            sIW[tid] = tIW;
+            sF[tid] = tF;
+            sB[tid] = tB;
         }
         __syncthreads();
         // valid segment of the line (contiguous: the crop box and the calc_bbox are convex)
@@ -1098,7 +1126,9 @@ struct ClassArgs {
     int ncls;
     uint32_t* bcnt;        // [blocks][kConeClasses]: counts, then the blocks' list offsets
     uint32_t* tot;         // [kConeClasses]: lines per class
-    uint32_t* list;        // line indices, class by class
+    ConeLine* list;        // lines with their cone and record, class by class
+    const ConeDev* cones;  // the beam's cones (the entries' cone indices)
+    int ncones;
     uint32_t* counts;      // profiling: the slots of empty lines are zeroed here (or nullptr)
 };
 
@@ -1169,7 +1199,11 @@ __global__ void __launch_bounds__(kClassBlock) class_scatter_kernel(ClassArgs a)
     if (cls >= 0) {
         uint32_t off = a.bcnt[(size_t)blockIdx.x * kConeClasses + cls] + rank;
         for (int q = 0; q < w; q++) off += s[q][cls];
-        a.list[off] = (uint32_t)i;
+        ConeLine e;
+        e.line = (uint32_t)i;
+        e.cone = (uint32_t)cone_of_line(a.cones, a.ncones, i);
+        e.meta = a.lines[i];
+        a.list[off] = e;
     }
 }
 
@@ -1290,6 +1324,8 @@ int run_cones(BeamScratch& S, const cccs_beam& b, const float* d_terma, float* d
     for (int c = 0; c < kConeClasses; c++) cl.hi[c] = c < ncls ? cls_hi[c] : 0;
     cl.ncls = ncls;
     cl.counts = ca.counts;
+    cl.cones = d_cones;
+    cl.ncones = g.nrays;
     const uint64_t ncb = (g.total_lines + kClassBlock - 1) / kClassBlock;
     CCCS_TRY(S.get(ctx->cls_cnt, (size_t)ncb * kConeClasses + kConeClasses, &cl.bcnt));
     cl.tot = cl.bcnt + (size_t)ncb * kConeClasses;
