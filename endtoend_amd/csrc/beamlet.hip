@@ -2201,6 +2201,11 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
     // columns of the last finished job, waiting in pinned host memory for the sink
     for (hipEvent_t& e : ctx->stage_ev)
         if (!e) CCCS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (sink && !ctx->cstream) {
+        CCCS_HIP(hipStreamCreateWithFlags(&ctx->cstream, hipStreamNonBlocking));
+        CCCS_HIP(hipEventCreateWithFlags(&ctx->copy_ev, hipEventDisableTiming));
+    }
+    hipEvent_t copy_done = nullptr;  // the last job's column copy (its buffers are rewritten after it)
     Delivery pending;
     int njob = 0;
     auto deliver = [&]() -> int {
@@ -2286,6 +2291,7 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
         ca.off = offs;
         ca.keys = nullptr;
         ca.vals = nullptr;
+        if (copy_done) CCCS_HIP(hipStreamWaitEvent(ctx->stream, copy_done, 0));
         int tok = prof_begin(ctx, K_BL_UNPACK);
         hipLaunchKernelGGL(bl_colscan_kernel, dim3(nbl), dim3(256), 0, ctx->stream, ca);
         prof_end(ctx, tok, 0.0, 0.0, (uint64_t)nbl);
@@ -2362,16 +2368,22 @@ extern "C" int cccs_beamlet_dose(cccs_ctx* ctx, const cccs_beam* beams, int nbea
             const BeamletDev& d = J.beamlets[q];
             D.cols[(size_t)q] = {plans[sel[d.batch].plan].beam, d.fidx};
         }
-        CCCS_HIP(hipMemcpyAsync(D.stat, stat, sbytes, hipMemcpyDeviceToHost, ctx->stream));
-        if (nroi > 0) CCCS_HIP(hipMemcpyAsync(D.rcnt, rcnt, rbytes, hipMemcpyDeviceToHost, ctx->stream));
+        // the copies run on their own stream: the next job's cone passes, queued behind this
+        // compaction, do not wait for them; the next compaction waits before rewriting the buffers
+        hipStream_t cs = ctx->cstream;
+        CCCS_HIP(hipEventRecord(ctx->copy_ev, ctx->stream));
+        CCCS_HIP(hipStreamWaitEvent(cs, ctx->copy_ev, 0));
+        CCCS_HIP(hipMemcpyAsync(D.stat, stat, sbytes, hipMemcpyDeviceToHost, cs));
+        if (nroi > 0) CCCS_HIP(hipMemcpyAsync(D.rcnt, rcnt, rbytes, hipMemcpyDeviceToHost, cs));
         if (nout) {
             CCCS_HIP(hipMemcpyAsync(D.keys, nroi > 0 ? ctx->roi_keys.p : ctx->bl_keys.p, kbytes, hipMemcpyDeviceToHost,
-                                    ctx->stream));
+                                    cs));
             CCCS_HIP(hipMemcpyAsync(D.vals, nroi > 0 ? ctx->roi_vals.p : ctx->bl_vals.p, vbytes, hipMemcpyDeviceToHost,
-                                    ctx->stream));
+                                    cs));
         }
-        CCCS_HIP(hipEventRecord(ctx->stage_ev[njob & 1], ctx->stream));
+        CCCS_HIP(hipEventRecord(ctx->stage_ev[njob & 1], cs));
         D.ev = ctx->stage_ev[njob & 1];
+        copy_done = D.ev;
         D.live = true;
         njob++;
     }

@@ -320,6 +320,7 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipStream_t x : c->xstream)
         if (x) (void)hipStreamSynchronize(x);
+    if (c->cstream) (void)hipStreamSynchronize(c->cstream);
     if (c->d_dens) (void)hipFree(c->d_dens);
     if (c->d_tables) (void)hipFree(c->d_tables);
     c->terma.release(); c->fmap.release(); c->partial.release(); c->rev.release(); c->cones.release();
@@ -332,6 +333,8 @@ extern "C" int cccs_ctx_destroy(cccs_ctx* c) {
     c->h_small.release(); c->h_cones.release(); c->h_stage.release();
     for (hipEvent_t& e : c->stage_ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->copy_ev) (void)hipEventDestroy(c->copy_ev);
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
     c->bl_tab.release(); c->bl_pdose.release(); c->bl_lcone.release(); c->bl_rowcnt.release(); c->bl_arena.release();
     c->bl_cnt.release(); c->bl_keys.release(); c->bl_vals.release(); c->bl_fmap.release();
     c->bl_h.release(); c->bl_queue.release(); c->bl_qcnt.release(); c->bl_tiles.release();
@@ -526,6 +529,7 @@ extern "C" int cccs_ctx_sync(cccs_ctx* ctx) {
     CCCS_HIP(hipSetDevice(ctx->device));
     CCCS_HIP(hipStreamSynchronize(ctx->stream));
     for (hipStream_t x : ctx->xstream) CCCS_HIP(hipStreamSynchronize(x));
+    if (ctx->cstream) CCCS_HIP(hipStreamSynchronize(ctx->cstream));
     return CCCS_OK;
 }
 

@@ -153,6 +153,8 @@ struct cccs_ctx {
         xqueue[cccs::kRaySets - 1], xscan[cccs::kRaySets - 1], xlcone[cccs::kRaySets - 1];
     cccs::HostBuf h_small, h_cones, h_stage;  // h_stage: pinned double buffer of download_staged
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    hipStream_t cstream = nullptr;  // beamlet columns to pinned host memory, beside the next job
+    hipEvent_t copy_ev = nullptr;
     // beamlet path: job tables, packed grids, column arena, sparse output, line grids
     cccs::DevBuf bl_tab, bl_pdose, bl_arena, bl_cnt, bl_keys, bl_vals, bl_fmap, bl_h, bl_queue, bl_qcnt, bl_tiles,
         bl_lcone, bl_rowcnt;
