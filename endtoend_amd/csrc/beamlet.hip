@@ -533,6 +533,14 @@ __global__ void __launch_bounds__(256) bl_siddon_kernel(BlArgs a, const TermaReq
     if (counts) block_count(steps, counts, blockIdx.x);
 }
 
+// A 64-bit value from lane addr / 4 (ds_bpermute on both halves).
+__device__ __forceinline__ double bperm_f64(int addr, double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Stage 3, one wave per non-empty REV line: PackRowConvolve (nvbbRayConvolve_device.cu:284-417)
 // for every sample.  Both walks of destination j run inside the pillar segment holding j, start at
 // j itself (the self sample adds only its first half step) and stop when the radiological
@@ -576,8 +584,13 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
     while (todo) {
         const int kl = __ffsll((unsigned long long)todo) - 1;
         todo &= todo - 1ull;
-        const LineMeta meta = line_of_lane(mine, kl);
-        const int m = meta.len;
+        LineMeta meta = line_of_lane(mine, kl);
+        meta.off = __builtin_amdgcn_readfirstlane(meta.off);  // uniform: scalar registers
+        const int m = __builtin_amdgcn_readfirstlane((int)meta.len);
+        // the scans' lane arithmetic is redone per line from an opaque lane id: hoisted out of
+        // the line loop, it held some 20 VGPRs for the whole kernel
+        int ln = tid;
+        __asm__ volatile("" : "+v"(ln));
         float* T = a.rev + meta.off;
         const float* Hl = H + meta.off;
         for (int j = tid; j < m; j += 64) sL[j] = make_float2(Hl[j], T[j]);
@@ -598,18 +611,19 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
         double pc = csum;
         int pre = last_wall, suf = first_wall, pf = nfp;
         for (int o = 1; o < 64; o <<= 1) {
-            const double uc = __shfl_up(pc, o, 64);
-            const int up = __shfl_up(pre, o, 64);
-            const int uf = __shfl_up(pf, o, 64);
-            const int dn = __shfl_down(suf, o, 64);
+            const int iu = max(ln - o, 0) << 2, id = min(ln + o, 63) << 2;
+            const double uc = bperm_f64(iu, pc);
+            const int up = __builtin_amdgcn_ds_bpermute(iu, pre);
+            const int uf = __builtin_amdgcn_ds_bpermute(iu, pf);
+            const int dn = __builtin_amdgcn_ds_bpermute(id, suf);
             if (tid >= o) { pc += uc; pre = max(pre, up); pf += uf; }
             if (tid + o < 64) suf = min(suf, dn);
         }
         double cacc = pc - csum;
-        int left = __shfl_up(pre, 1, 64);
-        int right = __shfl_down(suf, 1, 64);
+        int left = __builtin_amdgcn_ds_bpermute(max(ln - 1, 0) << 2, pre);
+        int right = __builtin_amdgcn_ds_bpermute(min(ln + 1, 63) << 2, suf);
         int fpos = pf - nfp;
-        const int nfoot = __shfl(pf, 63, 64);
+        const int nfoot = __builtin_amdgcn_readlane(pf, 63);
         if (tid == 0) left = -1;
         if (tid == 63) right = m;
         for (int k = ca; k < cb; k++) {
