@@ -265,14 +265,17 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     __shared__ BatchDev s_batch;
     static_assert(sizeof(BlConeDev) % 8 == 0 && sizeof(BatchDev) % 8 == 0 &&
                       (sizeof(BlConeDev) + sizeof(BatchDev)) / 8 <= 64, "one 8-byte word per lane");
-    const int tid = threadIdx.x;
-    const uint64_t lt = (1ull << tid) - 1ull;
     uint32_t stored = 0;
     uint32_t staged = 0xffffffffu;  // grid whose records sit in LDS
     for (uint64_t line = xcd_remap(blockIdx.x, gridDim.x); line < a.nlines; line += gridDim.x) {
     const LineMeta meta = a.lines[line];
     const uint32_t ci = a.line_cone[line];
     if (meta.len <= 0) continue;
+    // the lane id is opaque per line, so that the lane arithmetic is redone per line rather than
+    // hoisted out of the line loop into registers held for the whole kernel
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));
+    const uint64_t lt = (1ull << tid) - 1ull;
     // each wave walks a stride of lines; a line's TERMA requests go to one of kSubQueues queues,
     // chosen by its run of 64 consecutive lines: the Siddon stage then marches neighbouring
     // lines' rays side by side (shared cache lines; -26 % stage time against one queue per wave),
