@@ -182,19 +182,43 @@ CCCS_HD float fmod_pos(float x, float y) {
     return r;
 }
 
+// floorf(x / y) of the correctly rounded quotient, for 0 <= x < 2^24 and a positive integer
+// y <= 2^20, with iy = 1 / y rounded.  q = x * iy lies within q * 2^-22 of the rounded quotient (two
+// roundings of 2^-24 relative, plus half an ulp of the quotient), so the two have the same floor
+// unless q lies within that distance of an integer; there the division decides.  Lanes near a
+// multiple of y are rare, so most waves skip the IEEE division.
+CCCS_HD float floor_quot(float x, float y, float iy) {
+    const float q = x * iy;
+    const float fq = floorf(q);
+    const float m = q * 4.76837158e-7f;  // 2^-21: twice the bound
+    if (q - fq > m && (fq + 1.f) - q > m) return fq;
+    return floorf(x / y);
+}
+
+// fmod_pos(x, y) given the floor n of its quotient (the same n fmod_pos computes).
+CCCS_HD float fmod_pos_n(float x, float y, float n) {
+    float r = fmaf(-n, y, x);
+    if (r < 0.f) r += y;
+    if (r >= y) r -= y;
+    return r;
+}
+
 // Pillar mapping of one REV sample (cudaBeamletRaytrace, nvbbRayConvolve_device.cu:33-68):
 // false for a wall, a sample outside the packed grid or past the last pillar; otherwise the
 // beamlet (pidx, batch-local) and the pillar-local coordinate inp (already shifted by -0.5).
-CCCS_HD bool bl_locate(const BlArgs& a, const BlConeDev& c, const BatchDev& B, int tX, int tY, int tZ, int& pidx,
-                       v3& inp) {
+// ipd holds the reciprocals of the pillar dimensions; the pillar index and the in-pillar
+// coordinate are those of the IEEE quotients pb / pd (floor_quot).
+CCCS_HD bool bl_locate(const BlArgs& a, const BlConeDev& c, const BatchDev& B, v3 ipd, int tX, int tY, int tZ,
+                       int& pidx, v3& inp) {
     const v3 pb = rev_to_packed(c, a.rrv, tX, tY, tZ);
     const v3 gdf = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]);
     if (pb.x < 0.f || pb.y < 0.f || pb.z < 0.f || pb.x >= gdf.x || pb.y >= gdf.y || pb.z >= gdf.z) return false;
     const v3 pdf = mk3((float)B.pd[0], (float)B.pd[1], (float)B.pd[2]);
-    const int pix = f2i_rd(pb.x / pdf.x), piy = f2i_rd(pb.z / pdf.z);
-    pidx = pix + piy * B.npx;
+    const float nx = floor_quot(pb.x, pdf.x, ipd.x), ny = floor_quot(pb.y, pdf.y, ipd.y),
+                nz = floor_quot(pb.z, pdf.z, ipd.z);
+    pidx = (int)nx + (int)nz * B.npx;
     if (pidx >= B.n) return false;
-    inp = mk3(fmod_pos(pb.x, pdf.x), fmod_pos(pb.y, pdf.y), fmod_pos(pb.z, pdf.z));
+    inp = mk3(fmod_pos_n(pb.x, pdf.x, nx), fmod_pos_n(pb.y, pdf.y, ny), fmod_pos_n(pb.z, pdf.z, nz));
     if (inp.x < (float)kWall || inp.y < (float)kWall || inp.z < (float)kWall) return false;
     inp = inp - 0.5f;
     return true;
@@ -306,6 +330,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     const v3 dl = mk3(c.kinv_m.m[1], c.kinv_m.m[4], c.kinv_m.m[7]);
     const float pdx = (float)B.pd[0], pdz = (float)B.pd[2], ipdx = 1.f / pdx, ipdz = 1.f / pdz;
     const float cull = B.fw + 0.05f;
+    const v3 ipd = mk3(ipdx, 1.f / (float)B.pd[1], ipdz);
     const bool track = B.n <= (int)kPillarBits;
     for (int k = tid; k < (int)(kPillarBits / 32); k += 64) s_pil[k] = 0u;
     // pass A.1: affine cull of every sample, candidates compacted into s_list
@@ -340,7 +365,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
             const int j = s_list[r + tid];
             int pidx;
             v3 inp;
-            if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
+            if (bl_locate(a, c, B, ipd, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
                 fabsf(inp.z - B.fc[1]) <= B.fw) {
                 // point_fluence(...) > 0 without its map load: a job beamlet's own cell has a
                 // positive weight (plan_beam keeps only those), so the cell test decides alone
@@ -453,7 +478,7 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         PillarRec bl;
         if (r + tid < nex) {
             j = s_list[r + tid];
-            if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) &&
+            if (bl_locate(a, c, B, ipd, i0 + j, tY, tZ, pidx, inp) &&
                 (!track || ((s_pil[pidx >> 5] >> (pidx & 31)) & 1u))) {
                 ok = true;
                 const uint32_t slot = track ? (uint32_t)s_rank[pidx >> 5] +
@@ -1057,7 +1082,8 @@ __global__ void __launch_bounds__(256) plain_ray_kernel(BlArgs a, float* __restr
     float d = -1e-12f, t = 0.f;
     int pidx;
     v3 inp;
-    if (bl_locate(a, c, B, tX, tY, tZ, pidx, inp)) {
+    const v3 ipd = mk3(1.f / (float)B.pd[0], 1.f / (float)B.pd[1], 1.f / (float)B.pd[2]);
+    if (bl_locate(a, c, B, ipd, tX, tY, tZ, pidx, inp)) {
         const PillarRec bl = pillar_rec(a.beamlets[B.beamlet0 + pidx]);
         t = point_terma(bl_terma_point(a, B, bl, inp), B.fmap, B.beam, a.spec, a.vd, a.dens, a.bhc, bl.fidx, nullptr);
         d = bl_density(a, B, bl, inp);

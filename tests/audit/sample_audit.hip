@@ -5,7 +5,8 @@
 // line's stored interval and store offset with the interval kernel's own function, and replays
 // the sample kernel's passes A.1-B.2 line by line on the CPU with the kernel's own helpers
 // (bl_locate, rev_to_packed, bl_terma_point, fluence_cell; point_fluence checks the cell test).  Every index the kernel forms is
-// checked against the bound of the array it addresses:
+// checked against the bound of the array it addresses (and bl_locate, whose quotients skip the
+// IEEE division away from multiples, against the division form of the mapping on every call):
 //   LDS:    s_fp / s_list (< 1024 and < the line's stored length), s_pil / s_rank words (< 128),
 //           staged pillar slots (< kStagePillars), the affine pillar index read in pass B.1;
 //   global: the beamlet record (< the job's beamlets), REV store and half-step offsets
@@ -17,9 +18,11 @@
 // exits 1 on any violation.
 #include "../../endtoend_amd/csrc/beamlet.hip"
 
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <random>
 #include <sstream>
 
 namespace {
@@ -27,14 +30,14 @@ namespace {
 struct Audit {
     uint64_t lines = 0, nonempty = 0, cand_lines = 0, foot_lines = 0, exact_lines = 0;
     uint64_t samples = 0, cand = 0, foot = 0, exact = 0, requests = 0, shortfall = 0, staged = 0,
-             stage_overflow = 0, violations = 0;
+             stage_overflow = 0, violations = 0, locates = 0;
     uint32_t max_len = 0, max_pil_word = 0, max_stage = 0;
     std::string first;
     void merge(const Audit& o) {
         lines += o.lines; nonempty += o.nonempty; cand_lines += o.cand_lines; foot_lines += o.foot_lines;
         exact_lines += o.exact_lines; samples += o.samples; cand += o.cand; foot += o.foot; exact += o.exact;
         requests += o.requests; shortfall += o.shortfall; staged += o.staged; stage_overflow += o.stage_overflow;
-        violations += o.violations;
+        violations += o.violations; locates += o.locates;
         max_len = std::max(max_len, o.max_len);
         max_pil_word = std::max(max_pil_word, o.max_pil_word);
         max_stage = std::max(max_stage, o.max_stage);
@@ -53,6 +56,23 @@ struct Audit {
             }                                                                                         \
         }                                                                                             \
     } while (0)
+
+// The pillar mapping with IEEE quotients throughout (pb / pd floored, fmod_pos): what bl_locate
+// must equal bit for bit.
+bool locate_div(const BlArgs& a, const BlConeDev& c, const BatchDev& B, int tX, int tY, int tZ, int& pidx, v3& inp) {
+    const v3 pb = rev_to_packed(c, a.rrv, tX, tY, tZ);
+    const v3 gdf = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]);
+    if (pb.x < 0.f || pb.y < 0.f || pb.z < 0.f || pb.x >= gdf.x || pb.y >= gdf.y || pb.z >= gdf.z) return false;
+    const v3 pdf = mk3((float)B.pd[0], (float)B.pd[1], (float)B.pd[2]);
+    pidx = f2i_rd(pb.x / pdf.x) + f2i_rd(pb.z / pdf.z) * B.npx;
+    if (pidx >= B.n) return false;
+    inp = mk3(fmod_pos(pb.x, pdf.x), fmod_pos(pb.y, pdf.y), fmod_pos(pb.z, pdf.z));
+    if (inp.x < (float)kWall || inp.y < (float)kWall || inp.z < (float)kWall) return false;
+    inp = inp - 0.5f;
+    return true;
+}
+
+bool same_bits(float x, float y) { return std::memcmp(&x, &y, sizeof(float)) == 0; }
 
 // Passes A.1-B.2 of bl_sample_kernel for one line, with its bounds.
 void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t nbeamlets, uint64_t line,
@@ -75,6 +95,19 @@ void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t n
     const v3 dl = mk3(c.kinv_m.m[1], c.kinv_m.m[4], c.kinv_m.m[7]);
     const float pdx = (float)B.pd[0], pdz = (float)B.pd[2], ipdx = 1.f / pdx, ipdz = 1.f / pdz;
     const float cull = B.fw + 0.05f;
+    const v3 ipd = mk3(ipdx, 1.f / (float)B.pd[1], ipdz);
+    // the kernel's mapping, checked against the division form
+    auto locate = [&](int tX, int& pidx, v3& inp) {
+        int pd = -1;
+        v3 id = mk3(0.f, 0.f, 0.f);
+        const bool ok = bl_locate(a, c, B, ipd, tX, tY, tZ, pidx, inp);
+        const bool okd = locate_div(a, c, B, tX, tY, tZ, pd, id);
+        A.locates++;
+        AUDIT(ok == okd && (!ok || (pidx == pd && same_bits(inp.x, id.x) && same_bits(inp.y, id.y) &&
+                                    same_bits(inp.z, id.z))),
+              "pillar mapping differs from its division form");
+        return ok;
+    };
     const bool track = B.n <= (int)kPillarBits;
     uint8_t s_fp[kMaxLine];
     short s_list[kMaxLine];
@@ -102,7 +135,7 @@ void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t n
         AUDIT(j >= 0 && j < m, "candidate index outside the line");
         int pidx = -1;
         v3 inp;
-        if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
+        if (locate(i0 + j, pidx, inp) && fabsf(inp.x - B.fc[0]) <= B.fw &&
             fabsf(inp.z - B.fc[1]) <= B.fw) {
             AUDIT(pidx >= 0 && pidx < B.n, "pillar index outside the batch");
             AUDIT((size_t)(B.beamlet0 + pidx) < nbeamlets, "beamlet record beyond the job's beamlets");
@@ -174,7 +207,7 @@ void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t n
         int pidx = -1;
         v3 inp;
         bool h = false;
-        if (bl_locate(a, c, B, i0 + j, tY, tZ, pidx, inp) && (!track || ((s_pil[pidx >> 5] >> (pidx & 31)) & 1u))) {
+        if (locate(i0 + j, pidx, inp) && (!track || ((s_pil[pidx >> 5] >> (pidx & 31)) & 1u))) {
             AUDIT(pidx >= 0 && pidx < B.n, "pillar index outside the batch");
             if (track) {
                 const uint32_t slot = s_rank[pidx >> 5] + (uint32_t)__builtin_popcount(s_pil[pidx >> 5] & ((1u << (pidx & 31)) - 1u));
@@ -192,6 +225,27 @@ void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t n
     A.requests += qdone;
     A.shortfall += nfoot - std::min(qdone, nfoot);
     if (nhi >= nlo) AUDIT(nlo >= 0 && nhi < m, "trimmed interval outside the stored one");
+}
+
+// floor_quot against the floor of the IEEE quotient: every divisor 1..256 (pillar dimensions are
+// small integers), 13 consecutive floats around each of the first 4096 multiples of it, and random
+// dividends below 2^14 times it.
+void check_floor_quot(uint64_t& checked, uint64_t& bad) {
+    std::mt19937_64 g(20261202);
+    for (int yi = 1; yi <= 256; yi++) {
+        const float y = (float)yi, iy = 1.f / y;
+        auto one = [&](float x) {
+            checked++;
+            if (!same_bits(floor_quot(x, y, iy), floorf(x / y))) bad++;
+        };
+        for (int k = 0; k <= 4096; k++) {
+            float x = (float)k * y;
+            for (int t = 0; t < 6 && x > 0.f; t++) x = std::nextafter(x, 0.f);
+            for (int t = 0; t < 13; t++, x = std::nextafter(x, 1e30f)) one(x);
+        }
+        std::uniform_real_distribution<float> U(0.f, 16384.f * y);
+        for (int t = 0; t < 20000; t++) one(U(g));
+    }
 }
 
 std::vector<float> read_raw(const std::string& path, size_t n) {
@@ -307,11 +361,16 @@ int main(int argc, char** argv) {
             total.merge(A);
         }
     }
+    uint64_t fq_checked = 0, fq_bad = 0;
+    check_floor_quot(fq_checked, fq_bad);
+    total.violations += fq_bad;
+    if (fq_bad && total.first.empty()) total.first = "floor_quot differs from the IEEE quotient's floor";
     std::printf("{\"beams\": %zu, \"batches\": %d, \"beamlets\": %zu, \"directions\": %d, \"lines\": %llu, "
                 "\"nonempty\": %llu, \"candidate_lines\": %llu, \"footprint_lines\": %llu, \"exact_lines\": %llu, "
                 "\"samples\": %llu, \"candidates\": %llu, \"footprint\": %llu, \"exact\": %llu, \"requests\": %llu, "
                 "\"shortfall\": %llu, \"staged\": %llu, \"stage_overflow_lines\": %llu, \"max_len\": %u, "
-                "\"max_pillar_word\": %u, \"max_footprint_pillars\": %u, \"violations\": %llu, \"first\": \"%s\"}\n",
+                "\"max_pillar_word\": %u, \"max_footprint_pillars\": %u, \"locates\": %llu, "
+                "\"floor_quot_checked\": %llu, \"floor_quot_mismatch\": %llu, \"violations\": %llu, \"first\": \"%s\"}\n",
                 plans.size(), nb, J.beamlets.size(), ctx.nrays, (unsigned long long)total.lines,
                 (unsigned long long)total.nonempty, (unsigned long long)total.cand_lines,
                 (unsigned long long)total.foot_lines, (unsigned long long)total.exact_lines,
@@ -319,6 +378,7 @@ int main(int argc, char** argv) {
                 (unsigned long long)total.exact, (unsigned long long)total.requests,
                 (unsigned long long)total.shortfall, (unsigned long long)total.staged,
                 (unsigned long long)total.stage_overflow, total.max_len, total.max_pil_word, total.max_stage,
+                (unsigned long long)total.locates, (unsigned long long)fq_checked, (unsigned long long)fq_bad,
                 (unsigned long long)total.violations, total.first.c_str());
     return total.violations ? 1 : 0;
 }
