@@ -117,8 +117,8 @@ struct Schedule {
     int bl_job_gb = 12;
     // the beamlet walk sizes its LDS for the longest line of its launch; it runs one launch per
     // length class: lines up to this many stored samples, up to twice that, ..., and the rest
-    // (0: one launch)
-    int bl_walk_split = 256;
+    // (0: one launch).  With four directions in flight, 128 beat 96, 160 and 256 (1.2 % on the step)
+    int bl_walk_split = 128;
     // cone directions of the beamlet path in flight at once (1 .. kRaySets): direction r runs on
     // stream r mod bl_streams, so consecutive directions' sample, Siddon and walk overlap; the
     // gathers stay in direction order (measured on the 50-beam job: 1 -> 10.8k, 2 -> 12.3k,
