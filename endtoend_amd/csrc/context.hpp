@@ -107,10 +107,10 @@ struct Schedule {
     // threads per line (64, 128 or 256) of each class: long lines spread over more waves
     int cone_class_threads[kConeClasses] = {64, 64, 128, 128};
     // waves of the beamlet path's persistent per-line kernels (each takes a stride of lines).  The
-    // sample stage holds 2 waves per SIMD: with several cone directions in flight, the other
-    // streams' Siddon, walk and gather waves run beside it (its own 4-wave occupancy would take
+    // sample stage holds 3 waves per SIMD: with several cone directions in flight, the other
+    // streams' Siddon, walk and gather waves run beside it (its own 5-wave occupancy would take
     // every register of the SIMD)
-    int bl_sample_waves = 2048;
+    int bl_sample_waves = 3072;
     int bl_walk_waves = 32768;
     // device memory of one beamlet job [GiB]: batches join a job while its packed grids, output
     // bounds and per-direction buffers fit; fewer, larger jobs drain the direction pipeline less often

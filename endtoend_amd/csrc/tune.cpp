@@ -108,7 +108,7 @@ extern "C" int cccs_autotune(cccs_ctx* ctx, int kind, const cccs_beam* beams, in
                  {"cone_class_threads3", {64, 128, 256}},
                  {"pvcs_pass_cones", {0, 2, 4, 8, 48}}};
     } else {
-        knobs = {{"bl_sample_waves", {1024, 2048, 4096, 65536, 262144}},
+        knobs = {{"bl_sample_waves", {1024, 2048, 3072, 4096, 65536, 262144}},
                  {"bl_walk_waves", {16384, 32768, 65536}},
                  {"bl_walk_split", {0, 128, 256, 384}},
                  {"bl_streams", {1, 2, 3, 4}},
