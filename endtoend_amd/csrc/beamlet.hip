@@ -78,6 +78,7 @@ struct BatchDev {
     int tiles[3];          // gather tiles along pY (16), pZ (4), pX (4)
     float fc[2];           // beamlet axis in pillar sample coordinates (x, z)
     float fw;              // bound on the footprint half-width [samples]: farther samples get no TERMA
+    float cb[8];           // footprint bounds linear in depth (footprint_cull): x lo, x hi, z lo, z hi
 };
 
 // A footprint sample whose TERMA needs a Siddon march.
@@ -324,7 +325,8 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
     if (tid == 0) stored += (uint32_t)m;  // summed over the wave's lanes by wave_count
     // affine form of the packed coordinate along the line (matrix rotation): used only to skip
     // samples that are far from every footprint or inside pillars without footprint samples;
-    // the exact mapping decides every other sample
+    // the exact mapping decides every other sample.  Pass A.1 keeps a sample only inside the
+    // footprint half-width fw and inside the batch's depth-linear footprint bounds (cb).
     const v3 q0 = mk3(fmaf(c.minc.x, a.rrv.x, (float)tZ), c.minc.y * a.rrv.y, fmaf(c.minc.z, a.rrv.z, (float)tY));
     const v3 p0 = c.kinv_m(q0) + 0.5f;
     const v3 dl = mk3(c.kinv_m.m[1], c.kinv_m.m[4], c.kinv_m.m[7]);
@@ -340,9 +342,10 @@ __global__ void __launch_bounds__(64) bl_sample_kernel(BlArgs a, float* __restri
         bool cand = false;
         if (j < m) {
             const float t = (float)(i0 + j);
-            const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z;
+            const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z, py = p0.y + t * dl.y;
             const float xl = px - floorf(px * ipdx) * pdx, zl = pz - floorf(pz * ipdz) * pdz;
-            cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull;
+            cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull && xl >= B.cb[0] + B.cb[1] * py &&
+                   xl <= B.cb[2] + B.cb[3] * py && zl >= B.cb[4] + B.cb[5] * py && zl <= B.cb[6] + B.cb[7] * py;
             s_fp[j] = 0;
         }
         const uint64_t mask = __ballot(cand);
@@ -1323,6 +1326,66 @@ struct BatchRef {
     int n;
 };
 
+// Linear-in-depth bounds of a batch's footprint in the pillar frame, for the sample stage's cull.
+// A sample is a footprint sample when its TERMA point lies in its beamlet's fluence cell, i.e.
+// inside the pyramid from the source through the cell's corners at the isocentre plane
+// (fluence_cell projects centrally from the source).  The pillar frame is a rotation of PVCS
+// scaled by 1/rvs (isotropic spacing), so the pyramid stays a pyramid there, and its section at
+// pillar depth y has the four corner rays' points as corners, whose x and z are linear in y.  The
+// lowest corner x over the batch's beamlets is a minimum of linear functions (concave in y), the
+// highest a maximum (convex), so the chords between the depths -0.5 and pd[1] - 0.5 bound them
+// from below and above.  (A point on the beam's central axis has no defined cell and takes cell
+// 0; the beamlet of cell 0 therefore also counts the axis line.)  out[0..3]: x lower and upper
+// bounds of the local packed coordinate (pillar frame + 0.5) as c + s * py, py the packed depth
+// coordinate, as {c_lo, s_lo, c_hi, s_hi}; out[4..7] the same for z; widened by `margin` samples.
+void footprint_cull(const BatchDev& B, const BeamletDev* bl, v3 rvs, float margin, float out[8]) {
+    const BeamDev& bm = B.beam;
+    const double y0 = -0.5, y1 = (double)B.pd[1] - 0.5;
+    double lo[2][2] = {{1e300, 1e300}, {1e300, 1e300}}, hi[2][2] = {{-1e300, -1e300}, {-1e300, -1e300}};  // [x|z][end]
+    for (int q = 0; q < B.n; q++) {
+        const BeamletDev& d = bl[q];
+        // pillar frame of a PVCS point: fwd_m (p - pst) / rvs
+        auto frame = [&](double px, double py, double pz, double f[3]) {
+            const double r[3] = {px - d.pst.x, py - d.pst.y, pz - d.pst.z};
+            for (int i = 0; i < 3; i++)
+                f[i] = ((double)d.fwd_m.m[3 * i] * r[0] + (double)d.fwd_m.m[3 * i + 1] * r[1] +
+                        (double)d.fwd_m.m[3 * i + 2] * r[2]) / (double)rvs.x;
+        };
+        double S[3];
+        frame(bm.src.x, bm.src.y, bm.src.z, S);
+        const int ix = d.fidx % bm.fmx, iz = d.fidx / bm.fmx;
+        const double cx0 = (double)bm.src.x + (double)bm.iso_dist * bm.dir_n.x,
+                     cy0 = (double)bm.src.y + (double)bm.iso_dist * bm.dir_n.y,
+                     cz0 = (double)bm.src.z + (double)bm.iso_dist * bm.dir_n.z;
+        const int ncorner = d.fidx == 0 ? 5 : 4;
+        for (int k = 0; k < ncorner; k++) {
+            v3 lat = mk3(0.f, 0.f, 0.f);
+            if (k < 4) {
+                const float ax = (float)(ix + (k & 1)) - 0.5f * (float)bm.fmx;
+                const float az = (float)(iz + (k >> 1)) - 0.5f * (float)bm.fmy;
+                lat = rot2(B.beam_inv, mk3(ax * bm.bsx, 0.f, az * bm.bsy));  // BEV -> PVCS
+            }
+            double C[3];
+            frame(cx0 + lat.x, cy0 + lat.y, cz0 + lat.z, C);
+            for (int e = 0; e < 2; e++) {
+                const double y = e == 0 ? y0 : y1;
+                const double u = (y - S[1]) / (C[1] - S[1]);
+                const double x = S[0] + u * (C[0] - S[0]), z = S[2] + u * (C[2] - S[2]);
+                lo[0][e] = std::min(lo[0][e], x); hi[0][e] = std::max(hi[0][e], x);
+                lo[1][e] = std::min(lo[1][e], z); hi[1][e] = std::max(hi[1][e], z);
+            }
+        }
+    }
+    // local coordinate = frame + 0.5, and depth y = py - 0.5: bound(py) = v(y0) + 0.5 + slope * py
+    for (int a = 0; a < 2; a++) {
+        const double sl = (lo[a][1] - lo[a][0]) / (y1 - y0), sh = (hi[a][1] - hi[a][0]) / (y1 - y0);
+        out[4 * a + 0] = (float)(lo[a][0] + 0.5 - margin);
+        out[4 * a + 1] = (float)sl;
+        out[4 * a + 2] = (float)(hi[a][0] + 0.5 + margin);
+        out[4 * a + 3] = (float)sh;
+    }
+}
+
 // Device tables and sizes of one job.
 struct Job {
     std::vector<BatchDev> batches;
@@ -1437,6 +1500,12 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
             J.rows += (uint64_t)d.boxn[0] * d.boxn[1];
             J.pvox += (uint64_t)P.pd[0] * P.pd[1] * P.pd[2];
             J.beamlets.push_back(d);
+        }
+        if (B.fw < 1e29f) {
+            footprint_cull(B, J.beamlets.data() + B.beamlet0, rvs, 0.05f, B.cb);
+        } else {  // anisotropic spacing: no bound
+            const float nb_[8] = {-1e30f, 0.f, 1e30f, 0.f, -1e30f, 0.f, 1e30f, 0.f};
+            std::copy(nb_, nb_ + 8, B.cb);
         }
         const v3 gsize = mk3((float)B.gd[0], (float)B.gd[1], (float)B.gd[2]) * rvs;
         for (int r = 0; r < nrays; r++) {

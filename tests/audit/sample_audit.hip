@@ -12,6 +12,9 @@
 //   global: the beamlet record (< the job's beamlets), REV store and half-step offsets
 //           (< the direction's store size), the trimmed interval (inside the stored one);
 //   queue:  requests written in pass B.2 never exceed the line's reservation from pass A.2.
+// The pass A.1 cull is checked to keep every footprint sample: on every third audited line, each
+// stored sample is classified exactly (mapping, half-width, fluence cell) whether or not the cull
+// keeps it, and a footprint sample the cull drops is a violation.
 //
 //   sample_audit <case dir> [line stride]
 // The case directory is the one tests/shim_case.py writes.  Prints one JSON line of counts and
@@ -30,14 +33,14 @@ namespace {
 struct Audit {
     uint64_t lines = 0, nonempty = 0, cand_lines = 0, foot_lines = 0, exact_lines = 0;
     uint64_t samples = 0, cand = 0, foot = 0, exact = 0, requests = 0, shortfall = 0, staged = 0,
-             stage_overflow = 0, violations = 0, locates = 0;
+             stage_overflow = 0, violations = 0, locates = 0, cull_checked = 0;
     uint32_t max_len = 0, max_pil_word = 0, max_stage = 0;
     std::string first;
     void merge(const Audit& o) {
         lines += o.lines; nonempty += o.nonempty; cand_lines += o.cand_lines; foot_lines += o.foot_lines;
         exact_lines += o.exact_lines; samples += o.samples; cand += o.cand; foot += o.foot; exact += o.exact;
         requests += o.requests; shortfall += o.shortfall; staged += o.staged; stage_overflow += o.stage_overflow;
-        violations += o.violations; locates += o.locates;
+        violations += o.violations; locates += o.locates; cull_checked += o.cull_checked;
         max_len = std::max(max_len, o.max_len);
         max_pil_word = std::max(max_pil_word, o.max_pil_word);
         max_stage = std::max(max_stage, o.max_stage);
@@ -116,10 +119,24 @@ void audit_line(const BlArgs& a, const BlConeDev& c, const BatchDev& B, size_t n
     int ncand = 0;
     for (int j = 0; j < m; j++) {
         const float t = (float)(i0 + j);
-        const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z;
+        const float px = p0.x + t * dl.x, pz = p0.z + t * dl.z, py = p0.y + t * dl.y;
         const float xl = px - floorf(px * ipdx) * pdx, zl = pz - floorf(pz * ipdz) * pdz;
-        const bool cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull;
+        const bool cand = fabsf(xl - B.fc[0]) <= cull && fabsf(zl - B.fc[1]) <= cull &&
+                          xl >= B.cb[0] + B.cb[1] * py && xl <= B.cb[2] + B.cb[3] * py &&
+                          zl >= B.cb[4] + B.cb[5] * py && zl <= B.cb[6] + B.cb[7] * py;
         s_fp[j] = 0;
+        if (line % 3 == 0) {  // every stored sample classified exactly: the cull must keep each footprint sample
+            int pe = -1;
+            v3 ie;
+            if (bl_locate(a, c, B, ipd, i0 + j, tY, tZ, pe, ie) && fabsf(ie.x - B.fc[0]) <= B.fw &&
+                fabsf(ie.z - B.fc[1]) <= B.fw && pe >= 0 && (size_t)(B.beamlet0 + pe) < nbeamlets) {
+                const PillarRec be = pillar_rec(a.beamlets[B.beamlet0 + pe]);
+                if (fluence_cell(bl_terma_point(a, B, be, ie), B.beam) == be.fidx) {
+                    A.cull_checked++;
+                    AUDIT(cand, "the pass A.1 cull drops a footprint sample");
+                }
+            }
+        }
         if (cand) {
             AUDIT(ncand < kMaxLine, "candidate list beyond 1024");
             s_list[ncand++] = (short)j;
@@ -369,7 +386,7 @@ int main(int argc, char** argv) {
                 "\"nonempty\": %llu, \"candidate_lines\": %llu, \"footprint_lines\": %llu, \"exact_lines\": %llu, "
                 "\"samples\": %llu, \"candidates\": %llu, \"footprint\": %llu, \"exact\": %llu, \"requests\": %llu, "
                 "\"shortfall\": %llu, \"staged\": %llu, \"stage_overflow_lines\": %llu, \"max_len\": %u, "
-                "\"max_pillar_word\": %u, \"max_footprint_pillars\": %u, \"locates\": %llu, "
+                "\"max_pillar_word\": %u, \"max_footprint_pillars\": %u, \"locates\": %llu, \"cull_checked\": %llu, "
                 "\"floor_quot_checked\": %llu, \"floor_quot_mismatch\": %llu, \"violations\": %llu, \"first\": \"%s\"}\n",
                 plans.size(), nb, J.beamlets.size(), ctx.nrays, (unsigned long long)total.lines,
                 (unsigned long long)total.nonempty, (unsigned long long)total.cand_lines,
@@ -378,7 +395,8 @@ int main(int argc, char** argv) {
                 (unsigned long long)total.exact, (unsigned long long)total.requests,
                 (unsigned long long)total.shortfall, (unsigned long long)total.staged,
                 (unsigned long long)total.stage_overflow, total.max_len, total.max_pil_word, total.max_stage,
-                (unsigned long long)total.locates, (unsigned long long)fq_checked, (unsigned long long)fq_bad,
+                (unsigned long long)total.locates, (unsigned long long)total.cull_checked,
+                (unsigned long long)fq_checked, (unsigned long long)fq_bad,
                 (unsigned long long)total.violations, total.first.c_str());
     return total.violations ? 1 : 0;
 }

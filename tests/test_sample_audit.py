@@ -4,8 +4,11 @@ host with the kernel's own helpers, checking every LDS, queue, beamlet-record an
 against its bound, and that no line queues more TERMA requests than it reserved.  The pillar
 mapping, whose quotients skip the IEEE division away from multiples, is compared bit for bit with
 its division form on every call, and that quotient floor with the IEEE one on an adversarial
-sweep.  Run on the 50-beam batched workload (every 13th REV line of every direction) and on two
-beams of the 4pi precompute (2.5 mm beamlets, 96 cones, every 5th line)."""
+sweep.  The pass A.1 cull (footprint half-width and the depth-linear bounds of footprint_cull)
+must keep every footprint sample: every sample of every third audited line is classified exactly,
+and each footprint sample the cull drops is a violation.  Run on the 50-beam batched workload
+(every 13th REV line of every direction) and on two beams of the 4pi precompute (2.5 mm
+beamlets, 96 cones, every 5th line)."""
 import json
 import os
 import subprocess
@@ -33,9 +36,11 @@ def test_batched_workload_indices_stay_in_bounds(tmp_path):
     assert out["max_len"] <= 1024 and out["max_pillar_word"] < 128 and out["max_footprint_pillars"] <= 64
     assert out["shortfall"] == 0
     assert out["locates"] > 100000 and out["floor_quot_checked"] > 10000000 and out["floor_quot_mismatch"] == 0
+    assert out["cull_checked"] > 1000000
 
 
 def test_4pi_sample_indices_stay_in_bounds(tmp_path):
     out = _audit(tmp_path, W.cfg5(indices=[0, 581]), 5)
     assert out["violations"] == 0 and out["footprint_lines"] > 1000 and out["directions"] == 48
     assert out["locates"] > 100000 and out["floor_quot_mismatch"] == 0
+    assert out["cull_checked"] > 50000
