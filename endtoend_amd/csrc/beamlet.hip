@@ -79,6 +79,7 @@ struct BatchDev {
     float fc[2];           // beamlet axis in pillar sample coordinates (x, z)
     float fw;              // bound on the footprint half-width [samples]: farther samples get no TERMA
     float cb[8];           // footprint bounds linear in depth (footprint_cull): x lo, x hi, z lo, z hi
+    float gw;              // largest footprint distance from the axis over the grid's depth (<= fw)
 };
 
 // A footprint sample whose TERMA needs a Siddon march.
@@ -734,15 +735,16 @@ __global__ void __launch_bounds__(64) bl_walk_kernel(BlArgs a, const float* __re
 }
 
 // Only REV lines with footprint samples in a pillar carry dose in it, and those lines pass within
-// sqrt(2)*fw of that pillar's beamlet axis (footprint samples satisfy |x-fc|,|z-fc| <= fw).  The
-// four lines a voxel reads lie within sqrt(2) of the line through it, so if that line passes
-// farther than sqrt(2)*(fw + 1) + 0.5 from the axes of the voxel's pillar and of the neighbouring
-// pillars it touches, the fetch is zero and is skipped.  Depends on pX and pZ only.
+// sqrt(2)*gw of that pillar's beamlet axis (footprint samples satisfy |x-fc|,|z-fc| <= gw, the
+// farthest reach of the footprint bounds, itself <= fw).  The four lines a voxel reads lie within
+// sqrt(2) of the line through it, so if that line passes farther than sqrt(2)*(gw + 1) + 0.5 from
+// the axes of the voxel's pillar and of the neighbouring pillars it touches, the fetch is zero and
+// is skipped.  Depends on pX and pZ only.
 __device__ __forceinline__ bool bl_gather_hit(const BatchDev& B, const BlConeDev& c, int pX, int pZ) {
     if (!(B.fw < 1e29f)) return true;
     const float wx = c.kinv_m.m[1], wz = c.kinv_m.m[7];
     const float wn = sqrtf(wx * wx + wz * wz);
-    const float bound = 1.41421356f * (B.fw + 1.f) + 0.5f;
+    const float bound = 1.41421356f * (B.gw + 1.f) + 0.5f;
     const int pix = pX / B.pd[0], piy = pZ / B.pd[2];
     const int rx = pX - pix * B.pd[0], rz = pZ - piy * B.pd[2];
     const int sx = rx < 2 ? -1 : (rx >= B.pd[0] - 2 ? 1 : 0);
@@ -1501,8 +1503,20 @@ int build_job(cccs_ctx* ctx, const std::vector<BeamPlan>& plans, const std::vect
             J.pvox += (uint64_t)P.pd[0] * P.pd[1] * P.pd[2];
             J.beamlets.push_back(d);
         }
+        B.gw = B.fw;
         if (B.fw < 1e29f) {
             footprint_cull(B, J.beamlets.data() + B.beamlet0, rvs, 0.05f, B.cb);
+            // the bounds' farthest reach from the axis (local frame: axis at fc + 0.5) at either end
+            // of the grid's depth range; the bounds are linear, so the ends hold the maximum
+            for (int e = 0; e < 2; e++) {
+                const float py = e == 0 ? 0.f : (float)B.gd[1];
+                for (int ax = 0; ax < 2; ax++) {
+                    const float ctr = B.fc[ax] + 0.5f;
+                    const float lo = B.cb[4 * ax] + B.cb[4 * ax + 1] * py, hi = B.cb[4 * ax + 2] + B.cb[4 * ax + 3] * py;
+                    B.gw = e == 0 && ax == 0 ? std::max(ctr - lo, hi - ctr) : std::max(B.gw, std::max(ctr - lo, hi - ctr));
+                }
+            }
+            B.gw = std::min(B.gw, B.fw);
         } else {  // anisotropic spacing: no bound
             const float nb_[8] = {-1e30f, 0.f, 1e30f, 0.f, -1e30f, 0.f, 1e30f, 0.f};
             std::copy(nb_, nb_ + 8, B.cb);
